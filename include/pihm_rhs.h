@@ -1,0 +1,155 @@
+/*
+ * pihm_rhs.h - C ABI of the MI355X MM-PIHM hydrology right-hand side.
+ *
+ * The entry point PIHM_ODE_Hip has the CVODE CVRhsFn signature
+ * (reference: cvode/include/cvode/cvode.h:136-150) and replaces the
+ * reference's `int ODE(realtype t, N_Vector CV_Y, N_Vector CV_Ydot,
+ * void *pihm_data)` (src/ode.c:4-399), registered by
+ * `CVodeInit(cvode_mem, ODE, 0.0, CV_Y)` (src/ode.c:436).  `user_data` is a
+ * pihm_ctx made by pihm_ctx_create instead of the reference's pihm_struct
+ * (src/ode.c:473).
+ *
+ * Index conventions of the mesh description follow the reference
+ * structures after Initialize() (src/initialize.c:16):
+ *   nabr[3*i+j]   > 0 : 1-based neighbour element across edge j
+ *                 < 0 : -(river segment index, 1-based) (src/init_river.c:24-35)
+ *                 = 0 : domain boundary, condition bc_type[3*i+j]
+ *   river down    > 0 : 1-based downstream segment; <= 0 outlet condition
+ *                       DIRICHLET -1, NEUMANN -2, ZERO_DPTH_GRAD -3,
+ *                       CRIT_DPTH -4 (src/include/pihm_const.h:140-143)
+ *   leftele/rightele : 1-based bank elements.
+ * Arrays are in the caller's (original) order.  The state vector is
+ * block-major: SURF(i)=i, UNSAT(i)=i+ne, GW(i)=i+2ne, RIVSTG(r)=r+3ne,
+ * RIVGW(r)=r+3ne+nr (src/include/pihm_func.h:7-11).
+ *
+ * Device vectors given to PIHM_ODE_Hip hold the state in *device order*:
+ * the same blocks, with elements inside each element block renumbered along
+ * a Hilbert curve of the element centroids.  Use pihm_ctx_state_to_device /
+ * pihm_ctx_state_from_device to move a state between the caller's order and
+ * device order; those two maps are exact permutations.
+ */
+#ifndef PIHM_RHS_H
+#define PIHM_RHS_H
+
+#include <stdint.h>
+#include "pihm_nvector.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIHM_KINEMATIC 1
+#define PIHM_DIFF_WAVE 2
+
+/* Return codes of the library (never exits the process). */
+#define PIHM_OK              0
+#define PIHM_ERR_ARG        -1   /* bad argument / inconsistent mesh */
+#define PIHM_ERR_HIP        -2   /* HIP runtime error */
+#define PIHM_ERR_NAN         1   /* NaN in ydot: recoverable per CVRhsFn */
+#define PIHM_ERR_TOO_LARGE  -3   /* does not fit the limits */
+
+/* Element and river description after the reference's Initialize().
+ * Field meanings are those of elem_struct / river_struct
+ * (src/include/elem_struct.h:1557-1618, src/include/river_struct.h:137-167). */
+typedef struct pihm_mesh_desc {
+  int32_t nelem;
+  int32_t nriver;
+  int32_t surf_mode;            /* ctrl.surf_mode: KINEMATIC or DIFF_WAVE */
+  int32_t riv_mode;             /* ctrl.riv_mode */
+  double  stepsize;             /* ctrl.stepsize [s], the dt of Infil */
+
+  /* elements: per-element arrays [nelem], per-edge arrays [3*nelem] */
+  const int32_t *nabr;
+  const int32_t *bc_type;       /* attrib.bc_type */
+  const double *x, *y;          /* topo.x, topo.y (centroid) */
+  const double *area, *zmin, *zmax;
+  const double *edge, *nabrdist, *nabr_x, *nabr_y;   /* [3*nelem] */
+  const double *depth, *ksath, *ksatv, *kinfv, *dinf, *alpha, *beta,
+               *porosity, *dmac, *kmach, *kmacv, *areafv, *areafh;
+  const double *rough;          /* lc.rough */
+  const double *rzd;            /* ps.rzd */
+
+  /* river segments [nriver] */
+  const int32_t *down, *leftele, *rightele;
+  const int32_t *riverbc_type;  /* attrib.riverbc_type */
+  const int32_t *intrpl_ord;    /* shp.intrpl_ord */
+  const double *rarea, *rzmin, *rzmax, *zbed, *node_zmax, *dist_left,
+               *dist_right;     /* river topo */
+  const double *rdepth, *coeff, *length, *width;     /* shp */
+  const double *rrough, *cwr, *rksath, *rksatv, *bedthick, *rporosity; /* matl */
+} pihm_mesh_desc;
+
+/* Per-step inputs read by the RHS (src/is_sm_et.c:127-224,
+ * src/forcing.c:127-175,214-244, src/update.c:46), caller's order.
+ * Any pointer may be NULL to leave that input unchanged. */
+typedef struct pihm_step_inputs {
+  const double *pcpdrp;         /* wf.pcpdrp [nelem], m/s */
+  const double *edir;           /* wf.edir   [nelem], m/s */
+  const double *ett;            /* wf.ett    [nelem], m/s */
+  const double *ws0_surf;       /* ws0.surf  [nelem], m */
+  const double *elem_bc;        /* bc.head / bc.flux union [3*nelem] */
+  const double *river_bc;       /* river bc.head / bc.flux union [nriver] */
+} pihm_step_inputs;
+
+typedef struct pihm_ctx pihm_ctx;
+
+/* Build a context on the current HIP device: validates the mesh, computes
+ * the Hilbert renumbering and uploads the structure-of-arrays mesh.
+ * Per-step inputs start at zero and the carried river-edge overland fluxes
+ * (read by Infil one call later) start at zero, as in src/initialize.c:735-743. */
+int  pihm_ctx_create(const pihm_mesh_desc *mesh, pihm_ctx **out);
+void pihm_ctx_destroy(pihm_ctx *ctx);
+int  pihm_ctx_set_stream(pihm_ctx *ctx, void *hip_stream);
+int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
+int  pihm_ctx_neq(const pihm_ctx *ctx);
+
+/* Exact permutations between caller order and device order. `perm_out`
+ * receives, for each device element position p, the caller's element index. */
+int  pihm_ctx_get_perm(const pihm_ctx *ctx, int32_t *perm_out);
+int  pihm_ctx_state_to_device(pihm_ctx *ctx, const double *host_orig, N_Vector dev);
+int  pihm_ctx_state_from_device(pihm_ctx *ctx, N_Vector dev, double *host_orig);
+
+/* The RHS (CVRhsFn).  Returns 0, or PIHM_ERR_NAN (> 0, recoverable) when a
+ * component of ydot is NaN, where the reference prints and exits
+ * (src/ode.c:401-410), or < 0 on a runtime error.  The NaN check costs one
+ * host synchronisation; pihm_ctx_set_nan_check(ctx, 0) turns it off. */
+int  PIHM_ODE_Hip(realtype t, N_Vector y, N_Vector ydot, void *user_data);
+int  pihm_ctx_set_nan_check(pihm_ctx *ctx, int on);
+
+/* Asynchronous form: launch one RHS evaluation on raw device arrays (device
+ * order, length neq) and return without synchronising.  A NaN raises a
+ * sticky flag that pihm_ctx_poll_nan reads and clears (PIHM_ERR_NAN). */
+int  pihm_rhs_launch(pihm_ctx *ctx, const double *y_dev, double *ydot_dev);
+int  pihm_ctx_poll_nan(pihm_ctx *ctx);
+
+/* Recording mode keeps every flux of the last call for pihm_ctx_get_fluxes
+ * (extra HBM writes; off by default). */
+int  pihm_ctx_set_record(pihm_ctx *ctx, int on);
+
+/* Wait for all work queued on the context's stream. */
+int  pihm_ctx_sync(pihm_ctx *ctx);
+
+/* Measurement: average device time [ms] of the three RHS kernels (neighbour
+ * records + slopes, fluxes, river/bank assembly) over `iters` evaluations at
+ * state y_dev, timed with HIP events on the context's stream. */
+int  pihm_ctx_time_kernels(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters,
+                           double *ms_prep, double *ms_flux, double *ms_assemble);
+
+/* Carried state: the overland flux on river edges written at the end of one
+ * RHS call and read by Infil in the next (src/river_flow.c:159-182 then
+ * src/vert_flow.c:59-62).  Arrays are [3*nelem] in caller order; entries of
+ * non-river edges are written as 0 and ignored on set. */
+int  pihm_ctx_get_river_edge_ovlflow(pihm_ctx *ctx, double *ovl3);
+int  pihm_ctx_set_river_edge_ovlflow(pihm_ctx *ctx, const double *ovl3);
+
+/* Flux read-back of the last RHS call, caller order (for Summary, water
+ * balance and outputs):  ovlflow/subsurf [3*nelem], infil/rechg [nelem],
+ * rivflow [11*nriver] (index r*11+k, k as src/include/pihm_const.h:127-137).
+ * Any pointer may be NULL. */
+int  pihm_ctx_get_fluxes(pihm_ctx *ctx, double *ovlflow, double *subsurf,
+                         double *infil, double *rechg, double *rivflow);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIHM_RHS_H */

@@ -1,0 +1,210 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py, as the checker or the timed CPU baseline; never by the product path.
+Build with ``make -C oracle``.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "pihm-cuda_amd"))
+from pihm_amd import _lib as P  # noqa: E402  (structure definitions only)
+
+dp = P.c_double_p
+ip = P.c_int32_p
+_o = None
+
+
+def lib():
+    global _o
+    if _o is not None:
+        return _o
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+    L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    protos = [
+        ("oracle_state_new", C.c_void_p, [C.c_int, C.c_int]),
+        ("oracle_state_free", None, [C.c_void_p]),
+        ("oracle_set_step_inputs", None, [C.c_void_p, C.POINTER(P.StepInputs)]),
+        ("oracle_ode", C.c_int, [C.POINTER(P.MeshDesc), C.c_void_p, C.c_double, dp, dp, dp]),
+        ("oracle_SurfH", C.c_double, [C.c_double]),
+        ("oracle_KrFunc", C.c_double, [C.c_double, C.c_double]),
+        ("oracle_Psi", C.c_double, [C.c_double, C.c_double, C.c_double]),
+        ("oracle_EffKh", C.c_double, [C.c_double] * 6),
+        ("oracle_AvgKv", C.c_double, [C.c_double] * 7),
+        ("oracle_EffKinf", C.c_double, [C.c_double] * 8),
+        ("oracle_DhByDl", C.c_double, [dp, dp, dp]),
+        ("oracle_OverLandFlow", C.c_double, [C.c_double] * 5),
+        ("oracle_RiverCroSectArea", C.c_double, [C.c_int, C.c_double, C.c_double]),
+        ("oracle_RiverPerim", C.c_double, [C.c_int, C.c_double, C.c_double]),
+        ("oracle_RiverEqWid", C.c_double, [C.c_int, C.c_double, C.c_double]),
+        ("oracle_OvlFlowElemToRiver_raw", C.c_double, [C.c_double] * 7),
+        ("oracle_init_topo", None, [C.c_int, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
+        ("oracle_init_river_tags", None, [C.c_int, C.c_int, ip, ip, ip]),
+        ("oracle_init_surfl", None, [C.c_int, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
+        ("oracle_init_river_topo", None, [C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp,
+                                          dp, dp, dp, dp, dp, dp, dp, dp]),
+        ("oracle_nv_new", C.c_void_p, [C.c_long, C.c_int]),
+        ("oracle_nv_destroy", None, [C.c_void_p]),
+        ("oracle_nv_data", C.c_void_p, [C.c_void_p]),
+        ("oracle_nv_length", C.c_long, [C.c_void_p]),
+        ("oracle_backend_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.POINTER(P.Forcing), C.c_int]),
+        ("oracle_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
+        ("oracle_backend_free", None, [C.c_void_p]),
+    ]
+    for name, res, args in protos:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _o = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(dp)
+
+
+def step_inputs(pcpdrp=None, edir=None, ett=None, ws0_surf=None, elem_bc=None, river_bc=None, keep=None):
+    s = P.StepInputs()
+    for name, arr in (("pcpdrp", pcpdrp), ("edir", edir), ("ett", ett), ("ws0_surf", ws0_surf),
+                      ("elem_bc", elem_bc), ("river_bc", river_bc)):
+        if arr is not None:
+            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            if keep is not None:
+                keep.append(arr)
+            setattr(s, name, _dptr(arr))
+    return s
+
+
+class OracleRHS:
+    """The reference RHS on the host (original order), with its carried state."""
+
+    def __init__(self, mesh):
+        self.L = lib()
+        self.mesh = mesh
+        self.desc = mesh.desc()
+        self.st = self.L.oracle_state_new(mesh.nelem, mesh.nriver)
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.L.oracle_state_free(self.st)
+        except Exception:
+            pass
+
+    def set_step_inputs(self, **kw):
+        keep = []
+        s = step_inputs(keep=keep, **kw)
+        self.L.oracle_set_step_inputs(self.st, C.byref(s))
+
+    def _field(self, name, n):
+        class _S(C.Structure):
+            pass
+        # the oracle_state layout: two ints then pointers in declaration order
+        names = ["pcpdrp", "edir", "ett", "ws0_surf", "elem_bc", "river_bc", "surf", "unsat", "gw",
+                 "surfh", "edir_surf", "edir_unsat", "edir_gw", "ett_unsat", "ett_gw", "infil",
+                 "rechg", "ovlflow", "subsurf", "dhbydx", "dhbydy", "stage", "rgw", "rivflow"]
+        _S._fields_ = [("nelem", C.c_int), ("nriver", C.c_int)] + [(k, dp) for k in names]
+        s = C.cast(self.st, C.POINTER(_S)).contents
+        ptr = getattr(s, name)
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    def field(self, name):
+        ne, nr = self.mesh.nelem, self.mesh.nriver
+        sizes = {"ovlflow": 3 * ne, "subsurf": 3 * ne, "elem_bc": 3 * ne, "river_bc": nr,
+                 "stage": nr, "rgw": nr, "rivflow": 11 * nr}
+        return self._field(name, sizes.get(name, ne))
+
+    def ode(self, y, t=0.0, with_mag=False):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        dy = np.zeros_like(y)
+        mag = np.zeros_like(y) if with_mag else None
+        rc = self.L.oracle_ode(C.byref(self.desc), self.st, t, _dptr(y), _dptr(dy),
+                               _dptr(mag) if with_mag else None)
+        if with_mag:
+            return rc, dy, mag
+        return rc, dy
+
+
+def init_geometry(mesh):
+    """Oracle restatement of InitTopo/InitRiver/InitSurfL on the mesh's raw topology."""
+    L = lib()
+    ne, nr = mesh.nelem, mesh.nriver
+    node = np.ascontiguousarray(mesh.tri, dtype=np.int32)
+    out = {k: np.zeros(ne) for k in ("area", "x", "y", "zmin", "zmax")}
+    edge = np.zeros(3 * ne)
+    L.oracle_init_topo(ne, node.ctypes.data_as(ip), _dptr(mesh.node_x), _dptr(mesh.node_y),
+                       _dptr(mesh.node_zmin), _dptr(mesh.node_zmax), _dptr(out["area"]),
+                       _dptr(out["x"]), _dptr(out["y"]), _dptr(out["zmin"]), _dptr(out["zmax"]),
+                       _dptr(edge))
+    out["edge"] = edge
+    nabr = np.ascontiguousarray(mesh.nabr_raw, dtype=np.int32).copy()
+    left = np.ascontiguousarray(mesh.rivers["leftele"], dtype=np.int32)
+    right = np.ascontiguousarray(mesh.rivers["rightele"], dtype=np.int32)
+    L.oracle_init_river_tags(ne, nr, nabr.ctypes.data_as(ip), left.ctypes.data_as(ip), right.ctypes.data_as(ip))
+    out["nabr"] = nabr
+    rk = {k: np.zeros(max(nr, 1)) for k in ("rx", "ry", "rzmax", "rzmin", "node_zmax", "dist_left",
+                                              "dist_right", "length")}
+    fn = np.ascontiguousarray(mesh.rivers["fromnode"], dtype=np.int32)
+    tn = np.ascontiguousarray(mesh.rivers["tonode"], dtype=np.int32)
+    L.oracle_init_river_topo(nr, fn.ctypes.data_as(ip), tn.ctypes.data_as(ip), left.ctypes.data_as(ip),
+                             right.ctypes.data_as(ip), _dptr(mesh.node_x), _dptr(mesh.node_y),
+                             _dptr(mesh.node_zmax), _dptr(out["zmax"]), _dptr(out["zmin"]),
+                             _dptr(out["x"]), _dptr(out["y"]), *[_dptr(rk[k]) for k in
+                             ("rx", "ry", "rzmax", "rzmin", "node_zmax", "dist_left", "dist_right",
+                              "length")])
+    for k, v in rk.items():
+        out[k] = v[:nr]
+    nx_, ny_, nd_ = np.zeros(3 * ne), np.zeros(3 * ne), np.zeros(3 * ne)
+    L.oracle_init_surfl(ne, node.ctypes.data_as(ip), nabr.ctypes.data_as(ip), _dptr(mesh.node_x),
+                        _dptr(mesh.node_y), _dptr(out["x"]), _dptr(out["y"]), _dptr(out["area"]),
+                        _dptr(edge), _dptr(rk["rx"]), _dptr(rk["ry"]), _dptr(nx_), _dptr(ny_), _dptr(nd_))
+    out["nabr_x"], out["nabr_y"], out["nabrdist"] = nx_, ny_, nd_
+    return out
+
+
+class OracleRun:
+    """The model-step driver (product integrator) over the oracle RHS on host vectors."""
+
+    def __init__(self, mesh, forcing, nthreads=1):
+        self.L = lib()
+        self.PL = P.lib()
+        self.mesh = mesh
+        self.desc = mesh.desc()
+        self.st = self.L.oracle_state_new(mesh.nelem, mesh.nriver)
+        self.forcing = forcing
+        self.fs = forcing.struct()
+        self.be = self.L.oracle_backend_new(C.byref(self.desc), self.st, C.byref(self.fs), nthreads)
+        self.nthreads = nthreads
+
+    def run(self, y0, cfg):
+        n = self.mesh.neq
+        v = self.L.oracle_nv_new(n, self.nthreads)
+        try:
+            data = np.ctypeslib.as_array(C.cast(self.L.oracle_nv_data(v), dp), shape=(n,))
+            data[:] = y0
+            # initial ws0 = initial surface state
+            ws0 = self._ws0()
+            ws0[:] = y0[:self.mesh.nelem]
+            st = P.RunStats()
+            rc = self.PL.pihm_drive(self.L.oracle_backend_get(self.be), v, C.byref(cfg), C.byref(st))
+            return rc, data.copy(), st
+        finally:
+            self.L.oracle_nv_destroy(v)
+
+    def _ws0(self):
+        o = OracleRHS.__new__(OracleRHS)
+        o.st, o.mesh = self.st, self.mesh
+        return o._field("ws0_surf", self.mesh.nelem)
+
+    def __del__(self):
+        try:
+            self.L.oracle_backend_free(self.be)
+            self.L.oracle_state_free(self.st)
+        except Exception:
+            pass

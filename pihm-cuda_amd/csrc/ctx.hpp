@@ -1,0 +1,113 @@
+// ctx.hpp - device-resident MM-PIHM mesh and RHS state.
+//
+// Layout in HBM (structure of arrays, element arrays in Hilbert order, river
+// arrays in the caller's order):
+//   element statics   18 x double[ne]      area .. rzd
+//   edge statics       4 x double[3*ne]    edge, nabrdist, nabr_x, nabr_y  (index j*ne+p)
+//   edge codes         2 x int32[3*ne]     nb (neighbour code), bc_type
+//   per-step inputs    4 x double[ne] + double[3*ne]
+//   neighbour record   NbrRec[ne]          values a neighbour reads, one 64-byte load
+//   bank records       BankRec[nbank]      partial sums of elements with a river edge
+//   river statics      17 x double[nr] + 6 x int32[nr], upstream CSR
+//   river fluxes       double[11*nr] (rf[k*nr+r]), carried river-edge fluxes
+#pragma once
+#include <cstdint>
+#include <vector>
+#include "../../include/pihm_rhs.h"
+
+namespace pihm {
+
+// Neighbour code nb[j*ne+p]:
+//   >= 0                 device index of the neighbouring element
+//   NB_BOUND             domain boundary (condition in bc_type)
+//   other negative       river edge: -(2*r + side + 1), side 0 = left bank, 1 = right
+constexpr int32_t NB_BOUND = INT32_MIN;
+
+// What a neighbouring element needs from element p in one RHS call.
+struct alignas(64) NbrRec {
+  double surfh;   // ws.surfh
+  double head;    // topo.zmax + ws.surfh
+  double gw;      // clamped ws.gw
+  double gwz;     // ws.gw + topo.zmin
+  double effkh;   // EffKh(soil, gw)
+  double sfmag;   // |grad h| from FrictSlope
+  double rough;   // lc.rough (static)
+  double zmax;    // topo.zmax (static)
+};
+
+// Partial sums of an element that has at least one river edge; finished once
+// the river fluxes are known, in the reference's edge order.
+struct alignas(64) BankRec {
+  double dys0;    // pcpdrp - infil - edir_surf
+  double dyg0;    // rechg - edir_gw - ett_gw
+  double ovl[3];
+  double sub[3];
+};
+
+struct ElemDev {
+  double *area, *zmin, *zmax, *depth, *ksath, *ksatv, *kinfv, *dinf, *alpha,
+      *beta, *porosity, *dmac, *kmach, *kmacv, *areafv, *areafh, *rough, *rzd;
+  double *edge, *nabrdist, *nabr_x, *nabr_y;  // [3*ne]
+  int32_t *nb, *bc_type;                      // [3*ne]
+  int32_t *bank_id;                           // [ne], -1 if no river edge
+  // per-step inputs
+  double *pcpdrp, *edir, *ett, *ws0s, *ebc;   // ebc [3*ne]
+  NbrRec *rec;
+  BankRec *bank;
+  int32_t *bank_elem;                         // [nbank] device element of each bank record
+};
+
+struct RiverDev {
+  int32_t *down;        // >= 0 device index of the downstream segment, else outlet code
+  int32_t *left, *right;  // device element index of the banks
+  int32_t *bctype, *ord;
+  int32_t *up_ptr, *up_idx;  // upstream segments of r, ascending caller index
+  double *area, *zmin, *zmax, *zbed, *node_zmax, *dist_l, *dist_r, *depth, *coeff,
+      *length, *width, *rough, *cwr, *ksath, *ksatv, *bedthick, *porosity;
+  double *bc;           // per-step river bc value
+  double *rf;           // [11*nr], rf[k*nr + r]
+  double *upinit;       // 0.0 + boundary flux of the upstream node
+  double *hid_ovl;      // [2][2*nr] carried river-edge overland flux, double-buffered
+  double *bank_sub;     // [2*nr] river-edge subsurface flux of the current call
+};
+
+struct RecordDev {      // optional flux recording (device order)
+  double *ovl, *sub;    // [3*ne]
+  double *infil, *rechg;
+};
+
+struct RhsArgs {
+  int ne, nr, nbank;
+  int surf_mode, riv_mode;
+  double dt;            // ctrl.stepsize
+  int cur;              // parity of the carried river-edge buffer read this call
+  int record;
+  ElemDev e;
+  RiverDev r;
+  RecordDev rec;
+  const double *y;
+  double *ydot;
+  int *nanflag;
+};
+
+}  // namespace pihm
+
+struct pihm_ctx {
+  int ne, nr, nbank, neq;
+  int surf_mode, riv_mode;
+  double dt;
+  int device;
+  hipStream_t stream;
+  int nan_check;
+  int record;
+  long long ncalls;
+  std::vector<int32_t> perm;     // device position -> caller index
+  std::vector<int32_t> iperm;    // caller index -> device position
+  std::vector<int32_t> nb_host;  // device-order neighbour codes [3*ne]
+  pihm::RhsArgs args;            // pointers filled at creation
+  void *arena;                   // one device allocation for everything
+  size_t arena_bytes;
+  int *nanflag_dev;
+  int *nanflag_host;             // pinned
+  double *stage_host;            // scratch for host-side permutations
+};

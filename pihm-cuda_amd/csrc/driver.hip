@@ -1,0 +1,218 @@
+// driver.hip - MM-PIHM model-step loop and the device backend.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pihm_driver.h"
+#include "common.hpp"
+#include "ctx.hpp"
+#include "nvector_impl.hpp"
+
+using namespace pihm;
+
+extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_config *cfg,
+                          pihm_run_stats *st) {
+  if (!be || !y || !cfg || !be->rhs) return PIHM_ERR_ARG;
+  if (!(cfg->stepsize > 0.0) || !(cfg->etstep > 0.0) || cfg->t_end < cfg->t_start) return PIHM_ERR_ARG;
+  const auto t0w = std::chrono::steady_clock::now();
+  pihm_run_stats local;
+  std::memset(&local, 0, sizeof(local));
+
+  // SetCVodeParam: src/ode.c:413-529
+  void *cv = PihmCVodeCreate(PIHM_CV_BDF, PIHM_CV_NEWTON);
+  if (!cv) return PIHM_ERR_ARG;
+  double maxstep = cfg->stepsize;
+  int flag = PihmCVodeInit(cv, be->rhs, 0.0, y);
+  if (!flag) flag = PihmCVodeSStolerances(cv, cfg->reltol, cfg->abstol);
+  if (!flag) flag = PihmCVodeSetUserData(cv, be->rhs_data);
+  if (!flag) flag = PihmCVodeSetInitStep(cv, cfg->initstep);
+  if (!flag) flag = PihmCVodeSetStabLimDet(cv, 1);
+  if (!flag) flag = PihmCVodeSetMaxStep(cv, maxstep);
+  if (!flag) flag = PihmCVodeSetMaxNumSteps(cv, cfg->max_steps_per_call > 0 ? cfg->max_steps_per_call
+                                                                            : (long)(cfg->stepsize * 10));
+  if (!flag) flag = PihmCVSpgmr(cv, 0, cfg->maxl);
+  if (flag) { PihmCVodeFree(&cv); return flag; }
+
+  long nst0 = 0, ncfn0 = 0, nni0 = 0;
+  const double tstart = cfg->t_start;
+  const long nsteps = (long)std::llround((cfg->t_end - cfg->t_start) / cfg->stepsize);
+  const long et_every = std::max(1L, (long)std::llround(cfg->etstep / cfg->stepsize));
+  for (long k = 0; k < nsteps; k++) {
+    const double t = tstart + k * cfg->stepsize;
+    // ApplyForc every LSM step (src/pihm.c:30-45)
+    if (k % et_every == 0 && be->apply_forcing) {
+      if ((flag = be->apply_forcing(be->user, t))) break;
+    }
+    // SolveCVode: src/ode.c:555-599 (solver time runs from 0 at t_start)
+    const double tout = (double)((k + 1) * cfg->stepsize);
+    double solvert = 0.0;
+    flag = PihmCVodeSetStopTime(cv, tout);
+    if (flag) break;
+    flag = PihmCVode(cv, tout, y, &solvert, PIHM_CV_NORMAL);
+    if (flag < 0) break;
+    // Summary: ws0 <- ws (src/update.c:46)
+    if (be->summary && (flag = be->summary(be->user, y))) break;
+    local.model_steps++;
+    if (be->model_step_end && (flag = be->model_step_end(be->user, tstart + tout, y))) break;
+    // AdjCVodeMaxStep: src/ode.c:601-661
+    long nst, ncfn, nni;
+    PihmCVodeGetNumSteps(cv, &nst);
+    PihmCVodeGetNumNonlinSolvConvFails(cv, &ncfn);
+    PihmCVodeGetNumNonlinSolvIters(cv, &nni);
+    const double dsteps = (double)(nst - nst0);
+    const double nfails = (double)(ncfn - ncfn0) / dsteps;
+    const double niters = (double)(nni - nni0) / dsteps;
+    if (nfails > cfg->nncfn || niters >= cfg->nnimax) maxstep /= cfg->decr;
+    if (nfails == 0.0 && niters <= cfg->nnimin) maxstep *= cfg->incr;
+    maxstep = (maxstep < cfg->stepsize) ? maxstep : cfg->stepsize;
+    maxstep = (maxstep > cfg->stmin) ? maxstep : cfg->stmin;
+    PihmCVodeSetMaxStep(cv, maxstep);
+    nst0 = nst; ncfn0 = ncfn; nni0 = nni;
+  }
+  if (flag > 0) flag = 0;   // CV_TSTOP_RETURN is the normal outcome of each step
+  PihmCVodeGetNumSteps(cv, &local.nst);
+  PihmCVodeGetNumRhsEvals(cv, &local.nfe);
+  PihmCVodeGetNumJtimesRhsEvals(cv, &local.nfes);
+  PihmCVodeGetNumNonlinSolvIters(cv, &local.nni);
+  PihmCVodeGetNumLinIters(cv, &local.nli);
+  PihmCVodeGetNumNonlinSolvConvFails(cv, &local.ncfn);
+  PihmCVodeGetNumErrTestFails(cv, &local.netf);
+  PihmCVodeFree(&cv);
+  local.flag = flag;
+  local.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0w).count();
+  if (st) *st = local;
+  return flag;
+}
+
+// ---------------------------------------------------------------- device backend
+namespace {
+
+__global__ void k_forcing(int ne, const int32_t *station, const double *prcp, const double *edir,
+                          const double *ett, int nrec, double rec_dt, double t, double *pcpdrp,
+                          double *oedir, double *oett) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ne) return;
+  long k = (long)floor(t / rec_dt);
+  if (k < 0) k = 0;
+  if (k > nrec - 2) k = nrec - 2;
+  const double ta = (double)k * rec_dt, tb = (double)(k + 1) * rec_dt;
+  const long s = station[p];
+  const double *P = prcp + s * (long)nrec;
+  // IntrplForc: src/forcing.c:514-519
+  pcpdrp[p] = ((tb - t) * P[k] + (t - ta) * P[k + 1]) / (tb - ta);
+  oedir[p] = ((tb - t) * edir[k] + (t - ta) * edir[k + 1]) / (tb - ta);
+  oett[p] = ((tb - t) * ett[k] + (t - ta) * ett[k + 1]) / (tb - ta);
+}
+
+__global__ void k_copy(long n, const double *src, double *dst) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+struct pihm_gpu_backend {
+  pihm_ctx *ctx;
+  int nrec;
+  double rec_dt;
+  int32_t *station;
+  double *prcp, *edir, *ett;
+};
+
+static int gpu_rhs(realtype t, N_Vector y, N_Vector ydot, void *ud) {
+  return PIHM_ODE_Hip(t, y, ydot, ud);
+}
+
+static int gpu_forcing(void *user, double t) {
+  pihm_gpu_backend *b = (pihm_gpu_backend *)user;
+  pihm_ctx *c = b->ctx;
+  hipLaunchKernelGGL(k_forcing, dim3(div_up(c->ne, kBlock)), dim3(kBlock), 0, c->stream, c->ne,
+                     b->station, b->prcp, b->edir, b->ett, b->nrec, b->rec_dt, t,
+                     c->args.e.pcpdrp, c->args.e.edir, c->args.e.ett);
+  return hipGetLastError() == hipSuccess ? 0 : PIHM_ERR_HIP;
+}
+
+static int gpu_summary(void *user, N_Vector y) {
+  pihm_gpu_backend *b = (pihm_gpu_backend *)user;
+  pihm_ctx *c = b->ctx;
+  // ws0.surf <- y[SURF] (device order on both sides)
+  hipLaunchKernelGGL(k_copy, dim3(div_up(c->ne, kBlock)), dim3(kBlock), 0, c->stream, (long)c->ne,
+                     N_VGetDeviceArrayPointer_Hip(y), c->args.e.ws0s);
+  return hipGetLastError() == hipSuccess ? 0 : PIHM_ERR_HIP;
+}
+
+extern "C" int pihm_gpu_backend_create(pihm_ctx *c, const pihm_forcing *f, pihm_gpu_backend **out) {
+  if (!c || !f || !out || f->nrec < 2 || f->nstation < 1 || !(f->rec_dt > 0.0)) return PIHM_ERR_ARG;
+  if (!f->prcp || !f->edir || !f->ett || !f->station) return PIHM_ERR_ARG;
+  for (int i = 0; i < c->ne; i++)
+    if (f->station[i] < 0 || f->station[i] >= f->nstation) return PIHM_ERR_ARG;
+  pihm_gpu_backend *b = new pihm_gpu_backend();
+  b->ctx = c;
+  b->nrec = f->nrec;
+  b->rec_dt = f->rec_dt;
+  std::vector<int32_t> st(c->ne);
+  for (int p = 0; p < c->ne; p++) st[p] = f->station[c->perm[p]];
+  PIHM_HIP_TRY(hipMalloc(&b->station, c->ne * sizeof(int32_t)));
+  PIHM_HIP_TRY(hipMalloc(&b->prcp, (size_t)f->nstation * f->nrec * sizeof(double)));
+  PIHM_HIP_TRY(hipMalloc(&b->edir, f->nrec * sizeof(double)));
+  PIHM_HIP_TRY(hipMalloc(&b->ett, f->nrec * sizeof(double)));
+  PIHM_HIP_TRY(hipMemcpy(b->station, st.data(), c->ne * sizeof(int32_t), hipMemcpyHostToDevice));
+  PIHM_HIP_TRY(hipMemcpy(b->prcp, f->prcp, (size_t)f->nstation * f->nrec * sizeof(double), hipMemcpyHostToDevice));
+  PIHM_HIP_TRY(hipMemcpy(b->edir, f->edir, f->nrec * sizeof(double), hipMemcpyHostToDevice));
+  PIHM_HIP_TRY(hipMemcpy(b->ett, f->ett, f->nrec * sizeof(double), hipMemcpyHostToDevice));
+  *out = b;
+  return PIHM_OK;
+}
+
+extern "C" void pihm_gpu_backend_destroy(pihm_gpu_backend *b) {
+  if (!b) return;
+  (void)hipFree(b->station);
+  (void)hipFree(b->prcp);
+  (void)hipFree(b->edir);
+  (void)hipFree(b->ett);
+  delete b;
+}
+
+extern "C" int pihm_gpu_backend_get(pihm_gpu_backend *b, pihm_backend *out) {
+  if (!b || !out) return PIHM_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  out->rhs = gpu_rhs;
+  out->rhs_data = b->ctx;
+  out->apply_forcing = gpu_forcing;
+  out->summary = gpu_summary;
+  out->user = b;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_config *cfg,
+                            double *y_inout, pihm_run_stats *st) {
+  if (!c || !f || !cfg || !y_inout) return PIHM_ERR_ARG;
+  pihm_gpu_backend *b = nullptr;
+  int rc = pihm_gpu_backend_create(c, f, &b);
+  if (rc) return rc;
+  N_Vector y = N_VNew_Hip(c->neq);
+  if (!y) { pihm_gpu_backend_destroy(b); return PIHM_ERR_HIP; }
+  N_VSetStream_Hip(y, c->stream);
+  const int saved_check = c->nan_check;
+  c->nan_check = 0;   // NaN shows up in the norms; polled once at the end
+  rc = pihm_ctx_state_to_device(c, y_inout, y);
+  if (!rc) {
+    // initial ws0 = initial state (src/initialize.c InitVar)
+    rc = gpu_summary(b, y);
+  }
+  pihm_backend be;
+  pihm_gpu_backend_get(b, &be);
+  if (!rc) rc = pihm_drive(&be, y, cfg, st);
+  if (!rc) rc = pihm_ctx_poll_nan(c);
+  if (!rc || rc == PIHM_ERR_NAN) {
+    int rc2 = pihm_ctx_state_from_device(c, y, y_inout);
+    if (!rc) rc = rc2;
+  }
+  c->nan_check = saved_check;
+  N_VDestroy_Hip(y);
+  pihm_gpu_backend_destroy(b);
+  return rc;
+}

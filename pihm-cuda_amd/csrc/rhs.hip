@@ -1,0 +1,1023 @@
+// rhs.hip - MM-PIHM CVODE right-hand side on MI355X (gfx950).
+//
+// Replaces ODE() (src/ode.c:4-399) and everything it calls: SurfH, EtExtract
+// (src/hydrol.c), LateralFlow/FrictSlope (src/lat_flow.c), VerticalFlow
+// (src/vert_flow.c) and RiverFlow (src/river_flow.c).
+//
+// One RHS call is three launches on one stream:
+//   k_prep     per element: clamp the state, surface depth, EffKh, and the
+//              FrictSlope gradient magnitude (neighbour heads recomputed
+//              from the state); writes one 64-byte neighbour record.
+//   k_flux     blocks [0, nbe): per element, lateral fluxes (one 64-byte
+//              gather per neighbour), EtExtract, Infil, Recharge and the
+//              element ydot; elements with a river edge leave a partial
+//              record.  Blocks [nbe, ..): per river segment, all segment
+//              fluxes and the two bank-edge fluxes.
+//   k_assemble per river: upstream inflow as a gather over the upstream
+//              list in ascending caller index (the order of the reference's
+//              serial loop, src/river_flow.c:96-110), then river ydot;
+//              per bank element: ydot with the river-edge fluxes in edge
+//              order.  No atomics: every output has one writer, so results
+//              are deterministic and, apart from device pow/log, equal to
+//              the reference's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "ctx.hpp"
+#include "physics.hpp"
+#include "nvector_impl.hpp"
+
+using namespace pihm;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int ne = a.ne;
+  if (p >= ne) return;
+  const double *y = a.y;
+  const ElemDev &e = a.e;
+
+  const double s = clamp0(y[p]);
+  const double g = clamp0(y[2 * ne + p]);
+  const double surfh = surf_h(s);
+  const double zmax = e.zmax[p];
+  const double head = zmax + surfh;
+  const double gwz = g + e.zmin[p];
+  const double effkh = eff_kh(e.depth[p], e.dmac[p], e.kmach[p], e.areafv[p], e.ksath[p], g);
+
+  double sfmag = 0.0;
+  if (a.surf_mode == DIFF_WAVE) {
+    // FrictSlope: src/lat_flow.c:130-185
+    double h[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int c = e.nb[j * ne + p];
+      if (c >= 0) {
+        h[j] = e.zmax[c] + surf_h(clamp0(y[c]));
+      } else if (c == NB_BOUND) {
+        h[j] = (e.bc_type[j * ne + p] == 0) ? zmax + surfh : e.ebc[j * ne + p];
+      } else {
+        const int r = (-c - 1) >> 1;
+        const double st = clamp0(y[3 * ne + r]);
+        h[j] = (st > a.r.depth[r]) ? a.r.zbed[r] + st : a.r.zmax[r];
+      }
+    }
+    const double x0 = e.nabr_x[p], x1 = e.nabr_x[ne + p], x2 = e.nabr_x[2 * ne + p];
+    const double y0 = e.nabr_y[p], y1 = e.nabr_y[ne + p], y2 = e.nabr_y[2 * ne + p];
+    const double dhdx = dh_by_dl(y0, y1, y2, x0, x1, x2, h[0], h[1], h[2]);
+    const double dhdy = dh_by_dl(x0, x1, x2, y0, y1, y2, h[0], h[1], h[2]);
+    sfmag = sqrt(dhdx * dhdx + dhdy * dhdy);
+  }
+  NbrRec *rp = &a.e.rec[p];
+  rp->surfh = surfh;
+  rp->head = head;
+  rp->gw = g;
+  rp->gwz = gwz;
+  rp->effkh = effkh;
+  rp->sfmag = sfmag;
+}
+
+__device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
+  const int ne = a.ne;
+  const ElemDev &e = a.e;
+  const double *y = a.y;
+  const NbrRec me = e.rec[p];
+  const double u = clamp0(y[ne + p]);
+  const double g = me.gw;
+  const double surfh = me.surfh;
+  const double area = e.area[p], zmax = e.zmax[p], zmin = e.zmin[p];
+  const double depth = e.depth[p], dinf = e.dinf[p];
+
+  // EtExtract: src/hydrol.c:33-97
+  const double edir = e.edir[p], ett = e.ett[p];
+  double edir_s, edir_u, edir_g, ett_u, ett_g;
+  if (surfh >= DEPRSTG) { edir_s = edir; edir_u = 0.0; edir_g = 0.0; }
+  else if (g > depth - dinf) { edir_s = 0.0; edir_u = 0.0; edir_g = edir; }
+  else { edir_s = 0.0; edir_u = edir; edir_g = 0.0; }
+  if (g > depth - e.rzd[p]) { ett_u = 0.0; ett_g = ett; }
+  else { ett_u = ett; ett_g = 0.0; }
+
+  // LateralFlow: src/lat_flow.c:29-63
+  double ovl[3], sub[3];
+  bool bank = false;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int c = e.nb[j * ne + p];
+    const double dist = e.nabrdist[j * ne + p];
+    const double edge = e.edge[j * ne + p];
+    if (c >= 0) {
+      const NbrRec nr = e.rec[c];
+      // SubFlowElemToElem: src/lat_flow.c:285-309
+      {
+        const double diff_h = me.gwz - nr.gwz;
+        const double avg_h = avg_h_sub(diff_h, g, nr.gw);
+        const double grad_h = diff_h / dist;
+        const double avg_ksat = 0.5 * (me.effkh + nr.effkh);
+        sub[j] = avg_ksat * grad_h * avg_h * edge;
+      }
+      // OvlFlowElemToElem: src/lat_flow.c:311-338
+      {
+        double avg_sf = 0.5 * (me.sfmag + nr.sfmag);
+        const double diff_h = (a.surf_mode == KINEMATIC) ? zmax - nr.zmax : me.head - nr.head;
+        const double avg_h = avg_hsurf(diff_h, surfh, nr.surfh);
+        const double grad_h = diff_h / dist;
+        if (a.surf_mode == KINEMATIC) avg_sf = (grad_h > 0.0) ? grad_h : GRADMIN;
+        else avg_sf = (avg_sf > GRADMIN) ? avg_sf : GRADMIN;
+        const double avg_rough = 0.5 * (me.rough + nr.rough);
+        const double crossa = avg_h * edge;
+        ovl[j] = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
+      }
+    } else if (c == NB_BOUND) {
+      // BoundFluxElem: src/lat_flow.c:340-382
+      const int bt = e.bc_type[j * ne + p];
+      const double bcv = e.ebc[j * ne + p];
+      ovl[j] = 0.0;
+      if (bt == 0) {
+        sub[j] = 0.0;
+      } else if (bt > 0) {
+        const double diff_h = me.gwz - bcv;
+        const double avg_h = avg_h_sub(diff_h, g, bcv - zmin);
+        const double grad_h = diff_h / dist;
+        sub[j] = me.effkh * grad_h * avg_h * edge;
+      } else {
+        sub[j] = -bcv;
+      }
+    } else {
+      // river edge: the overland flux carried from the previous call is what
+      // Infil reads (src/vert_flow.c:59-62); both fluxes are replaced later.
+      bank = true;
+      ovl[j] = a.r.hid_ovl[-c - 1];
+      sub[j] = 0.0;
+    }
+  }
+
+  // Unsaturated-zone terms shared by Infil and Recharge (same deficit and
+  // saturation in both, src/vert_flow.c:93-102 and :149-156).
+  const bool wet_bottom = g > depth - dinf;
+  double deficit = 0.0, satn = 0.0, psi_raw = 0.0, kr = 0.0;
+  if (!wet_bottom) {
+    deficit = depth - g;
+    satn = u / deficit;
+    satn = (satn > 1.0) ? 1.0 : satn;
+    satn = (satn < SATMIN) ? SATMIN : satn;
+    psi_raw = psi_vg(satn, e.alpha[p], e.beta[p]);
+    kr = kr_func(e.beta[p], satn);
+  }
+
+  // Infil: src/vert_flow.c:33-130
+  double infil;
+  if (u + g > depth) {
+    infil = 0.0;
+  } else {
+    double applrate = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) applrate += -ovl[j] / area;
+    applrate = (applrate > 0.0) ? applrate : 0.0;
+    applrate += e.pcpdrp[p];
+    double wetfrac = surfh / DEPRSTG;
+    wetfrac = (wetfrac > 0.0) ? wetfrac : 0.0;
+    wetfrac = (wetfrac < 1.0) ? wetfrac : 1.0;
+    double dh_by_dz, kinf;
+    if (wet_bottom) {
+      dh_by_dz = (surfh + zmax - (g + zmin)) / (0.5 * (surfh + dinf));
+      dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
+      const double satkfunc = kr_func(e.beta[p], 1.0);
+      kinf = eff_kinf(e.kinfv[p], e.kmacv[p], e.areafh[p], dh_by_dz, satkfunc, 1.0, applrate, surfh);
+      infil = kinf * dh_by_dz;
+    } else {
+      const double psi_u = (psi_raw > PSIMIN) ? psi_raw : PSIMIN;
+      const double h_u = psi_u + zmax - 0.5 * dinf;
+      dh_by_dz = (surfh + zmax - h_u) / (0.5 * (surfh + dinf));
+      dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
+      kinf = eff_kinf(e.kinfv[p], e.kmacv[p], e.areafh[p], dh_by_dz, kr, satn, applrate, surfh);
+      infil = kinf * dh_by_dz;
+      infil = (infil > 0.0) ? infil : 0.0;
+    }
+    const double ws0 = e.ws0s[p];
+    const double infil_max = applrate + ((ws0 > 0.0) ? ws0 / a.dt : 0.0);
+    infil = (infil > infil_max) ? infil_max : infil;
+    infil *= wetfrac;
+  }
+
+  // Recharge: src/vert_flow.c:132-170
+  double rechg;
+  if (wet_bottom) {
+    rechg = infil;
+  } else {
+    const double dh_by_dz = (0.5 * deficit + psi_raw) / (0.5 * (deficit + g));
+    const double kavg = avg_kv(e.dmac[p], e.ksatv[p], e.kmacv[p], e.areafh[p], deficit, g, kr);
+    rechg = kavg * dh_by_dz;
+    rechg = (rechg > 0.0 && u <= 0.0) ? 0.0 : rechg;
+    rechg = (rechg < 0.0 && g <= 0.0) ? 0.0 : rechg;
+  }
+
+  if (a.record) {
+    a.rec.infil[p] = infil;
+    a.rec.rechg[p] = rechg;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a.rec.ovl[j * ne + p] = ovl[j];
+      a.rec.sub[j * ne + p] = sub[j];
+    }
+  }
+
+  // element ODEs: src/ode.c:121-149
+  double dys = 0.0, dyu = 0.0, dyg = 0.0;
+  dys += e.pcpdrp[p] - infil - edir_s;
+  dyu += infil - rechg - edir_u - ett_u;
+  dyg += rechg - edir_g - ett_g;
+  dyu /= e.porosity[p];
+  a.ydot[ne + p] = dyu;
+  bool nan = isnan(dyu);
+  if (!bank) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      dys -= ovl[j] / area;
+      dyg -= sub[j] / area;
+    }
+    dyg /= e.porosity[p];
+    a.ydot[p] = dys;
+    a.ydot[2 * ne + p] = dyg;
+    nan = nan || isnan(dys) || isnan(dyg);
+  } else {
+    BankRec *b = &e.bank[e.bank_id[p]];
+    b->dys0 = dys;
+    b->dyg0 = dyg;
+#pragma unroll
+    for (int j = 0; j < 3; j++) { b->ovl[j] = ovl[j]; b->sub[j] = sub[j]; }
+  }
+  if (nan) *a.nanflag = 1;
+}
+
+__device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
+  const int ne = a.ne, nr = a.nr;
+  const RiverDev &R = a.r;
+  const double *y = a.y;
+  const double stage = clamp0(y[3 * ne + r]);
+  const double rgw = clamp0(y[3 * ne + nr + r]);
+  const int ord = R.ord[r];
+  const double coeff = R.coeff[r], zbed = R.zbed[r], length = R.length[r];
+  const double rzmin = R.zmin[r];
+  const double bcv = R.bc[r];
+  const int dcode = R.down[r];
+  const int L = R.left[r], Rr = R.right[r];
+  const NbrRec recL = a.e.rec[L];
+  const NbrRec recR = a.e.rec[Rr];
+
+  double upinit = 0.0, rf1, rf9;
+  if (dcode >= 0) {
+    const int d = dcode;
+    if (R.bctype[r] != 0) {
+      // BoundFluxRiver: src/river_flow.c:392-427
+      double flux = 0.0;
+      if (R.bctype[r] > 0) {
+        const double total_h = stage + zbed;
+        const double distance = 0.5 * length;
+        const double grad_h = (total_h - bcv) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+        flux = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+      } else {
+        flux = -bcv;
+      }
+      upinit += flux;
+    }
+    // ChanFlowRiverToRiver: src/river_flow.c:257-300
+    {
+      const double dstage = clamp0(y[3 * ne + d]);
+      const double total_h = stage + zbed;
+      const double perim = riv_perim(ord, stage, coeff);
+      const double total_h_down = dstage + R.zbed[d];
+      const double perim_down = riv_perim(R.ord[d], dstage, R.coeff[d]);
+      const double avg_perim = (perim + perim_down) / 2.0;
+      const double avg_rough = (R.rough[r] + R.rough[d]) / 2.0;
+      const double distance = 0.5 * (length + R.length[d]);
+      const double diff_h = (a.riv_mode == KINEMATIC) ? (zbed - R.zbed[d]) : (total_h - total_h_down);
+      const double grad_h = diff_h / distance;
+      const double avg_sf = (grad_h > 0.0) ? grad_h : RIVGRADMIN;
+      const double crossa = riv_cs_area(ord, stage, coeff);
+      const double crossa_down = riv_cs_area(R.ord[d], dstage, R.coeff[d]);
+      const double avg_crossa = 0.5 * (crossa + crossa_down);
+      const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
+      rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
+    }
+    // SubFlowRiverToRiver (_ARITH_): src/river_flow.c:302-331
+    {
+      const double effk = 0.5 * (recL.effkh + recR.effkh);
+      const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
+      const double dgw = clamp0(y[3 * ne + nr + d]);
+      const double total_h = rgw + rzmin;
+      const double total_h_down = dgw + R.zmin[d];
+      const double avg_wid = (R.width[r] + R.width[d]) / 2.0;
+      const double diff_h = total_h - total_h_down;
+      const double avg_h = avg_h_sub(diff_h, rgw, dgw);
+      const double distance = 0.5 * (length + R.length[d]);
+      const double grad_h = diff_h / distance;
+      const double avg_ksat = 0.5 * (effk + effk_nabr);
+      rf9 = avg_ksat * grad_h * avg_h * avg_wid;
+    }
+  } else {
+    // OutletFlux: src/river_flow.c:333-390
+    double q = 0.0;
+    switch (dcode) {
+      case DIRICHLET: {
+        const double total_h = stage + zbed;
+        const double distance = 0.5 * length;
+        const double grad_h = (total_h - bcv) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+        q = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+        break;
+      }
+      case NEUMANN: q = -bcv; break;
+      case ZERO_DPTH_GRAD: {
+        const double distance = 0.5 * length;
+        const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        q = sqrt(grad_h) * crossa * ((perim > 0.0) ? pow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
+        break;
+      }
+      default: {  // CRIT_DPTH (other codes rejected at creation)
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        q = crossa * sqrt(GRAV * stage);
+        break;
+      }
+    }
+    rf1 = q;
+    rf9 = 0.0;
+  }
+
+  // RiverToElem: src/river_flow.c:113-183
+  const double rzmax = R.zmax[r];
+  const double cwr = R.cwr[r];
+  const double rf2 = ovl_elem_to_river(rzmax, recL.zmax, recL.surfh, zbed, stage, cwr, length);
+  const double rf3 = ovl_elem_to_river(rzmax, recR.zmax, recR.surfh, zbed, stage, cwr, length);
+  const double effk_left = recL.effkh, effk_right = recR.effkh;
+  const double zminL = a.e.zmin[L], zminR = a.e.zmin[Rr];
+  // ChanFlowElemToRiver: src/river_flow.c:429-460
+  auto chan_e2r = [&](const NbrRec &ec, double ezmin, double effk, double distance) {
+    const double diff_h = (stage + zbed) - ec.gwz;
+    double avg_h;
+    if (ezmin > zbed) avg_h = ec.gw;
+    else if (ec.gwz > zbed) avg_h = ec.gwz - zbed;
+    else avg_h = 0.0;
+    avg_h = avg_h_sub(diff_h, stage, avg_h);
+    const double grad_h = diff_h / distance;
+    const double avg_ksat = 0.5 * (effk + R.ksath[r]);
+    return length * avg_ksat * grad_h * avg_h;
+  };
+  // SubFlowElemToRiver (_ARITH_): src/river_flow.c:462-498
+  auto sub_e2r = [&](const NbrRec &ec, double ezmin, double effk, double effk_riv, double distance) {
+    const double diff_h = (rgw + rzmin) - ec.gwz;
+    double avg_h;
+    if (ezmin > zbed) avg_h = 0.0;
+    else if (ec.gwz > zbed) avg_h = zbed - ezmin;
+    else avg_h = ec.gw;
+    avg_h = avg_h_sub(diff_h, rgw, avg_h);
+    const double avg_ksat = 0.5 * (effk + effk_riv);
+    const double grad_h = diff_h / distance;
+    return length * avg_ksat * grad_h * avg_h;
+  };
+  const double rf4 = chan_e2r(recL, zminL, effk_left, R.dist_l[r]);
+  const double rf5 = chan_e2r(recR, zminR, effk_right, R.dist_r[r]);
+  const double effk_riv = 0.5 * (effk_left + effk_right);
+  const double rf7 = sub_e2r(recL, zminL, effk_left, effk_riv, R.dist_l[r]);
+  const double rf8 = sub_e2r(recR, zminR, effk_right, effk_riv, R.dist_r[r]);
+
+  // ChanLeak: src/river_flow.c:500-518
+  double rf6;
+  {
+    double diff_h;
+    if (zbed - (rgw + rzmin) > 0.0) diff_h = stage;
+    else diff_h = stage + zbed - (rgw + rzmin);
+    const double grad_h = diff_h / R.bedthick[r];
+    rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
+  }
+
+  double *rf = R.rf;
+  rf[1 * nr + r] = rf1;
+  rf[2 * nr + r] = rf2;
+  rf[3 * nr + r] = rf3;
+  rf[4 * nr + r] = rf4;
+  rf[5 * nr + r] = rf5;
+  rf[6 * nr + r] = rf6;
+  rf[7 * nr + r] = rf7;
+  rf[8 * nr + r] = rf8;
+  rf[9 * nr + r] = rf9;
+  R.upinit[r] = upinit;
+  // bank-edge replacement values (src/river_flow.c:159-182)
+  R.hid_ovl[2 * nr + 2 * r] = -rf2;
+  R.bank_sub[2 * r] = -(rf4 + rf7);
+  R.hid_ovl[2 * nr + 2 * r + 1] = -rf3;
+  R.bank_sub[2 * r + 1] = -(rf5 + rf8);
+}
+
+__global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
+  if ((int)blockIdx.x < nbe) {
+    const int p = xcd_tile(blockIdx.x, nbe) * kBlock + threadIdx.x;
+    if (p < a.ne) elem_flux(a, p);
+  } else {
+    const int r = (blockIdx.x - nbe) * kBlock + threadIdx.x;
+    if (r < a.nr) river_flux(a, r);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int ne = a.ne, nr = a.nr;
+  const RiverDev &R = a.r;
+  if (t < nr) {
+    const int r = t;
+    // src/river_flow.c:96-110 in ascending upstream index
+    double up = R.upinit[r], upa = 0.0;
+    for (int k = R.up_ptr[r]; k < R.up_ptr[r + 1]; k++) {
+      const int u = R.up_idx[k];
+      up -= R.rf[1 * nr + u];
+      upa -= R.rf[9 * nr + u];
+    }
+    const double area = R.area[r];
+    const double rf6 = R.rf[6 * nr + r];
+    // src/ode.c:238-251
+    double ds = 0.0;
+    ds -= up / area;
+#pragma unroll
+    for (int j = 1; j <= 6; j++) ds -= R.rf[j * nr + r] / area;
+    double dg = 0.0;
+    dg += -R.rf[7 * nr + r] - R.rf[8 * nr + r] - R.rf[9 * nr + r] - upa + rf6;
+    dg /= R.porosity[r] * area;
+    a.ydot[3 * ne + r] = ds;
+    a.ydot[3 * ne + nr + r] = dg;
+    if (a.record) {
+      R.rf[0 * nr + r] = up;
+      R.rf[10 * nr + r] = upa;
+    }
+    if (isnan(ds) || isnan(dg)) *a.nanflag = 1;
+  } else if (t < nr + a.nbank) {
+    const int k = t - nr;
+    const int p = a.e.bank_elem[k];
+    const BankRec b = a.e.bank[k];
+    const double area = a.e.area[p];
+    double dys = b.dys0, dyg = b.dyg0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const int c = a.e.nb[j * ne + p];
+      double o, s;
+      if (c >= 0 || c == NB_BOUND) {
+        o = b.ovl[j];
+        s = b.sub[j];
+      } else {
+        const int slot = -c - 1;
+        o = R.hid_ovl[2 * nr + slot];
+        s = R.bank_sub[slot];
+        R.hid_ovl[slot] = o;  // carried to the next call's Infil
+        if (a.record) {
+          a.rec.ovl[j * ne + p] = o;
+          a.rec.sub[j * ne + p] = s;
+        }
+      }
+      dys -= o / area;
+      dyg -= s / area;
+    }
+    dyg /= a.e.porosity[p];
+    a.ydot[p] = dys;
+    a.ydot[2 * ne + p] = dyg;
+    if (isnan(dys) || isnan(dyg)) *a.nanflag = 1;
+  }
+}
+
+__global__ void k_permute_state(const double *src, double *dst, const int32_t *perm, int ne, int nr,
+                                int to_device) {
+  // to_device: dst[b*ne+p] = src[b*ne+perm[p]]; otherwise the inverse.
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = 3L * ne + 2L * nr;
+  if (i >= n) return;
+  if (i >= 3L * ne) { dst[i] = src[i]; return; }
+  const long b = i / ne, p = i % ne;
+  const long q = b * ne + perm[p];
+  if (to_device) dst[i] = src[q];
+  else dst[q] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+uint64_t hilbert_key(uint32_t x, uint32_t y) {
+  const uint32_t n = 1u << 16;
+  uint64_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+struct Arena {
+  std::vector<std::pair<void **, size_t>> req;
+  template <class T> void add(T **p, size_t n) { req.push_back({(void **)p, n * sizeof(T)}); }
+  size_t total() const {
+    size_t t = 0;
+    for (auto &r : req) t += (r.second + 255) & ~size_t(255);
+    return t;
+  }
+  void carve(char *base) {
+    size_t off = 0;
+    for (auto &r : req) { *r.first = base + off; off += (r.second + 255) & ~size_t(255); }
+  }
+};
+
+int validate(const pihm_mesh_desc *m) {
+  if (!m || m->nelem <= 0 || m->nriver < 0) return PIHM_ERR_ARG;
+  if (m->surf_mode != PIHM_KINEMATIC && m->surf_mode != PIHM_DIFF_WAVE) return PIHM_ERR_ARG;
+  if (m->riv_mode != PIHM_KINEMATIC && m->riv_mode != PIHM_DIFF_WAVE) return PIHM_ERR_ARG;
+  if (!(m->stepsize > 0.0)) return PIHM_ERR_ARG;
+  const void *ep[] = {m->nabr, m->bc_type, m->x, m->y, m->area, m->zmin, m->zmax, m->edge,
+                      m->nabrdist, m->nabr_x, m->nabr_y, m->depth, m->ksath, m->ksatv, m->kinfv,
+                      m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
+                      m->areafv, m->areafh, m->rough, m->rzd};
+  for (auto q : ep) if (!q) return PIHM_ERR_ARG;
+  const int ne = m->nelem, nr = m->nriver;
+  if (nr > 0) {
+    const void *rp[] = {m->down, m->leftele, m->rightele, m->riverbc_type, m->intrpl_ord, m->rarea,
+                        m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left, m->dist_right,
+                        m->rdepth, m->coeff, m->length, m->width, m->rrough, m->cwr, m->rksath,
+                        m->rksatv, m->bedthick, m->rporosity};
+    for (auto q : rp) if (!q) return PIHM_ERR_ARG;
+  }
+  for (long k = 0; k < 3L * ne; k++) {
+    const int c = m->nabr[k];
+    if (c > ne || c < -nr) return PIHM_ERR_ARG;
+  }
+  for (int r = 0; r < nr; r++) {
+    const int d = m->down[r];
+    if (d > nr || d == 0 || d < -4) return PIHM_ERR_ARG;
+    if (d == r + 1) return PIHM_ERR_ARG;
+    if (m->leftele[r] < 1 || m->leftele[r] > ne || m->rightele[r] < 1 || m->rightele[r] > ne)
+      return PIHM_ERR_ARG;
+    if (m->intrpl_ord[r] < 1 || m->intrpl_ord[r] > 4) return PIHM_ERR_ARG;
+    // each bank must carry the river tag on one edge (src/init_river.c:24-35)
+    for (int side = 0; side < 2; side++) {
+      const int el = (side ? m->rightele[r] : m->leftele[r]) - 1;
+      bool found = false;
+      for (int j = 0; j < 3; j++) found = found || (m->nabr[3 * el + j] == -(r + 1));
+      if (!found) return PIHM_ERR_ARG;
+    }
+  }
+  // every river edge must belong to a bank of that river
+  for (int i = 0; i < ne; i++)
+    for (int j = 0; j < 3; j++) {
+      const int c = m->nabr[3 * i + j];
+      if (c < 0) {
+        const int r = -c - 1;
+        if (m->leftele[r] != i + 1 && m->rightele[r] != i + 1) return PIHM_ERR_ARG;
+      }
+    }
+  return PIHM_OK;
+}
+
+}  // namespace
+
+extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
+  if (!out) return PIHM_ERR_ARG;
+  *out = nullptr;
+  int rc = validate(m);
+  if (rc != PIHM_OK) return rc;
+  const int ne = m->nelem, nr = m->nriver;
+
+  pihm_ctx *c = new pihm_ctx();
+  c->ne = ne; c->nr = nr; c->neq = 3 * ne + 2 * nr;
+  c->surf_mode = m->surf_mode; c->riv_mode = m->riv_mode; c->dt = m->stepsize;
+  c->stream = nullptr; c->nan_check = 1; c->record = 0; c->ncalls = 0;
+  PIHM_HIP_TRY(hipGetDevice(&c->device));
+
+  // Hilbert order of centroids
+  double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
+  for (int i = 1; i < ne; i++) {
+    xmin = std::min(xmin, m->x[i]); xmax = std::max(xmax, m->x[i]);
+    ymin = std::min(ymin, m->y[i]); ymax = std::max(ymax, m->y[i]);
+  }
+  const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
+  std::vector<uint64_t> key(ne);
+  for (int i = 0; i < ne; i++) {
+    const double fx = (m->x[i] - xmin) / span, fy = (m->y[i] - ymin) / span;
+    const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fx * 65535.0)));
+    const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fy * 65535.0)));
+    key[i] = hilbert_key(qx, qy);
+  }
+  c->perm.resize(ne);
+  std::iota(c->perm.begin(), c->perm.end(), 0);
+  std::stable_sort(c->perm.begin(), c->perm.end(),
+                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  c->iperm.resize(ne);
+  for (int p = 0; p < ne; p++) c->iperm[c->perm[p]] = p;
+
+  // neighbour codes and bank records
+  c->nb_host.assign(3L * ne, 0);
+  std::vector<int32_t> bank_id(ne, -1), bank_elem;
+  for (int p = 0; p < ne; p++) {
+    const int i = c->perm[p];
+    bool bank = false;
+    for (int j = 0; j < 3; j++) {
+      const int v = m->nabr[3 * i + j];
+      int32_t code;
+      if (v > 0) code = c->iperm[v - 1];
+      else if (v == 0) code = NB_BOUND;
+      else {
+        const int r = -v - 1;
+        const int side = (m->leftele[r] == i + 1) ? 0 : 1;
+        code = -(2 * r + side + 1);
+        bank = true;
+      }
+      c->nb_host[(long)j * ne + p] = code;
+    }
+    if (bank) { bank_id[p] = (int32_t)bank_elem.size(); bank_elem.push_back(p); }
+  }
+  // The reference replaces the first edge of each bank tagged with the river
+  // (src/river_flow.c:161-182); a bank with two edges tagged by the same
+  // river cannot occur in a valid mesh and was rejected above only if untagged.
+  c->nbank = (int)bank_elem.size();
+
+  // upstream lists in ascending caller index
+  std::vector<int32_t> up_ptr(nr + 1, 0), up_idx;
+  for (int r = 0; r < nr; r++) if (m->down[r] > 0) up_ptr[m->down[r]]++;
+  for (int r = 0; r < nr; r++) up_ptr[r + 1] += up_ptr[r];
+  up_idx.assign(up_ptr[nr] > 0 ? up_ptr[nr] : 1, 0);
+  {
+    std::vector<int32_t> fill(up_ptr.begin(), up_ptr.end() - 1);
+    for (int r = 0; r < nr; r++)
+      if (m->down[r] > 0) up_idx[fill[m->down[r] - 1]++] = r;
+  }
+
+  // device allocation
+  RhsArgs &A = c->args;
+  ElemDev &E = A.e;
+  RiverDev &R = A.r;
+  Arena ar;
+  double **ed[] = {&E.area, &E.zmin, &E.zmax, &E.depth, &E.ksath, &E.ksatv, &E.kinfv, &E.dinf,
+                   &E.alpha, &E.beta, &E.porosity, &E.dmac, &E.kmach, &E.kmacv, &E.areafv,
+                   &E.areafh, &E.rough, &E.rzd, &E.pcpdrp, &E.edir, &E.ett, &E.ws0s};
+  for (auto p : ed) ar.add(p, ne);
+  double **ed3[] = {&E.edge, &E.nabrdist, &E.nabr_x, &E.nabr_y, &E.ebc};
+  for (auto p : ed3) ar.add(p, 3L * ne);
+  ar.add(&E.nb, 3L * ne);
+  ar.add(&E.bc_type, 3L * ne);
+  ar.add(&E.bank_id, ne);
+  ar.add(&E.rec, ne);
+  ar.add(&E.bank, std::max(c->nbank, 1));
+  ar.add(&E.bank_elem, std::max(c->nbank, 1));
+  const int nr1 = std::max(nr, 1);
+  int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord};
+  for (auto p : ri) ar.add(p, nr1);
+  ar.add(&R.up_ptr, nr + 1);
+  ar.add(&R.up_idx, up_idx.size());
+  double **rd[] = {&R.area, &R.zmin, &R.zmax, &R.zbed, &R.node_zmax, &R.dist_l, &R.dist_r,
+                   &R.depth, &R.coeff, &R.length, &R.width, &R.rough, &R.cwr, &R.ksath,
+                   &R.ksatv, &R.bedthick, &R.porosity, &R.bc, &R.upinit};
+  for (auto p : rd) ar.add(p, nr1);
+  ar.add(&R.rf, 11L * nr1);
+  ar.add(&R.hid_ovl, 4L * nr1);
+  ar.add(&R.bank_sub, 2L * nr1);
+  ar.add(&A.rec.ovl, 3L * ne);
+  ar.add(&A.rec.sub, 3L * ne);
+  ar.add(&A.rec.infil, ne);
+  ar.add(&A.rec.rechg, ne);
+  ar.add(&c->nanflag_dev, 1);
+  c->arena_bytes = ar.total();
+  PIHM_HIP_TRY(hipMalloc(&c->arena, c->arena_bytes));
+  PIHM_HIP_TRY(hipMemset(c->arena, 0, c->arena_bytes));
+  ar.carve((char *)c->arena);
+  PIHM_HIP_TRY(hipHostMalloc((void **)&c->nanflag_host, sizeof(int), hipHostMallocDefault));
+  *c->nanflag_host = 0;
+
+  // upload element statics in device order
+  std::vector<double> hd(3L * ne);
+  auto up1 = [&](double *dst, const double *src) -> hipError_t {
+    for (int p = 0; p < ne; p++) hd[p] = src[c->perm[p]];
+    return hipMemcpy(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice);
+  };
+  auto up3 = [&](double *dst, const double *src) -> hipError_t {
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) hd[(long)j * ne + p] = src[3L * c->perm[p] + j];
+    return hipMemcpy(dst, hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice);
+  };
+  const double *es[] = {m->area, m->zmin, m->zmax, m->depth, m->ksath, m->ksatv, m->kinfv,
+                        m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
+                        m->areafv, m->areafh, m->rough, m->rzd};
+  for (int k = 0; k < 18; k++) PIHM_HIP_TRY(up1(*ed[k], es[k]));
+  PIHM_HIP_TRY(up3(E.edge, m->edge));
+  PIHM_HIP_TRY(up3(E.nabrdist, m->nabrdist));
+  PIHM_HIP_TRY(up3(E.nabr_x, m->nabr_x));
+  PIHM_HIP_TRY(up3(E.nabr_y, m->nabr_y));
+  {
+    std::vector<int32_t> bt(3L * ne);
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) bt[(long)j * ne + p] = m->bc_type[3L * c->perm[p] + j];
+    PIHM_HIP_TRY(hipMemcpy(E.bc_type, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(E.nb, c->nb_host.data(), c->nb_host.size() * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(E.bank_id, bank_id.data(), ne * 4, hipMemcpyHostToDevice));
+    if (c->nbank)
+      PIHM_HIP_TRY(hipMemcpy(E.bank_elem, bank_elem.data(), c->nbank * 4, hipMemcpyHostToDevice));
+  }
+  {
+    // static part of the neighbour records
+    std::vector<NbrRec> rec(ne);
+    std::memset(rec.data(), 0, ne * sizeof(NbrRec));
+    for (int p = 0; p < ne; p++) {
+      rec[p].rough = m->rough[c->perm[p]];
+      rec[p].zmax = m->zmax[c->perm[p]];
+    }
+    PIHM_HIP_TRY(hipMemcpy(E.rec, rec.data(), ne * sizeof(NbrRec), hipMemcpyHostToDevice));
+  }
+  if (nr > 0) {
+    std::vector<int32_t> down(nr), left(nr), right(nr);
+    for (int r = 0; r < nr; r++) {
+      down[r] = (m->down[r] > 0) ? m->down[r] - 1 : m->down[r];
+      left[r] = c->iperm[m->leftele[r] - 1];
+      right[r] = c->iperm[m->rightele[r] - 1];
+    }
+    PIHM_HIP_TRY(hipMemcpy(R.down, down.data(), nr * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.left, left.data(), nr * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.right, right.data(), nr * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.bctype, m->riverbc_type, nr * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.ord, m->intrpl_ord, nr * 4, hipMemcpyHostToDevice));
+    const double *rs[] = {m->rarea, m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left,
+                          m->dist_right, m->rdepth, m->coeff, m->length, m->width, m->rrough,
+                          m->cwr, m->rksath, m->rksatv, m->bedthick, m->rporosity};
+    for (int k = 0; k < 17; k++)
+      PIHM_HIP_TRY(hipMemcpy(*rd[k], rs[k], nr * sizeof(double), hipMemcpyHostToDevice));
+  }
+  PIHM_HIP_TRY(hipMemcpy(R.up_ptr, up_ptr.data(), (nr + 1) * 4, hipMemcpyHostToDevice));
+  PIHM_HIP_TRY(hipMemcpy(R.up_idx, up_idx.data(), up_idx.size() * 4, hipMemcpyHostToDevice));
+
+  A.ne = ne; A.nr = nr; A.nbank = c->nbank;
+  A.surf_mode = m->surf_mode; A.riv_mode = m->riv_mode; A.dt = m->stepsize;
+  A.cur = 0; A.record = 0;
+  A.nanflag = c->nanflag_dev;
+  c->stage_host = nullptr;
+  *out = c;
+  return PIHM_OK;
+}
+
+extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
+  if (!c) return;
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->nanflag_host) (void)hipHostFree(c->nanflag_host);
+  delete c;
+}
+
+extern "C" int pihm_ctx_set_stream(pihm_ctx *c, void *s) {
+  if (!c) return PIHM_ERR_ARG;
+  c->stream = (hipStream_t)s;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_neq(const pihm_ctx *c) { return c ? c->neq : -1; }
+
+extern "C" int pihm_ctx_set_nan_check(pihm_ctx *c, int on) {
+  if (!c) return PIHM_ERR_ARG;
+  c->nan_check = on ? 1 : 0;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_set_record(pihm_ctx *c, int on) {
+  if (!c) return PIHM_ERR_ARG;
+  c->record = on ? 1 : 0;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_perm(const pihm_ctx *c, int32_t *out) {
+  if (!c || !out) return PIHM_ERR_ARG;
+  std::memcpy(out, c->perm.data(), c->ne * sizeof(int32_t));
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in) {
+  if (!c || !in) return PIHM_ERR_ARG;
+  const int ne = c->ne, nr = c->nr;
+  std::vector<double> hd(3L * ne);
+  auto up1 = [&](double *dst, const double *src) -> int {
+    for (int p = 0; p < ne; p++) hd[p] = src[c->perm[p]];
+    PIHM_HIP_TRY(hipMemcpyAsync(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PIHM_OK;
+  };
+  ElemDev &E = c->args.e;
+  int rc;
+  if (in->pcpdrp && (rc = up1(E.pcpdrp, in->pcpdrp))) return rc;
+  if (in->edir && (rc = up1(E.edir, in->edir))) return rc;
+  if (in->ett && (rc = up1(E.ett, in->ett))) return rc;
+  if (in->ws0_surf && (rc = up1(E.ws0s, in->ws0_surf))) return rc;
+  if (in->elem_bc) {
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) hd[(long)j * ne + p] = in->elem_bc[3L * c->perm[p] + j];
+    PIHM_HIP_TRY(hipMemcpyAsync(E.ebc, hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  if (in->river_bc && nr > 0) {
+    PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.bc, in->river_bc, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return PIHM_OK;
+}
+
+static int check_vec(const pihm_ctx *c, N_Vector v) {
+  if (!v || !v->ops || v->ops->nvgetvectorid != N_VGetVectorID_Hip) return PIHM_ERR_ARG;
+  if (N_VGetLength_Hip(v) != c->neq) return PIHM_ERR_ARG;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_state_to_device(pihm_ctx *c, const double *host, N_Vector dev) {
+  if (!c || !host || check_vec(c, dev)) return PIHM_ERR_ARG;
+  std::vector<double> tmp(c->neq);
+  const int ne = c->ne;
+  for (int b = 0; b < 3; b++)
+    for (int p = 0; p < ne; p++) tmp[(long)b * ne + p] = host[(long)b * ne + c->perm[p]];
+  for (long i = 3L * ne; i < c->neq; i++) tmp[i] = host[i];
+  return N_VCopyToDevice_Hip(dev, tmp.data());
+}
+
+extern "C" int pihm_ctx_state_from_device(pihm_ctx *c, N_Vector dev, double *host) {
+  if (!c || !host || check_vec(c, dev)) return PIHM_ERR_ARG;
+  std::vector<double> tmp(c->neq);
+  int rc = N_VCopyFromDevice_Hip(dev, tmp.data());
+  if (rc) return rc;
+  const int ne = c->ne;
+  for (int b = 0; b < 3; b++)
+    for (int p = 0; p < ne; p++) host[(long)b * ne + c->perm[p]] = tmp[(long)b * ne + p];
+  for (long i = 3L * ne; i < c->neq; i++) host[i] = tmp[i];
+  return PIHM_OK;
+}
+
+// Launch the three kernels of one RHS call (no synchronisation).
+extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
+  RhsArgs a = c->args;
+  a.y = y;
+  a.ydot = ydot;
+  a.record = c->record;
+  const int nbe = div_up(c->ne, kBlock);
+  const int nbr = div_up(c->nr, kBlock);
+  hipLaunchKernelGGL(k_prep, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_flux, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  const int nas = div_up((long)c->nr + c->nbank, kBlock);
+  if (nas > 0) hipLaunchKernelGGL(k_assemble, dim3(nas), dim3(kBlock), 0, c->stream, a);
+  PIHM_HIP_TRY(hipGetLastError());
+  c->ncalls++;
+  return PIHM_OK;
+}
+
+// Reads and clears the sticky NaN flag (synchronises the stream).
+extern "C" int pihm_ctx_poll_nan(pihm_ctx *c) {
+  if (!c) return PIHM_ERR_ARG;
+  PIHM_HIP_TRY(hipMemcpyAsync(c->nanflag_host, c->nanflag_dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (*c->nanflag_host) {
+    *c->nanflag_host = 0;
+    PIHM_HIP_TRY(hipMemcpyAsync(c->nanflag_dev, c->nanflag_host, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PIHM_ERR_NAN;
+  }
+  return PIHM_OK;
+}
+
+extern "C" int PIHM_ODE_Hip(realtype t, N_Vector y, N_Vector ydot, void *user_data) {
+  (void)t;
+  pihm_ctx *c = (pihm_ctx *)user_data;
+  if (!c || check_vec(c, y) || check_vec(c, ydot)) return PIHM_ERR_ARG;
+  int rc = pihm_rhs_launch(c, N_VGetDeviceArrayPointer_Hip(y), N_VGetDeviceArrayPointer_Hip(ydot));
+  if (rc) return rc;
+  if (c->nan_check) return pihm_ctx_poll_nan(c);
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
+  if (!c || !ovl3) return PIHM_ERR_ARG;
+  const int ne = c->ne, nr = c->nr;
+  std::vector<double> hid(2L * std::max(nr, 1));
+  if (nr > 0) {
+    PIHM_HIP_TRY(hipMemcpyAsync(hid.data(), c->args.r.hid_ovl, 2L * nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  for (int p = 0; p < ne; p++)
+    for (int j = 0; j < 3; j++) {
+      const int code = c->nb_host[(long)j * ne + p];
+      double v = 0.0;
+      if (code < 0 && code != NB_BOUND) v = hid[-code - 1];
+      ovl3[3L * c->perm[p] + j] = v;
+    }
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_set_river_edge_ovlflow(pihm_ctx *c, const double *ovl3) {
+  if (!c || !ovl3) return PIHM_ERR_ARG;
+  const int ne = c->ne, nr = c->nr;
+  if (nr == 0) return PIHM_OK;
+  std::vector<double> hid(2L * nr, 0.0);
+  for (int p = 0; p < ne; p++)
+    for (int j = 0; j < 3; j++) {
+      const int code = c->nb_host[(long)j * ne + p];
+      if (code < 0 && code != NB_BOUND) hid[-code - 1] = ovl3[3L * c->perm[p] + j];
+    }
+  PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.hid_ovl, hid.data(), 2L * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_fluxes(pihm_ctx *c, double *ovl, double *sub, double *infil,
+                                   double *rechg, double *rivflow) {
+  if (!c) return PIHM_ERR_ARG;
+  if (!c->record) return PIHM_ERR_ARG;  // fluxes are kept only in recording mode
+  const int ne = c->ne, nr = c->nr;
+  std::vector<double> h(std::max(3L * ne, 11L * nr));
+  auto get = [&](const double *src, long n) -> int {
+    PIHM_HIP_TRY(hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PIHM_OK;
+  };
+  int rc;
+  if (ovl) {
+    if ((rc = get(c->args.rec.ovl, 3L * ne))) return rc;
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) ovl[3L * c->perm[p] + j] = h[(long)j * ne + p];
+  }
+  if (sub) {
+    if ((rc = get(c->args.rec.sub, 3L * ne))) return rc;
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) sub[3L * c->perm[p] + j] = h[(long)j * ne + p];
+  }
+  if (infil) {
+    if ((rc = get(c->args.rec.infil, ne))) return rc;
+    for (int p = 0; p < ne; p++) infil[c->perm[p]] = h[p];
+  }
+  if (rechg) {
+    if ((rc = get(c->args.rec.rechg, ne))) return rc;
+    for (int p = 0; p < ne; p++) rechg[c->perm[p]] = h[p];
+  }
+  if (rivflow && nr > 0) {
+    if ((rc = get(c->args.r.rf, 11L * nr))) return rc;
+    for (int r = 0; r < nr; r++)
+      for (int k = 0; k < 11; k++) rivflow[11L * r + k] = h[(long)k * nr + r];
+  }
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_sync(pihm_ctx *c) {
+  if (!c) return PIHM_ERR_ARG;
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PIHM_OK;
+}
+
+// Average duration of each of the three RHS kernels over `iters` back-to-back
+// evaluations, measured with HIP events on the context's stream.
+extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot, int iters,
+                                     double *ms_prep, double *ms_flux, double *ms_asm) {
+  if (!c || !y || !ydot || iters <= 0) return PIHM_ERR_ARG;
+  RhsArgs a = c->args;
+  a.y = y;
+  a.ydot = ydot;
+  a.record = 0;
+  const int nbe = div_up(c->ne, kBlock);
+  const int nbr = div_up(c->nr, kBlock);
+  const int nas = div_up((long)c->nr + c->nbank, kBlock);
+  hipEvent_t ev[4];
+  for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
+  float t[3] = {0, 0, 0};
+  for (int it = 0; it < iters; it++) {
+    PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
+    hipLaunchKernelGGL(k_prep, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
+    hipLaunchKernelGGL(k_flux, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    PIHM_HIP_TRY(hipEventRecord(ev[2], c->stream));
+    if (nas > 0) hipLaunchKernelGGL(k_assemble, dim3(nas), dim3(kBlock), 0, c->stream, a);
+    PIHM_HIP_TRY(hipEventRecord(ev[3], c->stream));
+    PIHM_HIP_TRY(hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; k++) {
+      float ms = 0;
+      PIHM_HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      t[k] += ms;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (ms_prep) *ms_prep = t[0] / iters;
+  if (ms_flux) *ms_flux = t[1] / iters;
+  if (ms_asm) *ms_asm = t[2] / iters;
+  return PIHM_OK;
+}

@@ -1,0 +1,213 @@
+"""ctypes bindings of libpihm_amd.so (the HIP RHS, device N_Vector, integrator, driver).
+
+The shared library is built in-tree (``make -C pihm-cuda_amd``); importing this module
+fails loudly when it is missing: there is no CPU fallback for the product path.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpihm_amd.so")
+
+c_double_p = C.POINTER(C.c_double)
+c_int32_p = C.POINTER(C.c_int32)
+
+PIHM_OK = 0
+PIHM_ERR_ARG = -1
+PIHM_ERR_HIP = -2
+PIHM_ERR_NAN = 1
+PIHM_ERR_TOO_LARGE = -3
+
+
+class MeshDesc(C.Structure):
+    """pihm_mesh_desc (include/pihm_rhs.h)."""
+
+    _fields_ = [
+        ("nelem", C.c_int32), ("nriver", C.c_int32), ("surf_mode", C.c_int32),
+        ("riv_mode", C.c_int32), ("stepsize", C.c_double),
+        ("nabr", c_int32_p), ("bc_type", c_int32_p),
+        ("x", c_double_p), ("y", c_double_p),
+        ("area", c_double_p), ("zmin", c_double_p), ("zmax", c_double_p),
+        ("edge", c_double_p), ("nabrdist", c_double_p), ("nabr_x", c_double_p), ("nabr_y", c_double_p),
+        ("depth", c_double_p), ("ksath", c_double_p), ("ksatv", c_double_p), ("kinfv", c_double_p),
+        ("dinf", c_double_p), ("alpha", c_double_p), ("beta", c_double_p), ("porosity", c_double_p),
+        ("dmac", c_double_p), ("kmach", c_double_p), ("kmacv", c_double_p), ("areafv", c_double_p),
+        ("areafh", c_double_p), ("rough", c_double_p), ("rzd", c_double_p),
+        ("down", c_int32_p), ("leftele", c_int32_p), ("rightele", c_int32_p),
+        ("riverbc_type", c_int32_p), ("intrpl_ord", c_int32_p),
+        ("rarea", c_double_p), ("rzmin", c_double_p), ("rzmax", c_double_p), ("zbed", c_double_p),
+        ("node_zmax", c_double_p), ("dist_left", c_double_p), ("dist_right", c_double_p),
+        ("rdepth", c_double_p), ("coeff", c_double_p), ("length", c_double_p), ("width", c_double_p),
+        ("rrough", c_double_p), ("cwr", c_double_p), ("rksath", c_double_p), ("rksatv", c_double_p),
+        ("bedthick", c_double_p), ("rporosity", c_double_p),
+    ]
+
+
+class StepInputs(C.Structure):
+    """pihm_step_inputs (include/pihm_rhs.h)."""
+
+    _fields_ = [("pcpdrp", c_double_p), ("edir", c_double_p), ("ett", c_double_p),
+                ("ws0_surf", c_double_p), ("elem_bc", c_double_p), ("river_bc", c_double_p)]
+
+
+class RunConfig(C.Structure):
+    """pihm_run_config (include/pihm_driver.h)."""
+
+    _fields_ = [("t_start", C.c_double), ("t_end", C.c_double), ("stepsize", C.c_double),
+                ("etstep", C.c_double), ("reltol", C.c_double), ("abstol", C.c_double),
+                ("initstep", C.c_double), ("nncfn", C.c_double), ("nnimax", C.c_double),
+                ("nnimin", C.c_double), ("decr", C.c_double), ("incr", C.c_double),
+                ("stmin", C.c_double), ("maxl", C.c_int), ("max_steps_per_call", C.c_long)]
+
+
+class Forcing(C.Structure):
+    """pihm_forcing (include/pihm_driver.h)."""
+
+    _fields_ = [("nstation", C.c_int32), ("nrec", C.c_int32), ("rec_dt", C.c_double),
+                ("prcp", c_double_p), ("edir", c_double_p), ("ett", c_double_p),
+                ("station", c_int32_p)]
+
+
+class RunStats(C.Structure):
+    """pihm_run_stats (include/pihm_driver.h)."""
+
+    _fields_ = [("nst", C.c_long), ("nfe", C.c_long), ("nfes", C.c_long), ("nni", C.c_long),
+                ("nli", C.c_long), ("ncfn", C.c_long), ("netf", C.c_long),
+                ("model_steps", C.c_long), ("wall_s", C.c_double), ("flag", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+RhsFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
+ForcingFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)
+SummaryFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+StepEndFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
+
+
+class Backend(C.Structure):
+    """pihm_backend (include/pihm_driver.h)."""
+
+    _fields_ = [("rhs", C.c_void_p), ("rhs_data", C.c_void_p), ("apply_forcing", C.c_void_p),
+                ("summary", C.c_void_p), ("model_step_end", C.c_void_p), ("user", C.c_void_p)]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_PROTOS = [
+    ("pihm_ctx_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    ("pihm_ctx_destroy", None, [C.c_void_p]),
+    ("pihm_ctx_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
+    ("pihm_ctx_neq", C.c_int, [C.c_void_p]),
+    ("pihm_ctx_get_perm", C.c_int, [C.c_void_p, c_int32_p]),
+    ("pihm_ctx_state_to_device", C.c_int, [C.c_void_p, c_double_p, C.c_void_p]),
+    ("pihm_ctx_state_from_device", C.c_int, [C.c_void_p, C.c_void_p, c_double_p]),
+    ("PIHM_ODE_Hip", C.c_int, [C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pihm_ctx_set_nan_check", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_rhs_launch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pihm_ctx_poll_nan", C.c_int, [C.c_void_p]),
+    ("pihm_ctx_set_record", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_ctx_sync", C.c_int, [C.c_void_p]),
+    ("pihm_ctx_time_kernels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
+                                        c_double_p, c_double_p]),
+    ("pihm_ctx_get_river_edge_ovlflow", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_ctx_set_river_edge_ovlflow", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_ctx_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                      c_double_p]),
+    ("N_VNew_Hip", C.c_void_p, [C.c_long]),
+    ("N_VNewEmpty_Hip", C.c_void_p, [C.c_long]),
+    ("N_VMake_Hip", C.c_void_p, [C.c_long, C.c_void_p]),
+    ("N_VDestroy_Hip", None, [C.c_void_p]),
+    ("N_VSetStream_Hip", None, [C.c_void_p, C.c_void_p]),
+    ("N_VCopyToDevice_Hip", C.c_int, [C.c_void_p, c_double_p]),
+    ("N_VCopyFromDevice_Hip", C.c_int, [C.c_void_p, c_double_p]),
+    ("N_VGetDeviceArrayPointer_Hip", C.c_void_p, [C.c_void_p]),
+    ("N_VGetLength_Hip", C.c_long, [C.c_void_p]),
+    ("N_VGetVectorID_Hip", C.c_int, [C.c_void_p]),
+    ("N_VClone_Hip", C.c_void_p, [C.c_void_p]),
+    ("N_VCloneEmpty_Hip", C.c_void_p, [C.c_void_p]),
+    ("N_VSpace_Hip", None, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("N_VGetArrayPointer_Hip", C.c_void_p, [C.c_void_p]),
+    ("N_VSetArrayPointer_Hip", None, [C.c_void_p, C.c_void_p]),
+    ("N_VLinearSum_Hip", None, [C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
+    ("N_VConst_Hip", None, [C.c_double, C.c_void_p]),
+    ("N_VProd_Hip", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("N_VDiv_Hip", None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("N_VScale_Hip", None, [C.c_double, C.c_void_p, C.c_void_p]),
+    ("N_VAbs_Hip", None, [C.c_void_p, C.c_void_p]),
+    ("N_VInv_Hip", None, [C.c_void_p, C.c_void_p]),
+    ("N_VAddConst_Hip", None, [C.c_void_p, C.c_double, C.c_void_p]),
+    ("N_VDotProd_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
+    ("N_VMaxNorm_Hip", C.c_double, [C.c_void_p]),
+    ("N_VWrmsNorm_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
+    ("N_VWrmsNormMask_Hip", C.c_double, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("N_VMin_Hip", C.c_double, [C.c_void_p]),
+    ("N_VWL2Norm_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
+    ("N_VL1Norm_Hip", C.c_double, [C.c_void_p]),
+    ("N_VCompare_Hip", None, [C.c_double, C.c_void_p, C.c_void_p]),
+    ("N_VInvTest_Hip", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("N_VConstrMask_Hip", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("N_VMinQuotient_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
+    ("N_VLinearCombination_Hip", C.c_int, [C.c_int, c_double_p, C.POINTER(C.c_void_p), C.c_void_p]),
+    ("N_VDotProdMulti_Hip", C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p), c_double_p]),
+    ("N_VLinearSumWrms_Hip", C.c_double, [C.c_double, C.c_void_p, C.c_double, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
+    ("PihmCVodeCreate", C.c_void_p, [C.c_int, C.c_int]),
+    ("PihmCVodeInit", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    ("PihmCVodeReInit", C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    ("PihmCVodeSStolerances", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    ("PihmCVodeSetUserData", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("PihmCVodeSetInitStep", C.c_int, [C.c_void_p, C.c_double]),
+    ("PihmCVodeSetMaxStep", C.c_int, [C.c_void_p, C.c_double]),
+    ("PihmCVodeSetMinStep", C.c_int, [C.c_void_p, C.c_double]),
+    ("PihmCVodeSetMaxNumSteps", C.c_int, [C.c_void_p, C.c_long]),
+    ("PihmCVodeSetMaxOrd", C.c_int, [C.c_void_p, C.c_int]),
+    ("PihmCVodeSetStopTime", C.c_int, [C.c_void_p, C.c_double]),
+    ("PihmCVodeSetStabLimDet", C.c_int, [C.c_void_p, C.c_int]),
+    ("PihmCVSpgmr", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("PihmCVode", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_double), C.c_int]),
+    ("PihmCVodeGetDky", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
+    ("PihmCVodeFree", None, [C.POINTER(C.c_void_p)]),
+    ("PihmCVodeGetNumSteps", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumRhsEvals", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumLinIters", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumJtimesRhsEvals", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumNonlinSolvIters", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumNonlinSolvConvFails", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetNumErrTestFails", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVodeGetLastStep", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("PihmCVodeGetLastOrder", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("PihmCVodeGetCurrentTime", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("PihmCVodeGetCurrentStep", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("pihm_drive", C.c_int, [C.POINTER(Backend), C.c_void_p, C.POINTER(RunConfig), C.POINTER(RunStats)]),
+    ("pihm_gpu_backend_create", C.c_int, [C.c_void_p, C.POINTER(Forcing), C.POINTER(C.c_void_p)]),
+    ("pihm_gpu_backend_destroy", None, [C.c_void_p]),
+    ("pihm_gpu_backend_get", C.c_int, [C.c_void_p, C.POINTER(Backend)]),
+    ("pihm_run_gpu", C.c_int, [C.c_void_p, C.POINTER(Forcing), C.POINTER(RunConfig), c_double_p,
+                               C.POINTER(RunStats)]),
+]
+
+
+def exported_symbols():
+    """Names the C ABI exports (used by the symbol-table test)."""
+    return [p[0] for p in _PROTOS]
+
+
+def lib():
+    """Load libpihm_amd.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libpihm_amd.so not found at {LIB_PATH}; build it with "
+            "`make -C pihm-cuda_amd` (or __graft_entry__.build()). There is no CPU fallback.")
+    L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    for name, res, args in _PROTOS:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,117 @@
+"""Host-side mirror of the reference RHS interface over the HIP library.
+
+``HipRHS(mesh).ode(y, t)`` has the meaning of the reference's
+``ODE(t, CV_Y, CV_Ydot, pihm)`` (src/ode.c:4-399): y and the returned ydot are in the
+caller's order; the carried river-edge overland flux (read by Infil one call later) lives
+in the device context between calls.  Every evaluation runs the HIP kernels through the
+C ABI (``PIHM_ODE_Hip``); there is no host implementation here.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_lib.c_double_p)
+
+
+class HipRHS:
+    def __init__(self, mesh, nan_check=True, record=False):
+        self.L = _lib.lib()
+        self.mesh = mesh
+        self.desc = mesh.desc()
+        ctx = C.c_void_p()
+        rc = self.L.pihm_ctx_create(C.byref(self.desc), C.byref(ctx))
+        if rc != 0:
+            raise RuntimeError(f"pihm_ctx_create failed ({rc})")
+        self.ctx = ctx
+        self.neq = self.L.pihm_ctx_neq(ctx)
+        self.y = self.L.N_VNew_Hip(self.neq)
+        self.yd = self.L.N_VNew_Hip(self.neq)
+        if not self.y or not self.yd:
+            raise RuntimeError("N_VNew_Hip failed")
+        self.L.pihm_ctx_set_nan_check(ctx, 1 if nan_check else 0)
+        self.L.pihm_ctx_set_record(ctx, 1 if record else 0)
+        self.record = record
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.N_VDestroy_Hip(self.y)
+            self.L.N_VDestroy_Hip(self.yd)
+            self.L.pihm_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def perm(self):
+        p = np.zeros(self.mesh.nelem, np.int32)
+        self.L.pihm_ctx_get_perm(self.ctx, p.ctypes.data_as(_lib.c_int32_p))
+        return p
+
+    def set_step_inputs(self, pcpdrp=None, edir=None, ett=None, ws0_surf=None, elem_bc=None,
+                        river_bc=None):
+        keep = []
+        s = _lib.StepInputs()
+        for name, arr in (("pcpdrp", pcpdrp), ("edir", edir), ("ett", ett), ("ws0_surf", ws0_surf),
+                          ("elem_bc", elem_bc), ("river_bc", river_bc)):
+            if arr is not None:
+                arr = np.ascontiguousarray(arr, dtype=np.float64)
+                keep.append(arr)
+                setattr(s, name, _dp(arr))
+        rc = self.L.pihm_ctx_set_step_inputs(self.ctx, C.byref(s))
+        if rc:
+            raise RuntimeError(f"set_step_inputs failed ({rc})")
+
+    def ode(self, y, t=0.0):
+        """ydot = f(t, y) (caller order).  Returns (rc, ydot); rc 1 = NaN in ydot."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if self.L.pihm_ctx_state_to_device(self.ctx, _dp(y), self.y):
+            raise RuntimeError("state_to_device failed")
+        rc = self.L.PIHM_ODE_Hip(t, self.y, self.yd, self.ctx)
+        out = np.empty(self.neq)
+        if self.L.pihm_ctx_state_from_device(self.ctx, self.yd, _dp(out)):
+            raise RuntimeError("state_from_device failed")
+        return rc, out
+
+    def get_river_edge_ovlflow(self):
+        o = np.zeros(3 * self.mesh.nelem)
+        self.L.pihm_ctx_get_river_edge_ovlflow(self.ctx, _dp(o))
+        return o
+
+    def set_river_edge_ovlflow(self, ovl3):
+        o = np.ascontiguousarray(ovl3, dtype=np.float64)
+        self.L.pihm_ctx_set_river_edge_ovlflow(self.ctx, _dp(o))
+
+    def fluxes(self):
+        ne, nr = self.mesh.nelem, self.mesh.nriver
+        ovl, sub = np.zeros(3 * ne), np.zeros(3 * ne)
+        infil, rechg = np.zeros(ne), np.zeros(ne)
+        rf = np.zeros(11 * max(nr, 1))
+        rc = self.L.pihm_ctx_get_fluxes(self.ctx, _dp(ovl), _dp(sub), _dp(infil), _dp(rechg), _dp(rf))
+        if rc:
+            raise RuntimeError("flux read-back needs record=True")
+        return dict(ovlflow=ovl, subsurf=sub, infil=infil, rechg=rechg, rivflow=rf[:11 * nr])
+
+
+def run_gpu(mesh, forcing, cfg, y0):
+    """Integrate on the device with the model-step driver (pihm_run_gpu); returns (rc, y, stats)."""
+    L = _lib.lib()
+    desc = mesh.desc()
+    ctx = C.c_void_p()
+    rc = L.pihm_ctx_create(C.byref(desc), C.byref(ctx))
+    if rc:
+        raise RuntimeError(f"pihm_ctx_create failed ({rc})")
+    try:
+        y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+        st = _lib.RunStats()
+        fs = forcing.struct()
+        rc = L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), _dp(y), C.byref(st))
+        return rc, y, st
+    finally:
+        L.pihm_ctx_destroy(ctx)

@@ -1,0 +1,60 @@
+"""Free-running integration: the device path (HIP RHS + device N_Vector + integrator,
+driven by the model-step loop) against the same integrator on the oracle RHS and host
+vectors.  The two differ only in reduction order and device pow/log rounding, so the
+trajectories must agree within the solver tolerance (WRMS of the difference, weighted
+with the run's own tolerances, below 1), and two device runs must agree bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pihm_amd.forcing import StormForcing, run_config
+from pihm_amd.rhs import run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _wrms(a, b, rtol=1e-3, atol=1e-4):
+    w = 1.0 / (rtol * np.abs(b) + atol)
+    return math.sqrt(np.mean(((a - b) * w) ** 2))
+
+
+def test_gpu_run_matches_cpu_run(c0):
+    m = c0
+    f = StormForcing(48, seed=5, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    cfg = run_config(6 * 3600.0)
+    print("[run] c0 6 h gpu", flush=True)
+    rc_g, y_g, st_g = run_gpu(m, f, cfg, y0)
+    assert rc_g == 0, st_g.as_dict()
+    rc_c, y_c, st_c = O.OracleRun(m, f, nthreads=1).run(y0, cfg)
+    assert rc_c == 0
+    print("gpu", st_g.as_dict(), "\ncpu", st_c.as_dict(), flush=True)
+    assert _wrms(y_g, y_c) < 1.0
+    assert abs(st_g.nst - st_c.nst) <= 0.1 * st_c.nst
+
+
+def test_gpu_run_deterministic(tree_small):
+    m = tree_small
+    f = StormForcing(24, seed=6, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    cfg = run_config(3600.0)
+    r1 = run_gpu(m, f, cfg, y0)
+    r2 = run_gpu(m, f, cfg, y0)
+    assert r1[0] == 0 and r2[0] == 0
+    assert np.array_equal(r1[1], r2[1])
+    assert r1[2].nst == r2[2].nst and r1[2].nfe == r2[2].nfe
+
+
+def test_gpu_run_full_size_short():
+    from pihm_amd.meshgen import config_mesh
+    m = config_mesh("c1", seed=1)
+    f = StormForcing(24, seed=7, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    cfg = run_config(1800.0)
+    print("[run] c1 30 min gpu", flush=True)
+    rc, y, st = run_gpu(m, f, cfg, y0)
+    print(st.as_dict(), flush=True)
+    assert rc == 0 and np.all(np.isfinite(y))
+    assert st.model_steps == 30
