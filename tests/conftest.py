@@ -46,6 +46,7 @@ def assert_rhs_close(got, ref, mag, rtol=1e-12, what="ydot"):
     err = np.abs(got - ref)
     bad = ~(err <= rtol * scale)
     bad &= ~((err == 0) & (scale == 0))
+    bad &= ~(np.isnan(got) & np.isnan(ref))      # NaN where the reference has NaN
     if bad.any():
         i = np.nonzero(bad)[0]
         raise AssertionError(f"{what}: {len(i)} components outside {rtol} (first {i[:5]}: "

@@ -122,9 +122,11 @@ def test_rhs_parity_branch_variants(kind):
     inp = _inputs(m, rng)
     ne, nr = m.nelem, m.nriver
     ebc = np.where(m.a["bc_type"] > 0, (m.a["zmin"] + 0.7 * m.a["depth"])[:, None], 2e-4).ravel()
-    rbc = np.where(m.a["riverbc_type"] > 0, m.a["zbed"] + 0.4, 0.05)
+    # Dirichlet river heads below the bed: the reference's formula takes the square root
+    # of the head gradient (src/river_flow.c:405-419, :348-362), so only outflow is defined.
+    rbc = np.where(m.a["riverbc_type"] > 0, m.a["zbed"] - 0.5, 0.05)
     if kind.startswith("outlet"):
-        rbc = np.where(m.a["down"] < 0, m.a["zbed"] + 0.2 if kind == "outlet-1" else 0.3, rbc)
+        rbc = np.where(m.a["down"] < 0, m.a["zbed"] - 0.5 if kind == "outlet-1" else 0.3, rbc)
     inp.update(elem_bc=ebc, river_bc=rbc)
     h.set_step_inputs(**inp)
     o.set_step_inputs(**inp)
