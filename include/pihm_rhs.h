@@ -99,6 +99,20 @@ typedef struct pihm_ctx pihm_ctx;
  * (read by Infil one call later) start at zero, as in src/initialize.c:735-743. */
 int  pihm_ctx_create(const pihm_mesh_desc *mesh, pihm_ctx **out);
 void pihm_ctx_destroy(pihm_ctx *ctx);
+
+/* Batch of independent solves (ensemble members, serving slots) in one
+ * context: no counterpart in the reference, which runs one watershed per
+ * process.  Members are concatenated; member k's state is the contiguous
+ * range [yoff_k, yoff_k + neq_k) of a vector of length pihm_ctx_neq, laid
+ * out as a single solve's state.  Host arrays of the other calls are the
+ * members' caller-order arrays concatenated in member order.  One RHS launch
+ * evaluates every active member; members share no data, so each member's
+ * ydot is the same bits as in a context of its own. */
+int  pihm_ctx_create_batch(int nmem, const pihm_mesh_desc *const *meshes, pihm_ctx **out);
+int  pihm_ctx_nmembers(const pihm_ctx *ctx);
+int  pihm_ctx_member_info(const pihm_ctx *ctx, int member, long *yoff, int *neq, int *nelem,
+                          int *nriver);
+int  pihm_ctx_set_active(pihm_ctx *ctx, const int *active);   /* [nmem], stream-ordered */
 int  pihm_ctx_set_stream(pihm_ctx *ctx, void *hip_stream);
 int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
 int  pihm_ctx_neq(const pihm_ctx *ctx);

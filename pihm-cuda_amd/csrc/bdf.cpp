@@ -428,6 +428,127 @@ int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
   }
 }
 
+// ---------------------------------------------------------------- fused (device vectors)
+// SPGMR as above with classical Gram-Schmidt: the inner products of an Arnoldi
+// step come from the same launch that forms the new vector, and the
+// orthogonalisation is applied when the next basis vector is formed
+// (pihm_gm_prep).  The norm of the orthogonalised vector follows from
+// ||w||^2 - sum h_k^2; when that loses more than six digits an explicit
+// second pass (pihm_gm_orth) is made.  The difference-quotient increment is
+// sig = 1/||V_l/s||_WRMS(s) = sqrt(N) for the unit basis vector V_l.
+enum { SP_ZERO = 10 };
+
+int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
+  N_Vector *V = m->V;
+  N_Vector s = m->ewt;
+  const double beta = std::sqrt(sumsq);
+  double r_norm = beta, rho = beta;
+  *krydim_out = 0;
+  if (r_norm <= delta) return SP_ZERO;
+  for (int i = 0; i <= m->maxl; i++)
+    for (int j = 0; j < m->maxl; j++) m->Hes[i][j] = 0.0;
+  double rotation_product = 1.0;
+  double lazy[MAXL_CAP + 1] = {};
+  double cscale = 1.0 / r_norm;
+  const double sig = m->sqrtN;
+  const double siginv = 1.0 / sig;
+  bool converged = false;
+  int krydim = 0;
+  for (int l = 0; l < m->maxl; l++) {
+    krydim = l + 1;
+    if (pihm_gm_prep(l, V, lazy, cscale, m->ycur, s, sig, m->ytemp)) return SP_ATIMES_FAIL_UNREC;
+    const int rv = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
+    m->nfes++;
+    if (rv < 0) return SP_ATIMES_FAIL_UNREC;
+    if (rv > 0) return SP_ATIMES_FAIL_REC;
+    double sums[MAXL_CAP + 2];
+    if (pihm_gm_arnoldi(l, V, s, m->vtemp, m->fcur, siginv, m->gamma, sums)) return SP_ATIMES_FAIL_UNREC;
+    const double ww = sums[l + 1];
+    double hsq = 0.0;
+    for (int k = 0; k <= l; k++) {
+      m->Hes[k][l] = sums[k];
+      lazy[k] = sums[k];
+      hsq += sums[k] * sums[k];
+    }
+    double nn = ww - hsq;
+    if (!(nn > 1e-6 * ww)) {
+      double s2[MAXL_CAP + 2];
+      if (pihm_gm_orth(l, V, lazy, s2)) return SP_GS_FAIL;
+      double hsq2 = 0.0;
+      for (int k = 0; k <= l; k++) {
+        m->Hes[k][l] += s2[k];
+        lazy[k] = s2[k];
+        hsq2 += s2[k] * s2[k];
+      }
+      nn = s2[l + 1] - hsq2;
+      if (nn < 0.0) nn = 0.0;
+    }
+    m->Hes[l + 1][l] = std::sqrt(nn);
+    if (qr_update(m, krydim) != 0) return SP_QRFACT_FAIL;
+    rotation_product *= m->givens[2 * l + 1];
+    rho = std::fabs(rotation_product * r_norm);
+    if (rho <= delta) { converged = true; break; }
+    cscale = 1.0 / m->Hes[l + 1][l];
+  }
+  m->yg[0] = r_norm;
+  for (int i = 1; i <= krydim; i++) m->yg[i] = 0.0;
+  if (qr_solve(m, krydim) != 0) return SP_QRSOL_FAIL;
+  *krydim_out = krydim;
+  if (converged) return SP_SUCCESS;
+  if (rho < beta) return SP_RES_REDUCED;
+  return SP_CONV_FAIL;
+}
+
+int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fused in
+  m->mnewt = 0;
+  int mi = 0;
+  double del = 0.0, delp = 0.0;
+  const double N = (double)m->neq;
+  for (;;) {
+    double sumsq = 0.0;
+    if (pihm_gm_start(m->rl1, m->zn[1], m->acor, m->gamma, m->ftemp, m->ewt, m->tempv, m->V[0], &sumsq))
+      return PIHM_CV_LSOLVE_FAIL;
+    const double deltar = m->eplifac * m->tq[4];
+    const double bnorm = std::sqrt(sumsq / N);
+    int mode = 0, krydim = 0, lret = 0;
+    if (bnorm <= deltar) {
+      mode = (m->mnewt > 0) ? 2 : 1;
+    } else {
+      m->ycur = m->y;
+      m->fcur = m->ftemp;
+      const int sp = spgmr_fused(m, sumsq, deltar * m->sqrtN, &krydim);
+      m->nli += krydim;
+      switch (sp) {
+        case SP_ZERO: mode = 2; break;
+        case SP_SUCCESS: mode = 0; break;
+        case SP_RES_REDUCED: mode = 0; lret = (m->mnewt == 0) ? 0 : 1; break;
+        case SP_CONV_FAIL: case SP_QRFACT_FAIL: case SP_ATIMES_FAIL_REC: lret = 1; break;
+        default: lret = -1; break;
+      }
+    }
+    m->nni++;
+    if (lret < 0) return PIHM_CV_LSOLVE_FAIL;
+    if (lret > 0) return CONV_FAIL;
+    double sums[2];
+    if (pihm_gm_finish(mode, krydim, m->yg, m->V, m->ewt, m->tempv, m->zn[0], m->acor, m->y, sums))
+      return PIHM_CV_LSOLVE_FAIL;
+    del = std::sqrt(sums[0] / N);
+    if (mi > 0) m->crate = std::fmax(CRDOWN * m->crate, del / delp);
+    const double dcon = del * std::fmin(1.0, m->crate) / m->tq[4];
+    if (dcon <= 1.0) {
+      m->acnrm = (mi == 0) ? del : std::sqrt(sums[1] / N);
+      return PIHM_CV_SUCCESS;
+    }
+    m->mnewt = ++mi;
+    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return CONV_FAIL;
+    delp = del;
+    const int fr = m->f(m->tn, m->y, m->ftemp, m->ud);
+    m->nfe++;
+    if (fr < 0) return PIHM_CV_RHSFUNC_FAIL;
+    if (fr > 0) return RHSFUNC_RECVR;
+  }
+}
+
 // ---------------------------------------------------------------- Newton
 int newton_iteration(Mem *m) {   // cvode.c:2729-2832
   m->mnewt = 0;
@@ -471,7 +592,7 @@ int nls_newton(Mem *m) {   // cvode.c:2651-2727 (no setup routine: PREC_NONE)
   if (rv > 0) return RHSFUNC_RECVR;
   VConst(0.0, m->acor);
   VScale(1.0, m->zn[0], m->y);
-  return newton_iteration(m);
+  return m->fused ? newton_iteration_fused(m) : newton_iteration(m);
 }
 
 int handle_nflag(Mem *m, int *nflag, double saved_t, int *ncf) {   // cvode.c:2834-2880

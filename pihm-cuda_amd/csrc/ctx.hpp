@@ -76,6 +76,17 @@ struct RecordDev {      // optional flux recording (device order)
   double *infil, *rechg;
 };
 
+// One solve of a batch: its ranges in the concatenated element, river and
+// state arrays.  A single solve is a batch of one with all offsets zero.
+struct MemberDev {
+  long yoff;            // first state entry (member-major: surf, unsat, gw, stage, river gw)
+  int eoff, ne;         // device element range
+  int roff, nr;         // river range
+  double dt;            // ctrl.stepsize of this solve (Infil)
+  int active;           // evaluated by the next RHS launch
+  int pad;
+};
+
 struct RhsArgs {
   int ne, nr, nbank;
   int surf_mode, riv_mode;
@@ -88,12 +99,26 @@ struct RhsArgs {
   const double *y;
   double *ydot;
   int *nanflag;
+  int nmem;
+  const MemberDev *mem;        // [nmem]
+  const int32_t *emem, *rmem;  // member of each device element / river
 };
 
 }  // namespace pihm
 
+struct pihm_member_host {
+  long yoff;
+  int eoff, ne, roff, nr, neq;
+  double dt;
+};
+
 struct pihm_ctx {
-  int ne, nr, nbank, neq;
+  int ne, nr, nbank, neq;       // totals over the batch
+  int nmem;
+  std::vector<pihm_member_host> members;
+  std::vector<int32_t> emem_host;   // member of each device element
+  std::vector<pihm::MemberDev> mem_host;
+  pihm::MemberDev *mem_dev;
   int surf_mode, riv_mode;
   double dt;
   int device;
@@ -101,8 +126,8 @@ struct pihm_ctx {
   int nan_check;
   int record;
   long long ncalls;
-  std::vector<int32_t> perm;     // device position -> caller index
-  std::vector<int32_t> iperm;    // caller index -> device position
+  std::vector<int32_t> perm;     // device position -> caller index within its member
+  std::vector<int32_t> iperm;    // (member eoff + caller index) -> device position
   std::vector<int32_t> nb_host;  // device-order neighbour codes [3*ne]
   pihm::RhsArgs args;            // pointers filled at creation
   void *arena;                   // one device allocation for everything

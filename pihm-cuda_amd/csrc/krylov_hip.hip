@@ -1,0 +1,209 @@
+// krylov_hip.hip - fused Newton/SPGMR vector work for device vectors.
+//
+// The SPGMR iteration of CVODE (cvode_spgmr.c:355-420, sundials_spgmr.c:165-425)
+// restructured for the device: each Arnoldi step is one element-wise kernel
+// fused with all of its inner products (classical Gram-Schmidt, the
+// orthogonalisation itself applied lazily when the next basis vector is
+// formed), so a Krylov iteration costs one synchronisation instead of one per
+// inner product.  Element-wise arithmetic keeps the operation order of the
+// N_Vector sequence it replaces.
+#include <hip/hip_runtime.h>
+
+#include "redux.hpp"
+
+using namespace pihm;
+
+namespace {
+
+inline double *D(N_Vector v) { return ((N_VectorContent_Hip)v->content)->data; }
+inline long L(N_Vector v) { return ((N_VectorContent_Hip)v->content)->length; }
+inline hipStream_t S(N_Vector v) { return (hipStream_t)((N_VectorContent_Hip)v->content)->stream; }
+
+constexpr int kMaxBasis = 11;   // maxl + 1 basis vectors (maxl <= 10)
+
+struct FStart {
+  double rl1, gamma;
+  const double *zn1, *acor, *f, *s;
+  double *b, *v0;
+  __device__ __forceinline__ void operator()(long i, double (&acc)[1]) const {
+    const double t = (rl1 * zn1[i]) + acor[i];      // LinearSum(rl1, zn1, 1, acor)
+    const double bi = (gamma * f[i]) - t;           // LinearSum(gamma, f, -1, t)
+    b[i] = bi;
+    const double v = s[i] * bi;                     // Prod(s1, vtemp)
+    v0[i] = v;
+    acc[0] += v * v;
+  }
+};
+
+struct PrepArgs {
+  int l;
+  double c, sig;
+  double *vl;
+  const double *vk[kMaxBasis];
+  double coef[kMaxBasis];
+  const double *y, *s;
+  double *work;
+};
+
+__global__ void k_prep(long n, PrepArgs a) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = a.vl[i];
+  for (int k = 0; k < a.l; k++) v = (-a.coef[k] * a.vk[k][i]) + v;
+  v = a.c * v;                                      // Scale(1/h, V_l)
+  a.vl[i] = v;
+  const double vt = v / a.s[i];                     // Div(V_l, s2)
+  a.work[i] = (a.sig * vt) + a.y[i];                // LinearSum(sig, v, 1, y)
+}
+
+template <int NV>
+struct FArnoldi {
+  const double *vl, *s, *jf, *fy;
+  double siginv, gamma;
+  double *w;
+  const double *vk[kMaxBasis];
+  __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const {
+    const double si = s[i];
+    const double vt = vl[i] / si;
+    const double jv = siginv * (jf[i] - fy[i]);     // LinearSum(siginv, Jv, -siginv, fy)
+    const double z = (-gamma * jv) + vt;            // LinearSum(1, v, -gamma, Jv)
+    const double wi = si * z;                       // Prod(s1, z)
+    w[i] = wi;
+#pragma unroll
+    for (int k = 0; k < NV - 1; k++) acc[k] += vk[k][i] * wi;
+    acc[NV - 1] += wi * wi;
+  }
+};
+
+template <int NV>
+struct FOrth {
+  double *w;
+  const double *vk[kMaxBasis];
+  double coef[kMaxBasis];
+  __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const {
+    double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < NV - 1; k++) wi = (-coef[k] * vk[k][i]) + wi;
+    w[i] = wi;
+#pragma unroll
+    for (int k = 0; k < NV - 1; k++) acc[k] += vk[k][i] * wi;
+    acc[NV - 1] += wi * wi;
+  }
+};
+
+struct FFinish {
+  int mode, k;
+  double yc[kMaxBasis];
+  const double *vk[kMaxBasis];
+  const double *s, *b, *zn0;
+  double *acor, *y;
+  __device__ __forceinline__ void operator()(long i, double (&acc)[2]) const {
+    double x;
+    if (mode == 0) {
+      double xc = 0.0;
+      for (int j = 0; j < k; j++) xc += yc[j] * vk[j][i];   // LinearSum(yg, V, 1, xcor, xcor)
+      xc = xc / s[i];                                        // Div(xcor, s2)
+      x = 0.0 + xc;                                          // x = 0 + vtemp
+    } else if (mode == 1) {
+      x = b[i];
+    } else {
+      x = 0.0;
+    }
+    const double si = s[i];
+    const double ac = acor[i] + x;                           // LinearSum(1, acor, 1, b, acor)
+    acor[i] = ac;
+    y[i] = zn0[i] + ac;                                      // LinearSum(1, zn0, 1, acor, y)
+    const double p = x * si, q = ac * si;
+    acc[0] += p * p;
+    acc[1] += q * q;
+  }
+};
+
+template <int NV>
+int arnoldi_n(N_Vector *V, int l, N_Vector s, N_Vector jf, N_Vector fy, double siginv, double gamma,
+              double *sums) {
+  FArnoldi<NV> f;
+  f.vl = D(V[l]); f.s = D(s); f.jf = D(jf); f.fy = D(fy);
+  f.siginv = siginv; f.gamma = gamma;
+  f.w = D(V[l + 1]);
+  for (int k = 0; k <= l; k++) f.vk[k] = D(V[k]);
+  return fused_sum<NV>(L(s), f, pihm_nv_redwork(s), S(s), sums);
+}
+
+template <int NV>
+int orth_n(N_Vector *V, int l, const double *coef, double *sums) {
+  FOrth<NV> f;
+  f.w = D(V[l + 1]);
+  for (int k = 0; k <= l; k++) { f.vk[k] = D(V[k]); f.coef[k] = coef[k]; }
+  return fused_sum<NV>(L(V[0]), f, pihm_nv_redwork(V[0]), S(V[0]), sums);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f, N_Vector s,
+                  N_Vector b, N_Vector v0, double *sumsq) {
+  FStart k{rl1, gamma, D(zn1), D(acor), D(f), D(s), D(b), D(v0)};
+  return fused_sum<1>(L(s), k, pihm_nv_redwork(s), S(s), sumsq);
+}
+
+int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s, double sig,
+                 N_Vector work) {
+  if (l < 0 || l >= kMaxBasis) return -1;
+  PrepArgs a;
+  a.l = l; a.c = c; a.sig = sig;
+  a.vl = D(V[l]);
+  for (int k = 0; k < l; k++) { a.vk[k] = D(V[k]); a.coef[k] = coef[k]; }
+  a.y = D(y); a.s = D(s); a.work = D(work);
+  const long n = L(s);
+  hipLaunchKernelGGL(k_prep, dim3(div_up(n, kBlock)), dim3(kBlock), 0, S(s), n, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
+                    double gamma, double *sums) {
+  switch (l + 2) {
+    case 2: return arnoldi_n<2>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 3: return arnoldi_n<3>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 4: return arnoldi_n<4>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 5: return arnoldi_n<5>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 6: return arnoldi_n<6>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 7: return arnoldi_n<7>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 8: return arnoldi_n<8>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 9: return arnoldi_n<9>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 10: return arnoldi_n<10>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 11: return arnoldi_n<11>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 12: return arnoldi_n<12>(V, l, s, jf, fy, siginv, gamma, sums);
+    default: return -1;
+  }
+}
+
+int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums) {
+  switch (l + 2) {
+    case 2: return orth_n<2>(V, l, coef, sums);
+    case 3: return orth_n<3>(V, l, coef, sums);
+    case 4: return orth_n<4>(V, l, coef, sums);
+    case 5: return orth_n<5>(V, l, coef, sums);
+    case 6: return orth_n<6>(V, l, coef, sums);
+    case 7: return orth_n<7>(V, l, coef, sums);
+    case 8: return orth_n<8>(V, l, coef, sums);
+    case 9: return orth_n<9>(V, l, coef, sums);
+    case 10: return orth_n<10>(V, l, coef, sums);
+    case 11: return orth_n<11>(V, l, coef, sums);
+    case 12: return orth_n<12>(V, l, coef, sums);
+    default: return -1;
+  }
+}
+
+int pihm_gm_finish(int mode, int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector b,
+                   N_Vector zn0, N_Vector acor, N_Vector y, double *sums) {
+  if (k < 0 || k > kMaxBasis) return -1;
+  FFinish f;
+  f.mode = mode; f.k = k;
+  for (int j = 0; j < k; j++) { f.yc[j] = ycoef[j]; f.vk[j] = D(V[j]); }
+  f.s = D(s); f.b = D(b); f.zn0 = D(zn0); f.acor = D(acor); f.y = D(y);
+  return fused_sum<2>(L(s), f, pihm_nv_redwork(s), S(s), sums);
+}
+
+}  // extern "C"

@@ -25,14 +25,10 @@ using namespace pihm;
 namespace {
 
 constexpr int kRedBlock = 256;
-constexpr int kMaxRedBlocks = 1024;
-constexpr int kMaxMulti = 16;
+constexpr int kMaxRedBlocks = PIHM_RED_MAXBLOCKS;
+constexpr int kMaxMulti = PIHM_RED_MAXSUMS;
 
-struct RedWork {
-  double *partials;   // device [kMaxRedBlocks * kMaxMulti]
-  unsigned *ticket;   // device counter
-  double *result;     // pinned host [kMaxMulti]
-};
+using RedWork = pihm_redwork;
 
 // One workspace per device vector family (clones share the parent's).
 struct Aux {
@@ -48,7 +44,7 @@ inline Aux *X(N_Vector v) { return (Aux *)C(v)->aux; }
 
 inline int ew_blocks(long n) { return (int)std::max<long>(1, (n + kBlock - 1) / kBlock); }
 inline int red_blocks(long n) {
-  long b = (n + 8L * kRedBlock - 1) / (8L * kRedBlock);
+  long b = (n + 2L * kRedBlock - 1) / (2L * kRedBlock);
   return (int)std::min<long>(std::max<long>(b, 1), kMaxRedBlocks);
 }
 
@@ -204,18 +200,33 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(RedArgs a) {
 // fused ops ------------------------------------------------------------------
 struct ZnPtrs { double *p[7]; };
 
-__global__ void k_predict(long n, int q, ZnPtrs z, int sign) {
+template <int Q>
+__global__ void k_predict(long n, ZnPtrs z, int sign) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double v[7];
+  double v[Q + 1];
 #pragma unroll
-  for (int j = 0; j < 7; j++) if (j <= q) v[j] = z.p[j][i];
-  for (int k = 1; k <= q; k++)
-    for (int j = q; j >= k; j--) {
+  for (int j = 0; j <= Q; j++) v[j] = z.p[j][i];
+#pragma unroll
+  for (int k = 1; k <= Q; k++)
+#pragma unroll
+    for (int j = Q; j >= k; j--) {
       if (sign > 0) v[j - 1] = v[j - 1] + v[j];   // LinearSum(1, zn[j-1], 1, zn[j]) == x+y
       else v[j - 1] = v[j - 1] - v[j];            // LinearSum(1, zn[j-1], -1, zn[j]) == x-y
     }
-  for (int j = 0; j < q; j++) z.p[j][i] = v[j];
+#pragma unroll
+  for (int j = 0; j < Q; j++) z.p[j][i] = v[j];
+}
+
+void launch_predict(long n, int q, ZnPtrs z, int sign, hipStream_t st) {
+  const dim3 g(ew_blocks(n)), b(kBlock);
+  switch (q) {
+    case 1: hipLaunchKernelGGL(k_predict<1>, g, b, 0, st, n, z, sign); break;
+    case 2: hipLaunchKernelGGL(k_predict<2>, g, b, 0, st, n, z, sign); break;
+    case 3: hipLaunchKernelGGL(k_predict<3>, g, b, 0, st, n, z, sign); break;
+    case 4: hipLaunchKernelGGL(k_predict<4>, g, b, 0, st, n, z, sign); break;
+    default: hipLaunchKernelGGL(k_predict<5>, g, b, 0, st, n, z, sign); break;
+  }
 }
 
 __global__ void k_rescale(long n, int q, ZnPtrs z, double f1, double f2, double f3, double f4, double f5) {
@@ -357,6 +368,9 @@ N_Vector make_vec(long length, double *data, int own, void *stream, Aux *aux) {
 extern "C" {
 
 N_Vector_ID N_VGetVectorID_Hip(N_Vector v) { (void)v; return PIHM_NVEC_HIP; }
+
+pihm_redwork pihm_nv_redwork(N_Vector v) { return X(v)->w; }
+void *pihm_nv_stream(N_Vector v) { return C(v)->stream; }
 
 N_Vector N_VNewEmpty_Hip(long int length) {
   Aux *aux = nullptr;
@@ -553,14 +567,14 @@ int pihm_nv_predict(int q, N_Vector *zn) {
   ZnPtrs z;
   for (int j = 0; j <= q; j++) z.p[j] = D(zn[j]);
   const long n = L(zn[0]);
-  hipLaunchKernelGGL(k_predict, dim3(ew_blocks(n)), dim3(kBlock), 0, S(zn[0]), n, q, z, 1);
+  launch_predict(n, q, z, 1, S(zn[0]));
   return 0;
 }
 int pihm_nv_restore(int q, N_Vector *zn) {
   ZnPtrs z;
   for (int j = 0; j <= q; j++) z.p[j] = D(zn[j]);
   const long n = L(zn[0]);
-  hipLaunchKernelGGL(k_predict, dim3(ew_blocks(n)), dim3(kBlock), 0, S(zn[0]), n, q, z, -1);
+  launch_predict(n, q, z, -1, S(zn[0]));
   return 0;
 }
 int pihm_nv_rescale(int q, const double *f, N_Vector *zn) {

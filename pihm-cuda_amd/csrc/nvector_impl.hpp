@@ -6,6 +6,39 @@
 extern "C" {
 #endif
 
+// Reduction workspace shared by a family of device vectors (clones share it).
+typedef struct pihm_redwork {
+  double *partials;   // device, [PIHM_RED_MAXBLOCKS * PIHM_RED_MAXSUMS]
+  unsigned *ticket;   // device counter, zero between launches
+  double *result;     // pinned host, [PIHM_RED_MAXSUMS]
+} pihm_redwork;
+#define PIHM_RED_MAXBLOCKS 1024
+#define PIHM_RED_MAXSUMS 16
+pihm_redwork pihm_nv_redwork(N_Vector v);
+void *pihm_nv_stream(N_Vector v);
+
+// Fused Newton / SPGMR operations (krylov_hip.hip).  All vectors are device
+// vectors of one family; s is the error-weight vector (the SPGMR scalings
+// s1 = s2 = ewt of cvode_spgmr.c:396-398).  Sums are returned through the
+// family's pinned result buffer after one stream synchronisation.
+//   start:   b = gamma*f - (rl1*zn1 + acor); v0 = s*b;  returns sum (s*b)^2
+//   prep:    V_l <- c * (V_l - sum_{k<l} coef[k] V_k)  (in place, l>0),
+//            V_0 <- c * V_0 (l=0);  work = sig*(V_l/s) + y
+//   arnoldi: w = s*((-gamma*(siginv*(jf - fy))) + V_l/s) into V_{l+1};
+//            sums[k] = <V_k, w> (k<=l), sums[l+1] = <w, w>
+//   orth:    V_{l+1} <- V_{l+1} - sum_{k<=l} coef[k] V_k; sums as arnoldi
+//   finish:  x = (sum_k ycoef[k] V_k)/s (mode 0) | b (mode 1) | 0 (mode 2);
+//            acor += x; y = zn0 + acor; sums: (x*s)^2, (acor*s)^2
+int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f,
+                  N_Vector s, N_Vector b, N_Vector v0, double *sumsq);
+int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s,
+                 double sig, N_Vector work);
+int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
+                    double gamma, double *sums);
+int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums);
+int pihm_gm_finish(int mode, int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector b,
+                   N_Vector zn0, N_Vector acor, N_Vector y, double *sums);
+
 // Fused Nordsieck-array updates used by the integrator (one launch each).
 // zn holds q+1 (or more) device vectors of equal length.
 //   predict:  for k=1..q, j=q..k: zn[j-1] += zn[j]        (cvode.c:2277-2306)

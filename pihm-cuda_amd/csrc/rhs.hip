@@ -40,15 +40,41 @@ using namespace pihm;
 // kernels
 // ---------------------------------------------------------------------------
 
+// Where element/river entries of one solve live in the state vector.  A batch
+// concatenates solves (member-major state, contiguous element and river
+// ranges); a single solve (B = false) has all offsets zero.
+struct Ix {
+  long yb;        // first state entry of the solve
+  int ne, eoff;   // element count and first device element
+  int nr, roff;   // river count and first river
+  double dt;
+  int active;
+};
+
+template <bool B>
+__device__ __forceinline__ Ix ix_of(const RhsArgs &a, int member) {
+  if (!B) return Ix{0, a.ne, 0, a.nr, 0, a.dt, 1};
+  const MemberDev M = a.mem[member];
+  return Ix{M.yoff, M.ne, M.eoff, M.nr, M.roff, M.dt, M.active};
+}
+template <bool B>
+__device__ __forceinline__ Ix ix_elem(const RhsArgs &a, int p) { return ix_of<B>(a, B ? a.emem[p] : 0); }
+template <bool B>
+__device__ __forceinline__ Ix ix_river(const RhsArgs &a, int r) { return ix_of<B>(a, B ? a.rmem[r] : 0); }
+
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
   const int p = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int ne = a.ne;
   if (p >= ne) return;
-  const double *y = a.y;
+  const Ix ix = ix_elem<B>(a, p);
+  if (!ix.active) return;
+  const double *y = a.y + ix.yb;      // this solve's state
+  const int lp = p - ix.eoff;
   const ElemDev &e = a.e;
 
-  const double s = clamp0(y[p]);
-  const double g = clamp0(y[2 * ne + p]);
+  const double s = clamp0(y[lp]);
+  const double g = clamp0(y[2 * ix.ne + lp]);
   const double surfh = surf_h(s);
   const double zmax = e.zmax[p];
   const double head = zmax + surfh;
@@ -63,12 +89,12 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
     for (int j = 0; j < 3; j++) {
       const int c = e.nb[j * ne + p];
       if (c >= 0) {
-        h[j] = e.zmax[c] + surf_h(clamp0(y[c]));
+        h[j] = e.zmax[c] + surf_h(clamp0(y[c - ix.eoff]));
       } else if (c == NB_BOUND) {
         h[j] = (e.bc_type[j * ne + p] == 0) ? zmax + surfh : e.ebc[j * ne + p];
       } else {
         const int r = (-c - 1) >> 1;
-        const double st = clamp0(y[3 * ne + r]);
+        const double st = clamp0(y[3 * ix.ne + (r - ix.roff)]);
         h[j] = (st > a.r.depth[r]) ? a.r.zbed[r] + st : a.r.zmax[r];
       }
     }
@@ -87,12 +113,17 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
   rp->sfmag = sfmag;
 }
 
+template <bool B>
 __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const int ne = a.ne;
   const ElemDev &e = a.e;
-  const double *y = a.y;
+  const Ix ix = ix_elem<B>(a, p);
+  if (!ix.active) return;
+  const int lp = p - ix.eoff;
+  const double *y = a.y + ix.yb;
+  double *ydot = a.ydot + ix.yb;
   const NbrRec me = e.rec[p];
-  const double u = clamp0(y[ne + p]);
+  const double u = clamp0(y[ix.ne + lp]);
   const double g = me.gw;
   const double surfh = me.surfh;
   const double area = e.area[p], zmax = e.zmax[p], zmin = e.zmin[p];
@@ -204,7 +235,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
       infil = (infil > 0.0) ? infil : 0.0;
     }
     const double ws0 = e.ws0s[p];
-    const double infil_max = applrate + ((ws0 > 0.0) ? ws0 / a.dt : 0.0);
+    const double infil_max = applrate + ((ws0 > 0.0) ? ws0 / ix.dt : 0.0);
     infil = (infil > infil_max) ? infil_max : infil;
     infil *= wetfrac;
   }
@@ -237,7 +268,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   dyu += infil - rechg - edir_u - ett_u;
   dyg += rechg - edir_g - ett_g;
   dyu /= e.porosity[p];
-  a.ydot[ne + p] = dyu;
+  ydot[ix.ne + lp] = dyu;
   bool nan = isnan(dyu);
   if (!bank) {
 #pragma unroll
@@ -246,8 +277,8 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
       dyg -= sub[j] / area;
     }
     dyg /= e.porosity[p];
-    a.ydot[p] = dys;
-    a.ydot[2 * ne + p] = dyg;
+    ydot[lp] = dys;
+    ydot[2 * ix.ne + lp] = dyg;
     nan = nan || isnan(dys) || isnan(dyg);
   } else {
     BankRec *b = &e.bank[e.bank_id[p]];
@@ -259,12 +290,16 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   if (nan) *a.nanflag = 1;
 }
 
+template <bool B>
 __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
-  const int ne = a.ne, nr = a.nr;
+  const int nr = a.nr;
   const RiverDev &R = a.r;
-  const double *y = a.y;
-  const double stage = clamp0(y[3 * ne + r]);
-  const double rgw = clamp0(y[3 * ne + nr + r]);
+  const Ix ix = ix_river<B>(a, r);
+  if (!ix.active) return;
+  const double *ys = a.y + ix.yb + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
+  const double *yg = ys + ix.nr;                           // yg[r']: river groundwater
+  const double stage = clamp0(ys[r]);
+  const double rgw = clamp0(yg[r]);
   const int ord = R.ord[r];
   const double coeff = R.coeff[r], zbed = R.zbed[r], length = R.length[r];
   const double rzmin = R.zmin[r];
@@ -295,7 +330,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
     }
     // ChanFlowRiverToRiver: src/river_flow.c:257-300
     {
-      const double dstage = clamp0(y[3 * ne + d]);
+      const double dstage = clamp0(ys[d]);
       const double total_h = stage + zbed;
       const double perim = riv_perim(ord, stage, coeff);
       const double total_h_down = dstage + R.zbed[d];
@@ -316,7 +351,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
     {
       const double effk = 0.5 * (recL.effkh + recR.effkh);
       const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
-      const double dgw = clamp0(y[3 * ne + nr + d]);
+      const double dgw = clamp0(yg[d]);
       const double total_h = rgw + rzmin;
       const double total_h_down = dgw + R.zmin[d];
       const double avg_wid = (R.width[r] + R.width[d]) / 2.0;
@@ -425,22 +460,26 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   R.bank_sub[2 * r + 1] = -(rf5 + rf8);
 }
 
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
   if ((int)blockIdx.x < nbe) {
     const int p = xcd_tile(blockIdx.x, nbe) * kBlock + threadIdx.x;
-    if (p < a.ne) elem_flux(a, p);
+    if (p < a.ne) elem_flux<B>(a, p);
   } else {
     const int r = (blockIdx.x - nbe) * kBlock + threadIdx.x;
-    if (r < a.nr) river_flux(a, r);
+    if (r < a.nr) river_flux<B>(a, r);
   }
 }
 
+template <bool B>
 __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int ne = a.ne, nr = a.nr;
   const RiverDev &R = a.r;
   if (t < nr) {
     const int r = t;
+    const Ix ix = ix_river<B>(a, r);
+    if (!ix.active) return;
     // src/river_flow.c:96-110 in ascending upstream index
     double up = R.upinit[r], upa = 0.0;
     for (int k = R.up_ptr[r]; k < R.up_ptr[r + 1]; k++) {
@@ -458,8 +497,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
     double dg = 0.0;
     dg += -R.rf[7 * nr + r] - R.rf[8 * nr + r] - R.rf[9 * nr + r] - upa + rf6;
     dg /= R.porosity[r] * area;
-    a.ydot[3 * ne + r] = ds;
-    a.ydot[3 * ne + nr + r] = dg;
+    double *ydr = a.ydot + ix.yb + 3L * ix.ne - ix.roff;
+    ydr[r] = ds;
+    ydr[ix.nr + r] = dg;
     if (a.record) {
       R.rf[0 * nr + r] = up;
       R.rf[10 * nr + r] = upa;
@@ -468,6 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
   } else if (t < nr + a.nbank) {
     const int k = t - nr;
     const int p = a.e.bank_elem[k];
+    const Ix ix = ix_elem<B>(a, p);
+    if (!ix.active) return;
     const BankRec b = a.e.bank[k];
     const double area = a.e.area[p];
     double dys = b.dys0, dyg = b.dyg0;
@@ -492,23 +534,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
       dyg -= s / area;
     }
     dyg /= a.e.porosity[p];
-    a.ydot[p] = dys;
-    a.ydot[2 * ne + p] = dyg;
+    double *yde = a.ydot + ix.yb + (p - ix.eoff);
+    yde[0] = dys;
+    yde[2 * ix.ne] = dyg;
     if (isnan(dys) || isnan(dyg)) *a.nanflag = 1;
   }
-}
-
-__global__ void k_permute_state(const double *src, double *dst, const int32_t *perm, int ne, int nr,
-                                int to_device) {
-  // to_device: dst[b*ne+p] = src[b*ne+perm[p]]; otherwise the inverse.
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long n = 3L * ne + 2L * nr;
-  if (i >= n) return;
-  if (i >= 3L * ne) { dst[i] = src[i]; return; }
-  const long b = i / ne, p = i % ne;
-  const long q = b * ne + perm[p];
-  if (to_device) dst[i] = src[q];
-  else dst[q] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -596,75 +626,109 @@ int validate(const pihm_mesh_desc *m) {
 
 }  // namespace
 
-extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
-  if (!out) return PIHM_ERR_ARG;
-  *out = nullptr;
-  int rc = validate(m);
-  if (rc != PIHM_OK) return rc;
-  const int ne = m->nelem, nr = m->nriver;
+namespace {
 
+// Build a context for one or more solves (members).  Arrays of all members
+// are concatenated; each member's elements are renumbered along the Hilbert
+// curve of its own centroids and its state occupies one contiguous range.
+int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
+  *out = nullptr;
+  const int nmem = (int)ms.size();
+  if (nmem < 1) return PIHM_ERR_ARG;
+  for (auto m : ms) {
+    int rc = validate(m);
+    if (rc != PIHM_OK) return rc;
+    if (m->surf_mode != ms[0]->surf_mode || m->riv_mode != ms[0]->riv_mode) return PIHM_ERR_ARG;
+  }
   pihm_ctx *c = new pihm_ctx();
-  c->ne = ne; c->nr = nr; c->neq = 3 * ne + 2 * nr;
-  c->surf_mode = m->surf_mode; c->riv_mode = m->riv_mode; c->dt = m->stepsize;
+  c->nmem = nmem;
+  c->members.resize(nmem);
+  long ne = 0, nr = 0, neq = 0;
+  for (int k = 0; k < nmem; k++) {
+    auto &M = c->members[k];
+    M.eoff = (int)ne; M.ne = ms[k]->nelem;
+    M.roff = (int)nr; M.nr = ms[k]->nriver;
+    M.yoff = neq; M.neq = 3 * M.ne + 2 * M.nr;
+    M.dt = ms[k]->stepsize;
+    ne += M.ne; nr += M.nr; neq += M.neq;
+  }
+  if (ne > INT32_MAX / 4 || nr > INT32_MAX / 16) { delete c; return PIHM_ERR_TOO_LARGE; }
+  c->ne = (int)ne; c->nr = (int)nr; c->neq = (int)neq;
+  c->surf_mode = ms[0]->surf_mode; c->riv_mode = ms[0]->riv_mode; c->dt = ms[0]->stepsize;
   c->stream = nullptr; c->nan_check = 1; c->record = 0; c->ncalls = 0;
   PIHM_HIP_TRY(hipGetDevice(&c->device));
 
-  // Hilbert order of centroids
-  double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
-  for (int i = 1; i < ne; i++) {
-    xmin = std::min(xmin, m->x[i]); xmax = std::max(xmax, m->x[i]);
-    ymin = std::min(ymin, m->y[i]); ymax = std::max(ymax, m->y[i]);
-  }
-  const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
-  std::vector<uint64_t> key(ne);
-  for (int i = 0; i < ne; i++) {
-    const double fx = (m->x[i] - xmin) / span, fy = (m->y[i] - ymin) / span;
-    const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fx * 65535.0)));
-    const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fy * 65535.0)));
-    key[i] = hilbert_key(qx, qy);
-  }
+  // per-member Hilbert order
   c->perm.resize(ne);
-  std::iota(c->perm.begin(), c->perm.end(), 0);
-  std::stable_sort(c->perm.begin(), c->perm.end(),
-                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
   c->iperm.resize(ne);
-  for (int p = 0; p < ne; p++) c->iperm[c->perm[p]] = p;
+  c->emem_host.resize(ne);
+  for (int k = 0; k < nmem; k++) {
+    const pihm_mesh_desc *m = ms[k];
+    const int off = c->members[k].eoff, n = m->nelem;
+    double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
+    for (int i = 1; i < n; i++) {
+      xmin = std::min(xmin, m->x[i]); xmax = std::max(xmax, m->x[i]);
+      ymin = std::min(ymin, m->y[i]); ymax = std::max(ymax, m->y[i]);
+    }
+    const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; i++) {
+      const double fx = (m->x[i] - xmin) / span, fy = (m->y[i] - ymin) / span;
+      const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fx * 65535.0)));
+      const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fy * 65535.0)));
+      key[i] = hilbert_key(qx, qy);
+    }
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    for (int lp = 0; lp < n; lp++) {
+      c->perm[off + lp] = order[lp];
+      c->iperm[off + order[lp]] = off + lp;
+      c->emem_host[off + lp] = k;
+    }
+  }
 
-  // neighbour codes and bank records
+  // neighbour codes, bank records, upstream lists (global indices)
   c->nb_host.assign(3L * ne, 0);
   std::vector<int32_t> bank_id(ne, -1), bank_elem;
-  for (int p = 0; p < ne; p++) {
-    const int i = c->perm[p];
-    bool bank = false;
-    for (int j = 0; j < 3; j++) {
-      const int v = m->nabr[3 * i + j];
-      int32_t code;
-      if (v > 0) code = c->iperm[v - 1];
-      else if (v == 0) code = NB_BOUND;
-      else {
-        const int r = -v - 1;
-        const int side = (m->leftele[r] == i + 1) ? 0 : 1;
-        code = -(2 * r + side + 1);
-        bank = true;
-      }
-      c->nb_host[(long)j * ne + p] = code;
-    }
-    if (bank) { bank_id[p] = (int32_t)bank_elem.size(); bank_elem.push_back(p); }
-  }
-  // The reference replaces the first edge of each bank tagged with the river
-  // (src/river_flow.c:161-182); a bank with two edges tagged by the same
-  // river cannot occur in a valid mesh and was rejected above only if untagged.
-  c->nbank = (int)bank_elem.size();
-
-  // upstream lists in ascending caller index
+  std::vector<int32_t> rmem(std::max<long>(nr, 1), 0);
   std::vector<int32_t> up_ptr(nr + 1, 0), up_idx;
-  for (int r = 0; r < nr; r++) if (m->down[r] > 0) up_ptr[m->down[r]]++;
-  for (int r = 0; r < nr; r++) up_ptr[r + 1] += up_ptr[r];
+  for (int k = 0; k < nmem; k++) {
+    const pihm_mesh_desc *m = ms[k];
+    const auto &M = c->members[k];
+    for (int lp = 0; lp < M.ne; lp++) {
+      const int p = M.eoff + lp, i = c->perm[p];
+      bool bank = false;
+      for (int j = 0; j < 3; j++) {
+        const int v = m->nabr[3 * i + j];
+        int32_t code;
+        if (v > 0) code = c->iperm[M.eoff + v - 1];
+        else if (v == 0) code = NB_BOUND;
+        else {
+          const int r = -v - 1;
+          const int side = (m->leftele[r] == i + 1) ? 0 : 1;
+          code = -(2 * (M.roff + r) + side + 1);
+          bank = true;
+        }
+        c->nb_host[(long)j * ne + p] = code;
+      }
+      if (bank) { bank_id[p] = (int32_t)bank_elem.size(); bank_elem.push_back(p); }
+    }
+    for (int r = 0; r < M.nr; r++) {
+      rmem[M.roff + r] = k;
+      if (m->down[r] > 0) up_ptr[M.roff + m->down[r]]++;
+    }
+  }
+  c->nbank = (int)bank_elem.size();
+  for (long r = 0; r < nr; r++) up_ptr[r + 1] += up_ptr[r];
   up_idx.assign(up_ptr[nr] > 0 ? up_ptr[nr] : 1, 0);
   {
     std::vector<int32_t> fill(up_ptr.begin(), up_ptr.end() - 1);
-    for (int r = 0; r < nr; r++)
-      if (m->down[r] > 0) up_idx[fill[m->down[r] - 1]++] = r;
+    for (int k = 0; k < nmem; k++) {
+      const auto &M = c->members[k];
+      for (int r = 0; r < M.nr; r++)   // ascending caller index within the member
+        if (ms[k]->down[r] > 0) up_idx[fill[M.roff + ms[k]->down[r] - 1]++] = M.roff + r;
+    }
   }
 
   // device allocation
@@ -684,7 +748,7 @@ extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
   ar.add(&E.rec, ne);
   ar.add(&E.bank, std::max(c->nbank, 1));
   ar.add(&E.bank_elem, std::max(c->nbank, 1));
-  const int nr1 = std::max(nr, 1);
+  const long nr1 = std::max<long>(nr, 1);
   int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord};
   for (auto p : ri) ar.add(p, nr1);
   ar.add(&R.up_ptr, nr + 1);
@@ -701,6 +765,10 @@ extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
   ar.add(&A.rec.infil, ne);
   ar.add(&A.rec.rechg, ne);
   ar.add(&c->nanflag_dev, 1);
+  ar.add(&c->mem_dev, nmem);
+  int32_t *emem_dev = nullptr, *rmem_dev = nullptr;
+  ar.add(&emem_dev, ne);
+  ar.add(&rmem_dev, nr1);
   c->arena_bytes = ar.total();
   PIHM_HIP_TRY(hipMalloc(&c->arena, c->arena_bytes));
   PIHM_HIP_TRY(hipMemset(c->arena, 0, c->arena_bytes));
@@ -708,73 +776,149 @@ extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
   PIHM_HIP_TRY(hipHostMalloc((void **)&c->nanflag_host, sizeof(int), hipHostMallocDefault));
   *c->nanflag_host = 0;
 
-  // upload element statics in device order
+  // element statics in device order
   std::vector<double> hd(3L * ne);
-  auto up1 = [&](double *dst, const double *src) -> hipError_t {
-    for (int p = 0; p < ne; p++) hd[p] = src[c->perm[p]];
-    return hipMemcpy(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice);
+  auto cat1 = [&](int field) {
+    for (int k = 0; k < nmem; k++) {
+      const pihm_mesh_desc *m = ms[k];
+      const double *es[] = {m->area, m->zmin, m->zmax, m->depth, m->ksath, m->ksatv, m->kinfv,
+                            m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
+                            m->areafv, m->areafh, m->rough, m->rzd};
+      const auto &M = c->members[k];
+      for (int lp = 0; lp < M.ne; lp++) hd[M.eoff + lp] = es[field][c->perm[M.eoff + lp]];
+    }
   };
-  auto up3 = [&](double *dst, const double *src) -> hipError_t {
-    for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) hd[(long)j * ne + p] = src[3L * c->perm[p] + j];
-    return hipMemcpy(dst, hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice);
+  for (int f = 0; f < 18; f++) {
+    cat1(f);
+    PIHM_HIP_TRY(hipMemcpy(*ed[f], hd.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+  }
+  auto cat3 = [&](int field) {
+    for (int k = 0; k < nmem; k++) {
+      const pihm_mesh_desc *m = ms[k];
+      const double *es[] = {m->edge, m->nabrdist, m->nabr_x, m->nabr_y};
+      const auto &M = c->members[k];
+      for (int j = 0; j < 3; j++)
+        for (int lp = 0; lp < M.ne; lp++)
+          hd[(long)j * ne + M.eoff + lp] = es[field][3L * c->perm[M.eoff + lp] + j];
+    }
   };
-  const double *es[] = {m->area, m->zmin, m->zmax, m->depth, m->ksath, m->ksatv, m->kinfv,
-                        m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
-                        m->areafv, m->areafh, m->rough, m->rzd};
-  for (int k = 0; k < 18; k++) PIHM_HIP_TRY(up1(*ed[k], es[k]));
-  PIHM_HIP_TRY(up3(E.edge, m->edge));
-  PIHM_HIP_TRY(up3(E.nabrdist, m->nabrdist));
-  PIHM_HIP_TRY(up3(E.nabr_x, m->nabr_x));
-  PIHM_HIP_TRY(up3(E.nabr_y, m->nabr_y));
+  double *e3d[] = {E.edge, E.nabrdist, E.nabr_x, E.nabr_y};
+  for (int f = 0; f < 4; f++) {
+    cat3(f);
+    PIHM_HIP_TRY(hipMemcpy(e3d[f], hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice));
+  }
   {
     std::vector<int32_t> bt(3L * ne);
-    for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) bt[(long)j * ne + p] = m->bc_type[3L * c->perm[p] + j];
+    for (int k = 0; k < nmem; k++) {
+      const auto &M = c->members[k];
+      for (int j = 0; j < 3; j++)
+        for (int lp = 0; lp < M.ne; lp++)
+          bt[(long)j * ne + M.eoff + lp] = ms[k]->bc_type[3L * c->perm[M.eoff + lp] + j];
+    }
     PIHM_HIP_TRY(hipMemcpy(E.bc_type, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
     PIHM_HIP_TRY(hipMemcpy(E.nb, c->nb_host.data(), c->nb_host.size() * 4, hipMemcpyHostToDevice));
     PIHM_HIP_TRY(hipMemcpy(E.bank_id, bank_id.data(), ne * 4, hipMemcpyHostToDevice));
     if (c->nbank)
       PIHM_HIP_TRY(hipMemcpy(E.bank_elem, bank_elem.data(), c->nbank * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(emem_dev, c->emem_host.data(), ne * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(rmem_dev, rmem.data(), nr1 * 4, hipMemcpyHostToDevice));
   }
   {
-    // static part of the neighbour records
     std::vector<NbrRec> rec(ne);
     std::memset(rec.data(), 0, ne * sizeof(NbrRec));
-    for (int p = 0; p < ne; p++) {
-      rec[p].rough = m->rough[c->perm[p]];
-      rec[p].zmax = m->zmax[c->perm[p]];
+    for (int k = 0; k < nmem; k++) {
+      const auto &M = c->members[k];
+      for (int lp = 0; lp < M.ne; lp++) {
+        rec[M.eoff + lp].rough = ms[k]->rough[c->perm[M.eoff + lp]];
+        rec[M.eoff + lp].zmax = ms[k]->zmax[c->perm[M.eoff + lp]];
+      }
     }
     PIHM_HIP_TRY(hipMemcpy(E.rec, rec.data(), ne * sizeof(NbrRec), hipMemcpyHostToDevice));
   }
   if (nr > 0) {
-    std::vector<int32_t> down(nr), left(nr), right(nr);
-    for (int r = 0; r < nr; r++) {
-      down[r] = (m->down[r] > 0) ? m->down[r] - 1 : m->down[r];
-      left[r] = c->iperm[m->leftele[r] - 1];
-      right[r] = c->iperm[m->rightele[r] - 1];
+    std::vector<int32_t> down(nr), left(nr), right(nr), bct(nr), ordv(nr);
+    std::vector<double> rv(nr);
+    for (int k = 0; k < nmem; k++) {
+      const pihm_mesh_desc *m = ms[k];
+      const auto &M = c->members[k];
+      for (int r = 0; r < M.nr; r++) {
+        const int g = M.roff + r;
+        down[g] = (m->down[r] > 0) ? M.roff + m->down[r] - 1 : m->down[r];
+        left[g] = c->iperm[M.eoff + m->leftele[r] - 1];
+        right[g] = c->iperm[M.eoff + m->rightele[r] - 1];
+        bct[g] = m->riverbc_type[r];
+        ordv[g] = m->intrpl_ord[r];
+      }
     }
     PIHM_HIP_TRY(hipMemcpy(R.down, down.data(), nr * 4, hipMemcpyHostToDevice));
     PIHM_HIP_TRY(hipMemcpy(R.left, left.data(), nr * 4, hipMemcpyHostToDevice));
     PIHM_HIP_TRY(hipMemcpy(R.right, right.data(), nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.bctype, m->riverbc_type, nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.ord, m->intrpl_ord, nr * 4, hipMemcpyHostToDevice));
-    const double *rs[] = {m->rarea, m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left,
-                          m->dist_right, m->rdepth, m->coeff, m->length, m->width, m->rrough,
-                          m->cwr, m->rksath, m->rksatv, m->bedthick, m->rporosity};
-    for (int k = 0; k < 17; k++)
-      PIHM_HIP_TRY(hipMemcpy(*rd[k], rs[k], nr * sizeof(double), hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.bctype, bct.data(), nr * 4, hipMemcpyHostToDevice));
+    PIHM_HIP_TRY(hipMemcpy(R.ord, ordv.data(), nr * 4, hipMemcpyHostToDevice));
+    for (int f = 0; f < 17; f++) {
+      for (int k = 0; k < nmem; k++) {
+        const pihm_mesh_desc *m = ms[k];
+        const double *rs[] = {m->rarea, m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left,
+                              m->dist_right, m->rdepth, m->coeff, m->length, m->width, m->rrough,
+                              m->cwr, m->rksath, m->rksatv, m->bedthick, m->rporosity};
+        const auto &M = c->members[k];
+        for (int r = 0; r < M.nr; r++) rv[M.roff + r] = rs[f][r];
+      }
+      PIHM_HIP_TRY(hipMemcpy(*rd[f], rv.data(), nr * sizeof(double), hipMemcpyHostToDevice));
+    }
   }
   PIHM_HIP_TRY(hipMemcpy(R.up_ptr, up_ptr.data(), (nr + 1) * 4, hipMemcpyHostToDevice));
   PIHM_HIP_TRY(hipMemcpy(R.up_idx, up_idx.data(), up_idx.size() * 4, hipMemcpyHostToDevice));
 
-  A.ne = ne; A.nr = nr; A.nbank = c->nbank;
-  A.surf_mode = m->surf_mode; A.riv_mode = m->riv_mode; A.dt = m->stepsize;
+  c->mem_host.resize(nmem);
+  for (int k = 0; k < nmem; k++) {
+    const auto &M = c->members[k];
+    c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, 0};
+  }
+  PIHM_HIP_TRY(hipMemcpy(c->mem_dev, c->mem_host.data(), nmem * sizeof(MemberDev), hipMemcpyHostToDevice));
+
+  A.ne = (int)ne; A.nr = (int)nr; A.nbank = c->nbank;
+  A.surf_mode = c->surf_mode; A.riv_mode = c->riv_mode; A.dt = c->dt;
   A.cur = 0; A.record = 0;
   A.nanflag = c->nanflag_dev;
+  A.nmem = nmem;
+  A.mem = c->mem_dev;
+  A.emem = emem_dev;
+  A.rmem = rmem_dev;
   c->stage_host = nullptr;
   *out = c;
   return PIHM_OK;
+}
+
+// Apply f(member, host_base, device_base) for the member-major caller-order
+// layout of a concatenated state: perm maps device -> caller within members.
+void state_permute(const pihm_ctx *c, const double *src, double *dst, bool to_device) {
+  for (int k = 0; k < c->nmem; k++) {
+    const auto &M = c->members[k];
+    const double *s = src + M.yoff;
+    double *d = dst + M.yoff;
+    for (int b = 0; b < 3; b++)
+      for (int lp = 0; lp < M.ne; lp++) {
+        const long dv = (long)b * M.ne + lp, hv = (long)b * M.ne + c->perm[M.eoff + lp];
+        if (to_device) d[dv] = s[hv];
+        else d[hv] = s[dv];
+      }
+    for (long i = 3L * M.ne; i < M.neq; i++) d[i] = s[i];
+  }
+}
+
+}  // namespace
+
+extern "C" int pihm_ctx_create(const pihm_mesh_desc *m, pihm_ctx **out) {
+  if (!out) return PIHM_ERR_ARG;
+  std::vector<const pihm_mesh_desc *> ms{m};
+  return build_ctx(ms, out);
+}
+
+extern "C" int pihm_ctx_create_batch(int nmem, const pihm_mesh_desc *const *meshes, pihm_ctx **out) {
+  if (!out || nmem < 1 || !meshes) return PIHM_ERR_ARG;
+  std::vector<const pihm_mesh_desc *> ms(meshes, meshes + nmem);
+  return build_ctx(ms, out);
 }
 
 extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
@@ -791,6 +935,25 @@ extern "C" int pihm_ctx_set_stream(pihm_ctx *c, void *s) {
 }
 
 extern "C" int pihm_ctx_neq(const pihm_ctx *c) { return c ? c->neq : -1; }
+extern "C" int pihm_ctx_nmembers(const pihm_ctx *c) { return c ? c->nmem : -1; }
+
+extern "C" int pihm_ctx_member_info(const pihm_ctx *c, int k, long *yoff, int *neq, int *ne, int *nr) {
+  if (!c || k < 0 || k >= c->nmem) return PIHM_ERR_ARG;
+  const auto &M = c->members[k];
+  if (yoff) *yoff = M.yoff;
+  if (neq) *neq = M.neq;
+  if (ne) *ne = M.ne;
+  if (nr) *nr = M.nr;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_set_active(pihm_ctx *c, const int *active) {
+  if (!c || !active) return PIHM_ERR_ARG;
+  for (int k = 0; k < c->nmem; k++) c->mem_host[k].active = active[k] ? 1 : 0;
+  PIHM_HIP_TRY(hipMemcpyAsync(c->mem_dev, c->mem_host.data(), c->nmem * sizeof(MemberDev),
+                              hipMemcpyHostToDevice, c->stream));
+  return PIHM_OK;
+}
 
 extern "C" int pihm_ctx_set_nan_check(pihm_ctx *c, int on) {
   if (!c) return PIHM_ERR_ARG;
@@ -814,8 +977,10 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   if (!c || !in) return PIHM_ERR_ARG;
   const int ne = c->ne, nr = c->nr;
   std::vector<double> hd(3L * ne);
+  // element arrays are concatenated over members in caller order: member k's
+  // element i sits at members[k].eoff + i
   auto up1 = [&](double *dst, const double *src) -> int {
-    for (int p = 0; p < ne; p++) hd[p] = src[c->perm[p]];
+    for (int p = 0; p < ne; p++) hd[p] = src[c->members[c->emem_host[p]].eoff + c->perm[p]];
     PIHM_HIP_TRY(hipMemcpyAsync(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
     return PIHM_OK;
@@ -828,7 +993,8 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   if (in->ws0_surf && (rc = up1(E.ws0s, in->ws0_surf))) return rc;
   if (in->elem_bc) {
     for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) hd[(long)j * ne + p] = in->elem_bc[3L * c->perm[p] + j];
+      for (int p = 0; p < ne; p++)
+        hd[(long)j * ne + p] = in->elem_bc[3L * (c->members[c->emem_host[p]].eoff + c->perm[p]) + j];
     PIHM_HIP_TRY(hipMemcpyAsync(E.ebc, hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
   }
@@ -848,10 +1014,7 @@ static int check_vec(const pihm_ctx *c, N_Vector v) {
 extern "C" int pihm_ctx_state_to_device(pihm_ctx *c, const double *host, N_Vector dev) {
   if (!c || !host || check_vec(c, dev)) return PIHM_ERR_ARG;
   std::vector<double> tmp(c->neq);
-  const int ne = c->ne;
-  for (int b = 0; b < 3; b++)
-    for (int p = 0; p < ne; p++) tmp[(long)b * ne + p] = host[(long)b * ne + c->perm[p]];
-  for (long i = 3L * ne; i < c->neq; i++) tmp[i] = host[i];
+  state_permute(c, host, tmp.data(), true);
   return N_VCopyToDevice_Hip(dev, tmp.data());
 }
 
@@ -860,10 +1023,7 @@ extern "C" int pihm_ctx_state_from_device(pihm_ctx *c, N_Vector dev, double *hos
   std::vector<double> tmp(c->neq);
   int rc = N_VCopyFromDevice_Hip(dev, tmp.data());
   if (rc) return rc;
-  const int ne = c->ne;
-  for (int b = 0; b < 3; b++)
-    for (int p = 0; p < ne; p++) host[(long)b * ne + c->perm[p]] = tmp[(long)b * ne + p];
-  for (long i = 3L * ne; i < c->neq; i++) host[i] = tmp[i];
+  state_permute(c, tmp.data(), host, false);
   return PIHM_OK;
 }
 
@@ -875,10 +1035,16 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
   a.record = c->record;
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
-  hipLaunchKernelGGL(k_prep, dim3(nbe), dim3(kBlock), 0, c->stream, a);
-  hipLaunchKernelGGL(k_flux, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   const int nas = div_up((long)c->nr + c->nbank, kBlock);
-  if (nas > 0) hipLaunchKernelGGL(k_assemble, dim3(nas), dim3(kBlock), 0, c->stream, a);
+  if (c->nmem > 1) {
+    hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+  } else {
+    hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+  }
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
   return PIHM_OK;
@@ -908,6 +1074,11 @@ extern "C" int PIHM_ODE_Hip(realtype t, N_Vector y, N_Vector ydot, void *user_da
   return PIHM_OK;
 }
 
+// element index in the concatenated caller order of device element p
+static inline long caller_elem(const pihm_ctx *c, int p) {
+  return (long)c->members[c->emem_host[p]].eoff + c->perm[p];
+}
+
 extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
   if (!c || !ovl3) return PIHM_ERR_ARG;
   const int ne = c->ne, nr = c->nr;
@@ -921,7 +1092,7 @@ extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
       const int code = c->nb_host[(long)j * ne + p];
       double v = 0.0;
       if (code < 0 && code != NB_BOUND) v = hid[-code - 1];
-      ovl3[3L * c->perm[p] + j] = v;
+      ovl3[3L * caller_elem(c, p) + j] = v;
     }
   return PIHM_OK;
 }
@@ -934,7 +1105,7 @@ extern "C" int pihm_ctx_set_river_edge_ovlflow(pihm_ctx *c, const double *ovl3) 
   for (int p = 0; p < ne; p++)
     for (int j = 0; j < 3; j++) {
       const int code = c->nb_host[(long)j * ne + p];
-      if (code < 0 && code != NB_BOUND) hid[-code - 1] = ovl3[3L * c->perm[p] + j];
+      if (code < 0 && code != NB_BOUND) hid[-code - 1] = ovl3[3L * caller_elem(c, p) + j];
     }
   PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.hid_ovl, hid.data(), 2L * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -956,20 +1127,20 @@ extern "C" int pihm_ctx_get_fluxes(pihm_ctx *c, double *ovl, double *sub, double
   if (ovl) {
     if ((rc = get(c->args.rec.ovl, 3L * ne))) return rc;
     for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) ovl[3L * c->perm[p] + j] = h[(long)j * ne + p];
+      for (int p = 0; p < ne; p++) ovl[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
   }
   if (sub) {
     if ((rc = get(c->args.rec.sub, 3L * ne))) return rc;
     for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) sub[3L * c->perm[p] + j] = h[(long)j * ne + p];
+      for (int p = 0; p < ne; p++) sub[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
   }
   if (infil) {
     if ((rc = get(c->args.rec.infil, ne))) return rc;
-    for (int p = 0; p < ne; p++) infil[c->perm[p]] = h[p];
+    for (int p = 0; p < ne; p++) infil[caller_elem(c, p)] = h[p];
   }
   if (rechg) {
     if ((rc = get(c->args.rec.rechg, ne))) return rc;
-    for (int p = 0; p < ne; p++) rechg[c->perm[p]] = h[p];
+    for (int p = 0; p < ne; p++) rechg[caller_elem(c, p)] = h[p];
   }
   if (rivflow && nr > 0) {
     if ((rc = get(c->args.r.rf, 11L * nr))) return rc;
@@ -1002,11 +1173,16 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
   float t[3] = {0, 0, 0};
   for (int it = 0; it < iters; it++) {
     PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
-    hipLaunchKernelGGL(k_prep, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    if (c->nmem > 1) hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
     PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
-    hipLaunchKernelGGL(k_flux, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    if (c->nmem > 1) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    else hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[2], c->stream));
-    if (nas > 0) hipLaunchKernelGGL(k_assemble, dim3(nas), dim3(kBlock), 0, c->stream, a);
+    if (nas > 0) {
+      if (c->nmem > 1) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+      else hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+    }
     PIHM_HIP_TRY(hipEventRecord(ev[3], c->stream));
     PIHM_HIP_TRY(hipEventSynchronize(ev[3]));
     for (int k = 0; k < 3; k++) {

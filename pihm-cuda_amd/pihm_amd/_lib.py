@@ -98,6 +98,11 @@ _lib = None
 _PROTOS = [
     ("pihm_ctx_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("pihm_ctx_destroy", None, [C.c_void_p]),
+    ("pihm_ctx_create_batch", C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshDesc)), C.POINTER(C.c_void_p)]),
+    ("pihm_ctx_nmembers", C.c_int, [C.c_void_p]),
+    ("pihm_ctx_member_info", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("pihm_ctx_set_active", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("pihm_ctx_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
     ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
     ("pihm_ctx_neq", C.c_int, [C.c_void_p]),

@@ -202,3 +202,55 @@ def test_rhs_parity_full_size_200k():
     for k, y in enumerate(_states(m, rng, 2)):
         _check(h, o, y, what=f"c1 state {k}")
     h.close()
+
+
+def test_batch_members_equal_solo_bitwise():
+    """Several watersheds of different sizes in one batch context: each member's ydot is
+    the same bits as its own single-solve context, and inactive members are untouched."""
+    import ctypes as C
+    from pihm_amd import _lib
+    L = _lib.lib()
+    meshes = [make_watershed(n, seed=s, n_seg=k, network=net) for n, s, k, net in
+              ((535, 1, 20, "valley"), (3000, 21, 80, "tree"), (1200, 5, 30, "tree"))]
+    rng = np.random.default_rng(17)
+    ins = [_inputs(m, rng) for m in meshes]
+    ys = [_states(m, rng, 1)[1] for m in meshes]
+    solo = []
+    for m, inp, y in zip(meshes, ins, ys):
+        h = HipRHS(m)
+        h.set_step_inputs(**inp)
+        solo.append([h.ode(y)[1] for _ in range(2)])     # two calls: carried state too
+        h.close()
+    descs = [m.desc() for m in meshes]
+    arr = (C.POINTER(_lib.MeshDesc) * 3)(*[C.pointer(d) for d in descs])
+    ctx = C.c_void_p()
+    assert L.pihm_ctx_create_batch(3, arr, C.byref(ctx)) == 0
+    assert L.pihm_ctx_nmembers(ctx) == 3
+    neq = L.pihm_ctx_neq(ctx)
+    assert neq == sum(m.neq for m in meshes)
+    cat = lambda k: np.concatenate([i[k] for i in ins])
+    keep = [cat(k) for k in ("pcpdrp", "edir", "ett", "ws0_surf")]
+    s = _lib.StepInputs()
+    s.pcpdrp, s.edir, s.ett, s.ws0_surf = [a.ctypes.data_as(_lib.c_double_p) for a in keep]
+    assert L.pihm_ctx_set_step_inputs(ctx, C.byref(s)) == 0
+    L.pihm_ctx_set_nan_check(ctx, 1)
+    yv, ydv = L.N_VNew_Hip(neq), L.N_VNew_Hip(neq)
+    ycat = np.concatenate(ys)
+    assert L.pihm_ctx_state_to_device(ctx, ycat.ctypes.data_as(_lib.c_double_p), yv) == 0
+    offs = np.cumsum([0] + [m.neq for m in meshes])
+    for call in range(2):
+        assert L.PIHM_ODE_Hip(0.0, yv, ydv, ctx) == 0
+        out = np.empty(neq)
+        L.pihm_ctx_state_from_device(ctx, ydv, out.ctypes.data_as(_lib.c_double_p))
+        for k in range(3):
+            assert np.array_equal(out[offs[k]:offs[k + 1]], solo[k][call]), (call, k)
+    # deactivate member 1: its ydot stays as it was
+    act = (C.c_int * 3)(1, 0, 1)
+    L.pihm_ctx_set_active(ctx, act)
+    before = out.copy()
+    assert L.PIHM_ODE_Hip(0.0, yv, ydv, ctx) == 0
+    L.pihm_ctx_state_from_device(ctx, ydv, out.ctypes.data_as(_lib.c_double_p))
+    assert np.array_equal(out[offs[1]:offs[2]], before[offs[1]:offs[2]])
+    L.N_VDestroy_Hip(yv)
+    L.N_VDestroy_Hip(ydv)
+    L.pihm_ctx_destroy(ctx)
