@@ -87,6 +87,14 @@ struct MemberDev {
   int pad;
 };
 
+// Per-call state and derivative of one batch member.
+struct MemberIO {
+  const double *y;
+  double *ydot;
+  int active;
+  int pad;
+};
+
 struct RhsArgs {
   int ne, nr, nbank;
   int surf_mode, riv_mode;
@@ -102,6 +110,8 @@ struct RhsArgs {
   int nmem;
   const MemberDev *mem;        // [nmem]
   const int32_t *emem, *rmem;  // member of each device element / river
+  const MemberIO *mio;         // [nmem] per-call member table, or nullptr
+  int *mnan;                   // [nmem] per-member NaN flags, or nullptr
 };
 
 }  // namespace pihm
@@ -111,6 +121,12 @@ struct pihm_member_host {
   int eoff, ne, roff, nr, neq;
   double dt;
 };
+
+// Launch one batched RHS evaluation whose member states come from the device
+// table mio (inactive members untouched); NaN in a member's derivative sets
+// mnan[member].  Asynchronous on the context's stream.
+struct pihm_ctx;
+int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
 
 struct pihm_ctx {
   int ne, nr, nbank, neq;       // totals over the batch

@@ -5,10 +5,10 @@
 // a == b and a*(x-y) when a == -b, (a*x)+(b*y) otherwise, which equals every
 // special case of the serial version bit for bit.
 //
-// Reductions run in one launch: each workgroup reduces a fixed, strided set
-// of entries (wavefront shuffles, then LDS), writes its partial, and the last
-// workgroup to finish (atomic ticket) adds the partials in workgroup order
-// and writes the result to pinned host memory.  The summation tree depends
+// Reductions run in two launches: each workgroup reduces a fixed, strided set
+// of entries (wavefront shuffles, then LDS) and writes its partial; one
+// workgroup then adds the partials in workgroup order and writes the result
+// to pinned host memory.  The summation tree depends
 // only on the length, so results are reproducible bit for bit.
 #include <hip/hip_runtime.h>
 
@@ -19,6 +19,7 @@
 
 #include "common.hpp"
 #include "nvector_impl.hpp"
+#include "redux.hpp"
 
 using namespace pihm;
 
@@ -168,7 +169,6 @@ __device__ __forceinline__ double red_elem(const RedArgs &a, long i, int k) {
 
 __global__ __launch_bounds__(kRedBlock) void k_reduce(RedArgs a) {
   __shared__ double lds[4];
-  __shared__ bool last;
   const long stride = (long)gridDim.x * blockDim.x;
   for (int k = 0; k < a.nvec; k++) {
     double v = red_identity(a.kind);
@@ -177,24 +177,15 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(RedArgs a) {
     const double r = block_reduce(a.kind, v, lds);
     if (threadIdx.x == 0) a.work.partials[(long)k * kMaxRedBlocks + blockIdx.x] = r;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(a.work.ticket, 1u);
-    last = (t == gridDim.x - 1);
+}
+
+// how the second pass combines the partials of a reduction kind
+unsigned red_comb(int kind) {
+  switch (kind) {
+    case R_MAXABS: case R_INVTEST: case R_CONSTR: return COMB_MAX;
+    case R_MIN: case R_MINQ: return COMB_MIN;
+    default: return COMB_SUM;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  for (int k = 0; k < a.nvec; k++) {
-    // fixed-order tree over the partials of workgroups 0..gridDim-1
-    double v = red_identity(a.kind);
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      v = red_combine(a.kind, v, __builtin_nontemporal_load(&a.work.partials[(long)k * kMaxRedBlocks + b]));
-    const double r = block_reduce(a.kind, v, lds);
-    if (threadIdx.x == 0) a.work.result[k] = r;
-  }
-  if (threadIdx.x == 0) *a.work.ticket = 0u;
 }
 
 // fused ops ------------------------------------------------------------------
@@ -256,7 +247,6 @@ __global__ void k_correct(long n, int q, ZnPtrs z, const double *acor, double l0
 __global__ __launch_bounds__(kRedBlock) void k_ewt(long n, const double *y, double rtol, double atol,
                                                   double *ewt, RedWork work) {
   __shared__ double lds[4];
-  __shared__ bool last;
   const long stride = (long)gridDim.x * blockDim.x;
   double mn = DBL_MAX, sq = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -273,24 +263,6 @@ __global__ __launch_bounds__(kRedBlock) void k_ewt(long n, const double *y, doub
   if (threadIdx.x == 0) {
     work.partials[blockIdx.x] = r0;
     work.partials[kMaxRedBlocks + blockIdx.x] = r1;
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(work.ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double v0 = DBL_MAX, v1 = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-    v0 = fmin(v0, __builtin_nontemporal_load(&work.partials[b]));
-    v1 += __builtin_nontemporal_load(&work.partials[kMaxRedBlocks + b]);
-  }
-  const double s0 = block_reduce(R_MIN, v0, lds);
-  const double s1 = block_reduce(R_DOT, v1, lds);
-  if (threadIdx.x == 0) {
-    work.result[0] = s0;
-    work.result[1] = s1;
-    *work.ticket = 0u;
   }
 }
 
@@ -316,8 +288,6 @@ int aux_new(Aux **out) {
   Aux *x = (Aux *)std::calloc(1, sizeof(Aux));
   if (!x) return -1;
   if (hipMalloc(&x->w.partials, sizeof(double) * kMaxRedBlocks * kMaxMulti) != hipSuccess) return -1;
-  if (hipMalloc(&x->w.ticket, sizeof(unsigned)) != hipSuccess) return -1;
-  if (hipMemset(x->w.ticket, 0, sizeof(unsigned)) != hipSuccess) return -1;
   if (hipHostMalloc((void **)&x->w.result, sizeof(double) * kMaxMulti, hipHostMallocDefault) != hipSuccess)
     return -1;
   x->refs = 1;
@@ -328,14 +298,18 @@ void aux_release(Aux *x) {
   if (!x) return;
   if (--x->refs > 0) return;
   (void)hipFree(x->w.partials);
-  (void)hipFree(x->w.ticket);
   (void)hipHostFree(x->w.result);
   std::free(x);
 }
 
 bool reduce(N_Vector v, RedArgs &a, double *out, int nout) {
   a.work = X(v)->w;
-  hipLaunchKernelGGL(k_reduce, dim3(red_blocks(a.n)), dim3(kRedBlock), 0, S(v), a);
+  const int nb = red_blocks(a.n);
+  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kRedBlock), 0, S(v), a);
+  unsigned combs = 0;
+  for (int k = 0; k < a.nvec; k++) combs |= red_comb(a.kind) << (2 * k);
+  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedBlock), 0, S(v), nb, a.nvec, combs,
+                     a.work.partials, a.work.result);
   if (hipStreamSynchronize(S(v)) != hipSuccess) return false;
   for (int k = 0; k < nout; k++) out[k] = a.work.result[k];
   return true;
@@ -600,7 +574,10 @@ int pihm_nv_correct(int q, const double *l, N_Vector acor, N_Vector *zn) {
 int pihm_nv_ewt_ss(N_Vector y, double rtol, double atol, N_Vector ewt, double *mn, double *ynrm) {
   const long n = L(y);
   RedWork w = X(y)->w;
-  hipLaunchKernelGGL(k_ewt, dim3(red_blocks(n)), dim3(kRedBlock), 0, S(y), n, D(y), rtol, atol, D(ewt), w);
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_ewt, dim3(nb), dim3(kRedBlock), 0, S(y), n, D(y), rtol, atol, D(ewt), w);
+  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedBlock), 0, S(y), nb, 2, COMB_MIN | (COMB_SUM << 2),
+                     w.partials, w.result);
   if (hipStreamSynchronize(S(y)) != hipSuccess) return -2;
   *mn = w.result[0];
   *ynrm = std::sqrt(w.result[1] / (double)n);

@@ -9,7 +9,6 @@ extern "C" {
 // Reduction workspace shared by a family of device vectors (clones share it).
 typedef struct pihm_redwork {
   double *partials;   // device, [PIHM_RED_MAXBLOCKS * PIHM_RED_MAXSUMS]
-  unsigned *ticket;   // device counter, zero between launches
   double *result;     // pinned host, [PIHM_RED_MAXSUMS]
 } pihm_redwork;
 #define PIHM_RED_MAXBLOCKS 1024

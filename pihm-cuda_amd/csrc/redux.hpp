@@ -1,12 +1,16 @@
-// redux.hpp - one-launch element-wise operation fused with NV deterministic sums.
+// redux.hpp - element-wise operation fused with NV deterministic sums.
 //
 // Each workgroup accumulates NV partial sums over a fixed strided set of
-// entries (wavefront butterfly, then the four waves in order), writes them,
-// and the last workgroup to take a ticket adds the partials in workgroup order
-// and writes the NV results to pinned host memory.  The summation tree depends
-// only on the length, so every result is reproducible bit for bit.
+// entries (wavefront butterfly, then the four waves in order) and writes
+// them; a one-workgroup second launch adds the partials in workgroup order
+// and writes the NV results to pinned host memory.  The summation tree
+// depends only on the length, so every result is reproducible bit for bit.
+// Two launches instead of a last-workgroup ticket: on a multi-XCD device the
+// device-scope fence such a ticket needs writes back the whole L2 of the
+// issuing XCD, once per workgroup.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cfloat>
 #include "common.hpp"
 #include "nvector_impl.hpp"
 
@@ -28,9 +32,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int NV, class F>
-__global__ __launch_bounds__(kRedThreads) void k_fused_sum(long n, F f, pihm_redwork w) {
+__global__ __launch_bounds__(kRedThreads) void k_fused_sum(long n, F f, double *partials) {
   __shared__ double lds[4 * NV];
-  __shared__ bool last;
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; k++) acc[k] = 0.0;
@@ -46,32 +49,54 @@ __global__ __launch_bounds__(kRedThreads) void k_fused_sum(long n, F f, pihm_red
   if (threadIdx.x < NV) {
     const int k = threadIdx.x;
     const double v = ((lds[k] + lds[NV + k]) + lds[2 * NV + k]) + lds[3 * NV + k];
-    w.partials[(long)k * PIHM_RED_MAXBLOCKS + blockIdx.x] = v;
+    partials[(long)k * PIHM_RED_MAXBLOCKS + blockIdx.x] = v;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(w.ticket, 1u) == gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      v += __builtin_nontemporal_load(&w.partials[(long)k * PIHM_RED_MAXBLOCKS + b]);
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) lds[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) w.result[k] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  }
-  if (threadIdx.x == 0) *w.ticket = 0u;
 }
 
-// Launch and wait; results in out[0..NV).
+// How the second pass combines the partials of slot k: 2 bits per slot.
+enum : unsigned { COMB_SUM = 0u, COMB_MIN = 1u, COMB_MAX = 2u };
+
+__device__ __forceinline__ double comb_ident(unsigned c) {
+  return c == COMB_MIN ? DBL_MAX : 0.0;
+}
+__device__ __forceinline__ double comb(unsigned c, double a, double b) {
+  return c == COMB_MIN ? fmin(a, b) : (c == COMB_MAX ? fmax(a, b) : a + b);
+}
+
+// Second pass of every reduction: one workgroup adds the nb partials of each
+// of the nv slots in workgroup order (strided per thread, wavefront
+// butterfly, the four waves in order) and writes the results.  A separate
+// launch, so the partials of the first pass are visible without any
+// device-wide fence in the first pass.
+namespace {
+__global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, unsigned combs,
+                                                                 const double *partials, double *result) {
+  __shared__ double lds[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < nv; k++) {
+    const unsigned c = (combs >> (2 * k)) & 3u;
+    double v = comb_ident(c);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v = comb(c, v, partials[(long)k * PIHM_RED_MAXBLOCKS + b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = comb(c, v, __shfl_xor(v, off, 64));
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = comb_ident(c);
+      for (int w2 = 0; w2 < (int)(blockDim.x >> 6); w2++) r = comb(c, r, lds[w2]);
+      result[k] = r;
+    }
+    __syncthreads();
+  }
+}
+}  // namespace
+
+// Launch both passes and wait; results in out[0..NV).
 template <int NV, class F>
 int fused_sum(long n, const F &f, pihm_redwork w, hipStream_t st, double *out) {
-  hipLaunchKernelGGL((k_fused_sum<NV, F>), dim3(fused_blocks(n)), dim3(kRedThreads), 0, st, n, f, w);
+  const int nb = fused_blocks(n);
+  hipLaunchKernelGGL((k_fused_sum<NV, F>), dim3(nb), dim3(kRedThreads), 0, st, n, f, w.partials);
+  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedThreads), 0, st, nb, NV, 0u, w.partials, w.result);
   if (hipGetLastError() != hipSuccess) return -2;
   if (hipStreamSynchronize(st) != hipSuccess) return -2;
   for (int k = 0; k < NV; k++) out[k] = w.result[k];

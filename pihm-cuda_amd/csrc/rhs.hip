@@ -44,18 +44,33 @@ using namespace pihm;
 // concatenates solves (member-major state, contiguous element and river
 // ranges); a single solve (B = false) has all offsets zero.
 struct Ix {
-  long yb;        // first state entry of the solve
-  int ne, eoff;   // element count and first device element
-  int nr, roff;   // river count and first river
+  const double *y;  // state of the solve
+  double *ydot;     // and its time derivative
+  int ne, eoff;     // element count and first device element
+  int nr, roff;     // river count and first river
   double dt;
   int active;
+  int m;            // member
 };
 
+// A batch reads each member's state either at its offset in one whole-batch
+// vector (mio == nullptr) or through the per-call table mio (state and
+// derivative pointers plus the active flag of every member).
 template <bool B>
 __device__ __forceinline__ Ix ix_of(const RhsArgs &a, int member) {
-  if (!B) return Ix{0, a.ne, 0, a.nr, 0, a.dt, 1};
+  if (!B) return Ix{a.y, a.ydot, a.ne, 0, a.nr, 0, a.dt, 1, 0};
   const MemberDev M = a.mem[member];
-  return Ix{M.yoff, M.ne, M.eoff, M.nr, M.roff, M.dt, M.active};
+  if (a.mio) {
+    const MemberIO io = a.mio[member];
+    return Ix{io.y, io.ydot, M.ne, M.eoff, M.nr, M.roff, M.dt, io.active, member};
+  }
+  return Ix{a.y + M.yoff, a.ydot + M.yoff, M.ne, M.eoff, M.nr, M.roff, M.dt, M.active, member};
+}
+
+template <bool B>
+__device__ __forceinline__ void flag_nan(const RhsArgs &a, const Ix &ix) {
+  *a.nanflag = 1;
+  if (B && a.mnan) a.mnan[ix.m] = 1;
 }
 template <bool B>
 __device__ __forceinline__ Ix ix_elem(const RhsArgs &a, int p) { return ix_of<B>(a, B ? a.emem[p] : 0); }
@@ -69,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
   if (p >= ne) return;
   const Ix ix = ix_elem<B>(a, p);
   if (!ix.active) return;
-  const double *y = a.y + ix.yb;      // this solve's state
+  const double *y = ix.y;
   const int lp = p - ix.eoff;
   const ElemDev &e = a.e;
 
@@ -120,8 +135,8 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const Ix ix = ix_elem<B>(a, p);
   if (!ix.active) return;
   const int lp = p - ix.eoff;
-  const double *y = a.y + ix.yb;
-  double *ydot = a.ydot + ix.yb;
+  const double *y = ix.y;
+  double *ydot = ix.ydot;
   const NbrRec me = e.rec[p];
   const double u = clamp0(y[ix.ne + lp]);
   const double g = me.gw;
@@ -287,7 +302,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
 #pragma unroll
     for (int j = 0; j < 3; j++) { b->ovl[j] = ovl[j]; b->sub[j] = sub[j]; }
   }
-  if (nan) *a.nanflag = 1;
+  if (nan) flag_nan<B>(a, ix);
 }
 
 template <bool B>
@@ -296,7 +311,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   const RiverDev &R = a.r;
   const Ix ix = ix_river<B>(a, r);
   if (!ix.active) return;
-  const double *ys = a.y + ix.yb + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
+  const double *ys = ix.y + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
   const double *yg = ys + ix.nr;                           // yg[r']: river groundwater
   const double stage = clamp0(ys[r]);
   const double rgw = clamp0(yg[r]);
@@ -497,14 +512,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
     double dg = 0.0;
     dg += -R.rf[7 * nr + r] - R.rf[8 * nr + r] - R.rf[9 * nr + r] - upa + rf6;
     dg /= R.porosity[r] * area;
-    double *ydr = a.ydot + ix.yb + 3L * ix.ne - ix.roff;
+    double *ydr = ix.ydot + 3L * ix.ne - ix.roff;
     ydr[r] = ds;
     ydr[ix.nr + r] = dg;
     if (a.record) {
       R.rf[0 * nr + r] = up;
       R.rf[10 * nr + r] = upa;
     }
-    if (isnan(ds) || isnan(dg)) *a.nanflag = 1;
+    if (isnan(ds) || isnan(dg)) flag_nan<B>(a, ix);
   } else if (t < nr + a.nbank) {
     const int k = t - nr;
     const int p = a.e.bank_elem[k];
@@ -534,10 +549,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
       dyg -= s / area;
     }
     dyg /= a.e.porosity[p];
-    double *yde = a.ydot + ix.yb + (p - ix.eoff);
+    double *yde = ix.ydot + (p - ix.eoff);
     yde[0] = dys;
     yde[2 * ix.ne] = dyg;
-    if (isnan(dys) || isnan(dyg)) *a.nanflag = 1;
+    if (isnan(dys) || isnan(dyg)) flag_nan<B>(a, ix);
   }
 }
 
@@ -885,6 +900,8 @@ int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
   A.mem = c->mem_dev;
   A.emem = emem_dev;
   A.rmem = rmem_dev;
+  A.mio = nullptr;
+  A.mnan = nullptr;
   c->stage_host = nullptr;
   *out = c;
   return PIHM_OK;
@@ -1045,6 +1062,25 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
     hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a);
   }
+  PIHM_HIP_TRY(hipGetLastError());
+  c->ncalls++;
+  return PIHM_OK;
+}
+
+int pihm_rhs_launch_io(pihm_ctx *c, const MemberIO *mio, int *mnan) {
+  if (!c || !mio || c->nmem < 1) return PIHM_ERR_ARG;
+  RhsArgs a = c->args;
+  a.y = nullptr;
+  a.ydot = nullptr;
+  a.mio = mio;
+  a.mnan = mnan;
+  a.record = 0;
+  const int nbe = div_up(c->ne, kBlock);
+  const int nbr = div_up(c->nr, kBlock);
+  const int nas = div_up((long)c->nr + c->nbank, kBlock);
+  hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
   return PIHM_OK;

@@ -192,6 +192,14 @@ _PROTOS = [
     ("pihm_gpu_backend_get", C.c_int, [C.c_void_p, C.POINTER(Backend)]),
     ("pihm_run_gpu", C.c_int, [C.c_void_p, C.POINTER(Forcing), C.POINTER(RunConfig), c_double_p,
                                C.POINTER(RunStats)]),
+    ("pihm_ens_create", C.c_int, [C.c_void_p, C.POINTER(Forcing), C.POINTER(RunConfig), C.POINTER(C.c_void_p)]),
+    ("pihm_ens_destroy", None, [C.c_void_p]),
+    ("pihm_ens_set_state", C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    ("pihm_ens_get_state", C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    ("pihm_ens_set_member_enabled", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("pihm_ens_run", C.c_int, [C.c_void_p, C.c_double]),
+    ("pihm_ens_member_stats", C.c_int, [C.c_void_p, C.c_int, C.POINTER(RunStats)]),
+    ("pihm_ens_counters", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
 ]
 
 
