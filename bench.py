@@ -83,6 +83,61 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
                 yend=yend)
 
 
+def ensemble_run(nmem, hours, sample, seed):
+    """C3: nmem members advanced together (pihm_ensemble.h) over [0, hours], against a
+    stratified sample of the same members run one at a time through the single-solve
+    device path (pihm_run_gpu) over the same window."""
+    from pihm_amd.ensemble import Ensemble
+    from pihm_amd.workloads import build_meshes, member_forcing, member_spec
+    L = _lib.lib()
+    specs = member_spec(nmem, seed=seed)
+    meshes = build_meshes(specs)
+    forc = [member_forcing(sp, m, hours + 3) for sp, m in zip(specs, meshes)]
+    cfg = run_config(hours * 3600.0)
+    e = Ensemble(meshes, forc, cfg)
+    for k, m in enumerate(meshes):
+        e.set_state(k, m.initial_state())
+    t0 = time.perf_counter()
+    nfail = e.run(hours * 3600.0)
+    wall = time.perf_counter() - t0
+    st = [e.stats(k) for k in range(nmem)]
+    cnt = e.counters()
+    e.close()
+    member_days = nmem * hours / 24.0
+    # one at a time: every (nmem/sample)-th member in size order
+    order = np.argsort([m.nelem for m in meshes])
+    pick = order[np.linspace(0, nmem - 1, min(sample, nmem)).round().astype(int)]
+    single_wall = []
+    for k in pick:
+        m = meshes[k]
+        desc = m.desc()
+        ctx = C.c_void_p()
+        assert L.pihm_ctx_create(C.byref(desc), C.byref(ctx)) == 0
+        y = np.ascontiguousarray(m.initial_state())
+        rs = _lib.RunStats()
+        fs = forc[k].struct()
+        L.pihm_ctx_sync(ctx)
+        t1 = time.perf_counter()
+        rc = L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), y.ctypes.data_as(_lib.c_double_p), C.byref(rs))
+        single_wall.append(time.perf_counter() - t1)
+        L.pihm_ctx_destroy(ctx)
+        if rc:
+            raise RuntimeError(f"single-solve member {k} failed: {rc}")
+    est_single_total = float(np.sum(single_wall)) * nmem / len(pick)
+    return dict(
+        metric="member-days/s", value=member_days / wall, unit="member-days/s", members=nmem,
+        elements=int(sum(m.nelem for m in meshes)), unknowns=int(sum(m.neq for m in meshes)),
+        window_hours=hours, wall_s=wall, failed=nfail,
+        steps_total=int(sum(s.nst for s in st)), steps_max=int(max(s.nst for s in st)),
+        rhs_evals_total=int(sum(s.nfe + s.nfes for s in st)), **cnt,
+        one_at_a_time={"value": member_days / est_single_total, "unit": "member-days/s",
+                       "sample": f"{len(pick)} members evenly spaced in size order, each timed alone "
+                                 f"through pihm_run_gpu over the same window; total scaled by "
+                                 f"{nmem}/{len(pick)}",
+                       "sample_wall_s": float(np.sum(single_wall))},
+        speedup_vs_one_at_a_time=(member_days / wall) / (member_days / est_single_total))
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -102,6 +157,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-hours", type=float, default=1.0)
+    ap.add_argument("--ensemble-sample", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -159,6 +218,8 @@ def main():
             gs = g["marks"][S] - g["marks"][0.0]
             out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_run(args.ensemble_members, args.ensemble_hours, args.ensemble_sample, 2024)
     print(json.dumps(out))
 
 
