@@ -148,6 +148,39 @@ def cpu_run(mesh, forcing, hours, nthreads):
     return dict(rc=rc, wall=time.perf_counter() - t0, stats=st.as_dict(), yend=y)
 
 
+def profile_kernels(iters, seed):
+    """Short fixed workload for hardware-counter passes: `iters` RHS evaluations at C1 on a
+    mid-storm state, then `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB
+    read and 512 MiB written per launch), the calibration kernel for the byte counters."""
+    L = _lib.lib()
+    mesh = config_mesh("c1", seed=seed)
+    desc = mesh.desc()
+    ctx = C.c_void_p()
+    assert L.pihm_ctx_create(C.byref(desc), C.byref(ctx)) == 0
+    L.pihm_ctx_set_nan_check(ctx, 0)
+    rng = np.random.default_rng(3)
+    y0 = mesh.initial_state()
+    y0[:mesh.nelem] = rng.uniform(0.0, 2e-3, mesh.nelem)        # ponded surface water
+    y = L.N_VNew_Hip(mesh.neq)
+    yd = L.N_VNew_Hip(mesh.neq)
+    assert L.pihm_ctx_state_to_device(ctx, y0.ctypes.data_as(_lib.c_double_p), y) == 0
+    for _ in range(iters):
+        L.PIHM_ODE_Hip(0.0, y, yd, ctx)
+    L.pihm_ctx_sync(ctx)
+    n = 64 << 20
+    a = L.N_VNew_Hip(n)
+    b = L.N_VNew_Hip(n)
+    L.N_VConst_Hip(1.0, a)
+    for _ in range(iters):
+        L.N_VScale_Hip(1.0, a, b)
+    L.pihm_ctx_sync(ctx)
+    for v in (y, yd, a, b):
+        L.N_VDestroy_Hip(v)
+    L.pihm_ctx_destroy(ctx)
+    print(json.dumps({"profile_kernels": iters, "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
+                      "copy_bytes": 2 * n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +194,12 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-hours", type=float, default=1.0)
     ap.add_argument("--ensemble-sample", type=int, default=16)
+    ap.add_argument("--profile-kernels", type=int, default=0,
+                    help="only run N RHS evaluations and N copy-kernel launches (counter passes)")
     args = ap.parse_args()
+    if args.profile_kernels > 0:
+        profile_kernels(args.profile_kernels, args.seed)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -68,6 +68,31 @@ int  pihm_ens_member_stats(pihm_ensemble *e, int k, pihm_run_stats *st);
  * right-hand-side evaluations since creation. */
 int  pihm_ens_counters(pihm_ensemble *e, long *ticks, long *op_launches, long *rhs_launches);
 
+/* ---- serving: members that come and go ----------------------------------
+ * An ensemble over a slot context (pihm_ctx_create_slots) starts with every
+ * slot empty; slot k takes forcing of up to nrec_cap[k] records and
+ * nstation_cap[k] stations.  pihm_ens_load_member puts a new solve into a
+ * slot that is not running (empty, done or failed): mesh, forcing, initial
+ * state (caller order) and the solver time t_end it runs to (a multiple of
+ * cfg->stepsize); it starts running at the next pihm_ens_advance, beside
+ * whatever else is running.  pihm_ens_advance runs at most max_ticks
+ * synchronisations of all running members and returns after the first tick
+ * in which any member finished or failed; it returns the number of such
+ * members (0 if max_ticks ran out first), and *nfailed the failed ones.
+ * A finished member's state stays readable (pihm_ens_get_state) until its
+ * slot is loaded again. */
+#define PIHM_ENS_EMPTY   0
+#define PIHM_ENS_READY   1
+#define PIHM_ENS_RUNNING 2
+#define PIHM_ENS_DONE    3
+#define PIHM_ENS_FAILED  (-1)
+int  pihm_ens_create_slots(pihm_ctx *slots, const int32_t *nrec_cap, const int32_t *nstation_cap,
+                           const pihm_run_config *cfg, pihm_ensemble **out);
+int  pihm_ens_load_member(pihm_ensemble *e, int k, const pihm_mesh_desc *mesh, const pihm_forcing *forc,
+                          const double *y0, double t_end);
+int  pihm_ens_advance(pihm_ensemble *e, long max_ticks, int *nfailed);
+int  pihm_ens_member_status(const pihm_ensemble *e, int k);
+
 #ifdef __cplusplus
 }
 #endif

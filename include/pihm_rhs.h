@@ -113,6 +113,19 @@ int  pihm_ctx_nmembers(const pihm_ctx *ctx);
 int  pihm_ctx_member_info(const pihm_ctx *ctx, int member, long *yoff, int *neq, int *nelem,
                           int *nriver);
 int  pihm_ctx_set_active(pihm_ctx *ctx, const int *active);   /* [nmem], stream-ordered */
+
+/* A context of nslots empty slots, slot k able to hold a mesh of up to
+ * ne_cap[k] elements and nr_cap[k] river segments (device ranges are laid out
+ * at these capacities; member_info's yoff/neq give a slot's state range and
+ * the loaded mesh's length).  pihm_ctx_load_member (re)loads slot k with a
+ * mesh, renumbering it and resetting the slot's carried river-edge fluxes and
+ * per-step inputs; it is how a serving loop reuses memory without
+ * reallocating.  Host-side arrays of the batch functions (step inputs, state
+ * permutations, flux read-back) list the loaded meshes in slot order at their
+ * actual sizes.  The reference has no counterpart (it runs one watershed). */
+int  pihm_ctx_create_slots(int nslots, const int32_t *ne_cap, const int32_t *nr_cap, int surf_mode,
+                           int riv_mode, pihm_ctx **out);
+int  pihm_ctx_load_member(pihm_ctx *ctx, int slot, const pihm_mesh_desc *mesh);
 int  pihm_ctx_set_stream(pihm_ctx *ctx, void *hip_stream);
 int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
 int  pihm_ctx_neq(const pihm_ctx *ctx);

@@ -61,7 +61,7 @@ struct RiverDev {
   int32_t *down;        // >= 0 device index of the downstream segment, else outlet code
   int32_t *left, *right;  // device element index of the banks
   int32_t *bctype, *ord;
-  int32_t *up_ptr, *up_idx;  // upstream segments of r, ascending caller index
+  int32_t *up_beg, *up_end, *up_idx;  // upstream segments of r: up_idx[up_beg[r] .. up_end[r]), ascending caller index
   double *area, *zmin, *zmax, *zbed, *node_zmax, *dist_l, *dist_r, *depth, *coeff,
       *length, *width, *rough, *cwr, *ksath, *ksatv, *bedthick, *porosity;
   double *bc;           // per-step river bc value
@@ -116,9 +116,21 @@ struct RhsArgs {
 
 }  // namespace pihm
 
+namespace pihm {
+// Capacity of one slot of a context: the largest mesh it can hold.
+struct SlotCap {
+  int ne, nr;
+};
+}  // namespace pihm
+
+// One slot: device ranges at its capacity (eoff, roff, yoff), the loaded
+// mesh's actual sizes (ne, nr, neq), and its offsets in caller order, where
+// the loaded slots are concatenated at their actual sizes (ceoff, croff, cyoff).
 struct pihm_member_host {
   long yoff;
   int eoff, ne, roff, nr, neq;
+  int ceoff, croff;
+  long cyoff;
   double dt;
 };
 
@@ -129,8 +141,12 @@ struct pihm_ctx;
 int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
 
 struct pihm_ctx {
-  int ne, nr, nbank, neq;       // totals over the batch
+  int ne, nr, nbank, neq;       // totals over the batch, at slot capacity
   int nmem;
+  std::vector<pihm::SlotCap> caps;
+  std::vector<int> boff, uoff;  // first bank record / upstream entry of each slot
+  int ne_loaded, nr_loaded;     // caller-order totals of the loaded meshes
+  long neq_loaded;
   std::vector<pihm_member_host> members;
   std::vector<int32_t> emem_host;   // member of each device element
   std::vector<pihm::MemberDev> mem_host;

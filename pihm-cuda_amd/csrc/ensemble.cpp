@@ -685,6 +685,8 @@ Task<int> cvode(Member &m, double tout) {
 }  // namespace
 
 // ------------------------------------------------------------------ the ensemble
+enum MemberStatus { ST_EMPTY = 0, ST_READY = 1, ST_RUNNING = 2, ST_DONE = 3, ST_FAILED = -1 };
+
 struct pihm_ensemble {
   pihm_ctx *ctx = nullptr;
   hipStream_t st = nullptr;
@@ -694,6 +696,7 @@ struct pihm_ensemble {
   int total_chunks = 0;
   Dev dev{};
   std::vector<MemberInfo> info;
+  std::vector<int> nrec_cap, nst_cap;
   MemberInfo *info_dev = nullptr;
   double *vecs = nullptr;
   double *partials = nullptr;
@@ -704,11 +707,15 @@ struct pihm_ensemble {
   char *stage_h = nullptr, *stage_d = nullptr;
   size_t stage_cap = 0;
   std::vector<Member> mem;
+  std::vector<int> status;     // MemberStatus per slot
+  std::vector<long> k_end;     // model steps each member runs to
+  std::vector<Task<int>> tops; // running model-step loops
   long ticks = 0, op_launches = 0, rhs_launches = 0;
 
   double *P(int v, int m) const { return vecs + (long)v * stride + info[m].yoff; }
   int ensure_stage(size_t bytes);
   int flush();
+  int upload_info(int k);
 };
 
 int pihm_ensemble::ensure_stage(size_t bytes) {
@@ -722,6 +729,12 @@ int pihm_ensemble::ensure_stage(size_t bytes) {
   PIHM_HIP_TRY(hipHostMalloc((void **)&stage_h, cap, hipHostMallocDefault));
   PIHM_HIP_TRY(hipMalloc((void **)&stage_d, cap));
   stage_cap = cap;
+  return PIHM_OK;
+}
+
+int pihm_ensemble::upload_info(int k) {
+  PIHM_HIP_TRY(hipMemcpyAsync(info_dev + k, &info[k], sizeof(MemberInfo), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipStreamSynchronize(st));
   return PIHM_OK;
 }
 
@@ -848,7 +861,7 @@ int pihm_ensemble::flush() {
 namespace {
 
 // The model-step loop of pihm_drive for one member (src/main.c:217-254,
-// src/pihm.c:10-187, AdjCVodeMaxStep src/ode.c:601-661).
+// src/pihm.c:10-187, AdjCVodeMaxStep src/ode.c:601-661), up to model step k_end.
 Task<int> drive(Member &m, const pihm_run_config &cfg, long k_end) {
   const long et_every = std::max(1L, (long)std::llround(cfg.etstep / cfg.stepsize));
   int flag = 0;
@@ -900,22 +913,22 @@ void reset_member(Member &m, const pihm_run_config &cfg) {
   m.maxl = (cfg.maxl <= 0 || cfg.maxl > kMaxL) ? kMaxL : cfg.maxl;
 }
 
-}  // namespace
+bool cfg_ok(const pihm_run_config *cfg) {
+  return cfg && cfg->stepsize > 0.0 && cfg->etstep > 0.0 && cfg->initstep > 0.0 && cfg->maxl <= kMaxL;
+}
 
-extern "C" {
+bool forcing_ok(const pihm_forcing &f, int ne) {
+  if (f.nrec < 2 || f.nstation < 1 || !(f.rec_dt > 0.0) || !f.prcp || !f.edir || !f.ett || !f.station) return false;
+  for (int i = 0; i < ne; i++)
+    if (f.station[i] < 0 || f.station[i] >= f.nstation) return false;
+  return true;
+}
 
-int pihm_ens_create(pihm_ctx *c, const pihm_forcing *forc, const pihm_run_config *cfg, pihm_ensemble **out) {
-  if (!c || !forc || !cfg || !out) return PIHM_ERR_ARG;
-  if (!(cfg->stepsize > 0.0) || !(cfg->etstep > 0.0) || !(cfg->initstep > 0.0)) return PIHM_ERR_ARG;
-  if (cfg->maxl > kMaxL) return PIHM_ERR_ARG;
+// Allocate an ensemble over every slot of a context, slot k able to hold
+// forcing records of up to nrec_cap[k] records and nst_cap[k] stations.
+int ens_alloc(pihm_ctx *c, const int *nrec_cap, const int *nst_cap, const pihm_run_config *cfg,
+              pihm_ensemble **out) {
   const int nmem = c->nmem;
-  for (int k = 0; k < nmem; k++) {
-    const pihm_forcing &f = forc[k];
-    if (f.nrec < 2 || f.nstation < 1 || !(f.rec_dt > 0.0) || !f.prcp || !f.edir || !f.ett || !f.station)
-      return PIHM_ERR_ARG;
-    for (int i = 0; i < c->members[k].ne; i++)
-      if (f.station[i] < 0 || f.station[i] >= f.nstation) return PIHM_ERR_ARG;
-  }
   pihm_ensemble *e = new (std::nothrow) pihm_ensemble();
   if (!e) return PIHM_ERR_ARG;
   e->ctx = c;
@@ -923,30 +936,26 @@ int pihm_ens_create(pihm_ctx *c, const pihm_forcing *forc, const pihm_run_config
   e->cfg = *cfg;
   e->nmem = nmem;
   e->info.resize(nmem);
+  e->nrec_cap.assign(nrec_cap, nrec_cap + nmem);
+  e->nst_cap.assign(nst_cap, nst_cap + nmem);
   long fb = 0;
   int rb = 0, cb = 0;
   for (int k = 0; k < nmem; k++) {
     const auto &M = c->members[k];
     MemberInfo &I = e->info[k];
+    std::memset(&I, 0, sizeof(I));
     I.yoff = M.yoff;
-    I.neq = M.neq;
-    I.ne = M.ne;
     I.eoff = M.eoff;
-    I.nch = (M.neq + kChunk - 1) / kChunk;
     I.cbase = cb;
-    cb += I.nch;
+    cb += (3 * c->caps[k].ne + 2 * c->caps[k].nr + kChunk - 1) / kChunk;
     I.fbase = fb;
-    fb += (long)forc[k].nstation * forc[k].nrec;
-    I.nrec = forc[k].nrec;
-    I.nstation = forc[k].nstation;
+    fb += (long)nst_cap[k] * nrec_cap[k];
     I.rec_off = rb;
-    rb += forc[k].nrec;
-    I.rec_dt = forc[k].rec_dt;
+    rb += nrec_cap[k];
+    I.rec_dt = 1.0;
   }
   e->total_chunks = cb;
-  e->stride = (long)c->neq + 32;   // keeps each vector 256-byte aligned apart
-  e->stride = (e->stride + 31) / 32 * 32;
-  int rc = PIHM_OK;
+  e->stride = ((long)c->neq + 31) / 32 * 32 + 32;   // 256-byte aligned vectors
   auto fail = [&](int r) { pihm_ens_destroy(e); return r; };
   if (hipMalloc((void **)&e->vecs, (size_t)NVEC * e->stride * sizeof(double)) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMemsetAsync(e->vecs, 0, (size_t)NVEC * e->stride * sizeof(double), e->st) != hipSuccess) return fail(PIHM_ERR_HIP);
@@ -959,26 +968,12 @@ int pihm_ens_create(pihm_ctx *c, const pihm_forcing *forc, const pihm_run_config
   if (hipMalloc((void **)&e->info_dev, nmem * sizeof(MemberInfo)) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMemcpy(e->info_dev, e->info.data(), nmem * sizeof(MemberInfo), hipMemcpyHostToDevice) != hipSuccess)
     return fail(PIHM_ERR_HIP);
-  // forcing records (station per device element within its member)
-  std::vector<int32_t> station(c->ne);
-  std::vector<double> prcp(std::max(fb, 1L)), fe(std::max(rb, 1)), ft(std::max(rb, 1));
-  for (int k = 0; k < nmem; k++) {
-    const auto &M = c->members[k];
-    const pihm_forcing &f = forc[k];
-    for (int lp = 0; lp < M.ne; lp++) station[M.eoff + lp] = f.station[c->perm[M.eoff + lp]];
-    std::memcpy(prcp.data() + e->info[k].fbase, f.prcp, (size_t)f.nstation * f.nrec * sizeof(double));
-    std::memcpy(fe.data() + e->info[k].rec_off, f.edir, f.nrec * sizeof(double));
-    std::memcpy(ft.data() + e->info[k].rec_off, f.ett, f.nrec * sizeof(double));
-  }
-  if (hipMalloc((void **)&e->station, station.size() * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void **)&e->prcp, prcp.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&e->fedir, fe.size() * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&e->fett, ft.size() * sizeof(double)) != hipSuccess)
+  if (hipMalloc((void **)&e->station, (size_t)std::max(c->ne, 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void **)&e->prcp, (size_t)std::max(fb, 1L) * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&e->fedir, (size_t)std::max(rb, 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&e->fett, (size_t)std::max(rb, 1) * sizeof(double)) != hipSuccess)
     return fail(PIHM_ERR_HIP);
-  if (hipMemcpy(e->station, station.data(), station.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->prcp, prcp.data(), prcp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->fedir, fe.data(), fe.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->fett, ft.data(), ft.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemsetAsync(e->station, 0, (size_t)std::max(c->ne, 1) * sizeof(int32_t), e->st) != hipSuccess)
     return fail(PIHM_ERR_HIP);
   Dev &d = e->dev;
   d.base = e->vecs;
@@ -995,22 +990,150 @@ int pihm_ens_create(pihm_ctx *c, const pihm_forcing *forc, const pihm_run_config
   d.fedir = e->fedir;
   d.fett = e->fett;
   e->mem.resize(nmem);
+  e->status.assign(nmem, ST_EMPTY);
+  e->k_end.assign(nmem, 0);
+  e->tops.resize(nmem);
   for (int k = 0; k < nmem; k++) {
     Member &m = e->mem[k];
     m.id = k;
-    m.neq = c->members[k].neq;
     m.res = e->result + (long)k * kRes;
-    reset_member(m, e->cfg);
   }
-  if ((rc = e->ensure_stage(1 << 20))) return fail(rc);
+  int rc = e->ensure_stage(1 << 20);
+  if (rc) return fail(rc);
   if (hipStreamSynchronize(e->st) != hipSuccess) return fail(PIHM_ERR_HIP);
   *out = e;
   return PIHM_OK;
 }
 
+// Forcing records of slot k (station of each element in device order).
+int ens_set_forcing(pihm_ensemble *e, int k, const pihm_forcing &f) {
+  pihm_ctx *c = e->ctx;
+  const auto &M = c->members[k];
+  if (!forcing_ok(f, M.ne)) return PIHM_ERR_ARG;
+  if (f.nrec > e->nrec_cap[k] || f.nstation > e->nst_cap[k]) return PIHM_ERR_TOO_LARGE;
+  MemberInfo &I = e->info[k];
+  I.nrec = f.nrec;
+  I.nstation = f.nstation;
+  I.rec_dt = f.rec_dt;
+  std::vector<int32_t> station(std::max(M.ne, 1));
+  for (int lp = 0; lp < M.ne; lp++) station[lp] = f.station[c->perm[M.eoff + lp]];
+  hipStream_t st = e->st;
+  if (M.ne > 0)
+    PIHM_HIP_TRY(hipMemcpyAsync(e->station + M.eoff, station.data(), M.ne * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipMemcpyAsync(e->prcp + I.fbase, f.prcp, (size_t)f.nstation * f.nrec * sizeof(double),
+                              hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipMemcpyAsync(e->fedir + I.rec_off, f.edir, f.nrec * sizeof(double), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipMemcpyAsync(e->fett + I.rec_off, f.ett, f.nrec * sizeof(double), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipStreamSynchronize(st));
+  return PIHM_OK;
+}
+
+// Sizes of slot k after its context slot was (re)loaded.
+void ens_sync_sizes(pihm_ensemble *e, int k) {
+  const auto &M = e->ctx->members[k];
+  MemberInfo &I = e->info[k];
+  I.neq = M.neq;
+  I.ne = M.ne;
+  I.nch = (M.neq + kChunk - 1) / kChunk;
+  e->mem[k].neq = M.neq;
+}
+
+// Start (or continue) member k's model-step loop up to model step ke.
+void ens_start(pihm_ensemble *e, int k, long ke) {
+  Member &m = e->mem[k];
+  e->k_end[k] = ke;
+  e->tops[k] = drive(m, e->cfg, ke);
+  m.waiting = e->tops[k].h;
+  e->status[k] = ST_RUNNING;
+}
+
+}  // namespace
+
+// Advance the running members by at most max_ticks synchronisations; stops
+// early after a tick in which a member finished when stop_on_finish is set.
+// Returns the number of members that finished or failed, or an error < 0.
+static int ens_advance(pihm_ensemble *e, long max_ticks, bool stop_on_finish, int *nfailed) {
+  int nfin = 0, nfail = 0;
+  for (long t = 0; t < max_ticks; t++) {
+    bool any = false;
+    for (int k = 0; k < e->nmem; k++) {
+      Member &m = e->mem[k];
+      if (e->status[k] != ST_RUNNING || !m.waiting) continue;
+      auto h = m.waiting;
+      m.waiting = nullptr;
+      h.resume();
+      any = true;
+      if (e->tops[k].done()) {
+        m.flag = e->tops[k].result();
+        e->tops[k] = Task<int>();
+        if (m.flag < 0) { m.failed = true; e->status[k] = ST_FAILED; nfail++; }
+        else e->status[k] = ST_DONE;
+        nfin++;
+      }
+    }
+    bool queued = false;
+    for (int k = 0; k < e->nmem; k++) queued = queued || !e->mem[k].q_items.empty();
+    if (!any && !queued) break;
+    const int rc = e->flush();
+    if (rc) return rc;
+    for (int k = 0; k < e->nmem; k++) {
+      if (!e->mnan[k]) continue;
+      e->mnan[k] = 0;
+      Member &m = e->mem[k];
+      if (e->status[k] == ST_FAILED) continue;
+      if (e->status[k] == ST_RUNNING) { e->tops[k] = Task<int>(); nfin++; }
+      m.failed = true;
+      m.flag = PIHM_ERR_NAN;
+      m.waiting = nullptr;
+      e->status[k] = ST_FAILED;
+      nfail++;
+    }
+    if (stop_on_finish && nfin > 0) break;
+  }
+  if (nfailed) *nfailed = nfail;
+  return nfin;
+}
+
+extern "C" {
+
+int pihm_ens_create(pihm_ctx *c, const pihm_forcing *forc, const pihm_run_config *cfg, pihm_ensemble **out) {
+  if (!c || !forc || !out || !cfg_ok(cfg)) return PIHM_ERR_ARG;
+  const int nmem = c->nmem;
+  std::vector<int> nrec(nmem), nst(nmem);
+  for (int k = 0; k < nmem; k++) {
+    if (c->members[k].ne == 0 || !forcing_ok(forc[k], c->members[k].ne)) return PIHM_ERR_ARG;
+    nrec[k] = forc[k].nrec;
+    nst[k] = forc[k].nstation;
+  }
+  pihm_ensemble *e = nullptr;
+  int rc = ens_alloc(c, nrec.data(), nst.data(), cfg, &e);
+  if (rc) return rc;
+  for (int k = 0; k < nmem; k++) {
+    ens_sync_sizes(e, k);
+    if ((rc = ens_set_forcing(e, k, forc[k])) || (rc = e->upload_info(k))) { pihm_ens_destroy(e); return rc; }
+    reset_member(e->mem[k], e->cfg);
+    e->status[k] = ST_READY;
+  }
+  *out = e;
+  return PIHM_OK;
+}
+
+int pihm_ens_create_slots(pihm_ctx *c, const int32_t *nrec_cap, const int32_t *nstation_cap,
+                          const pihm_run_config *cfg, pihm_ensemble **out) {
+  if (!c || !nrec_cap || !nstation_cap || !out || !cfg_ok(cfg)) return PIHM_ERR_ARG;
+  std::vector<int> nrec(c->nmem), nst(c->nmem);
+  for (int k = 0; k < c->nmem; k++) {
+    if (nrec_cap[k] < 2 || nstation_cap[k] < 1) return PIHM_ERR_ARG;
+    nrec[k] = nrec_cap[k];
+    nst[k] = nstation_cap[k];
+  }
+  return ens_alloc(c, nrec.data(), nst.data(), cfg, out);
+}
+
 void pihm_ens_destroy(pihm_ensemble *e) {
   if (!e) return;
   (void)hipStreamSynchronize(e->st);
+  e->tops.clear();
   (void)hipFree(e->vecs);
   (void)hipFree(e->partials);
   if (e->result) (void)hipHostFree(e->result);
@@ -1027,6 +1150,7 @@ void pihm_ens_destroy(pihm_ensemble *e) {
 
 int pihm_ens_set_state(pihm_ensemble *e, int k, const double *y) {
   if (!e || !y || k < 0 || k >= e->nmem) return PIHM_ERR_ARG;
+  if (e->status[k] == ST_RUNNING || e->status[k] == ST_EMPTY) return PIHM_ERR_ARG;
   pihm_ctx *c = e->ctx;
   const auto &M = c->members[k];
   std::vector<double> d(M.neq);
@@ -1037,16 +1161,40 @@ int pihm_ens_set_state(pihm_ensemble *e, int k, const double *y) {
   PIHM_HIP_TRY(hipMemcpyAsync(e->P(ZN0, k), d.data(), M.neq * sizeof(double), hipMemcpyHostToDevice, e->st));
   // ws0 = initial surface state; carried river-edge fluxes start at 0 (InitVar)
   PIHM_HIP_TRY(hipMemcpyAsync(c->args.e.ws0s + M.eoff, d.data(), M.ne * sizeof(double), hipMemcpyHostToDevice, e->st));
-  PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
-  PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * c->nr + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
+  if (M.nr > 0) {
+    PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
+    PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * c->nr + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
+  }
   PIHM_HIP_TRY(hipStreamSynchronize(e->st));
-  Member &m = e->mem[k];
-  reset_member(m, e->cfg);
+  reset_member(e->mem[k], e->cfg);
+  e->status[k] = ST_READY;
+  return PIHM_OK;
+}
+
+int pihm_ens_load_member(pihm_ensemble *e, int k, const pihm_mesh_desc *mesh, const pihm_forcing *forc,
+                         const double *y0, double t_end) {
+  if (!e || !mesh || !forc || !y0 || k < 0 || k >= e->nmem) return PIHM_ERR_ARG;
+  if (e->status[k] == ST_RUNNING) return PIHM_ERR_ARG;
+  const double ks = t_end / e->cfg.stepsize;
+  const long ke = std::llround(ks);
+  if (ke < 1 || std::fabs(ks - (double)ke) > 1e-9 * ks) return PIHM_ERR_ARG;
+  if (mesh->stepsize != e->cfg.stepsize) return PIHM_ERR_ARG;
+  e->status[k] = ST_EMPTY;
+  int rc = pihm_ctx_load_member(e->ctx, k, mesh);
+  if (rc) return rc;
+  ens_sync_sizes(e, k);
+  if ((rc = ens_set_forcing(e, k, *forc))) return rc;
+  if ((rc = e->upload_info(k))) return rc;
+  e->status[k] = ST_READY;
+  if ((rc = pihm_ens_set_state(e, k, y0))) return rc;
+  e->mem[k].enabled = true;
+  e->mem[k].failed = false;
+  ens_start(e, k, ke);
   return PIHM_OK;
 }
 
 int pihm_ens_get_state(pihm_ensemble *e, int k, double *y) {
-  if (!e || !y || k < 0 || k >= e->nmem) return PIHM_ERR_ARG;
+  if (!e || !y || k < 0 || k >= e->nmem || e->status[k] == ST_EMPTY) return PIHM_ERR_ARG;
   pihm_ctx *c = e->ctx;
   const auto &M = c->members[k];
   std::vector<double> d(M.neq);
@@ -1067,51 +1215,35 @@ int pihm_ens_set_member_enabled(pihm_ensemble *e, int k, int on) {
 int pihm_ens_run(pihm_ensemble *e, double t_end) {
   if (!e) return PIHM_ERR_ARG;
   const double ks = t_end / e->cfg.stepsize;
-  const long k_end = std::llround(ks);
-  if (std::fabs(ks - (double)k_end) > 1e-9 * std::max(1.0, std::fabs(ks)) || k_end < 0) return PIHM_ERR_ARG;
-  std::vector<Task<int>> tops(e->nmem);
-  std::vector<char> live(e->nmem, 0);
+  const long ke = std::llround(ks);
+  if (std::fabs(ks - (double)ke) > 1e-9 * std::max(1.0, std::fabs(ks)) || ke < 0) return PIHM_ERR_ARG;
   for (int k = 0; k < e->nmem; k++) {
     Member &m = e->mem[k];
-    if (!m.enabled || m.failed || m.ksteps >= k_end) continue;
-    tops[k] = drive(m, e->cfg, k_end);
-    m.waiting = tops[k].h;
-    live[k] = 1;
+    if (e->status[k] == ST_RUNNING || e->status[k] == ST_EMPTY || e->status[k] == ST_FAILED) continue;
+    if (!m.enabled || m.failed || m.ksteps >= ke) continue;
+    ens_start(e, k, ke);
   }
-  int nfail = 0;
+  int nfail_total = 0;
   for (;;) {
-    bool any = false;
-    for (int k = 0; k < e->nmem; k++) {
-      Member &m = e->mem[k];
-      if (!live[k] || !m.waiting) continue;
-      auto h = m.waiting;
-      m.waiting = nullptr;
-      h.resume();
-      any = true;
-      if (tops[k].done()) {
-        live[k] = 0;
-        m.flag = tops[k].result();
-        if (m.flag < 0) { m.failed = true; nfail++; }
-      }
-    }
-    bool queued = false;
-    for (int k = 0; k < e->nmem; k++) queued = queued || !e->mem[k].q_items.empty();
-    if (!any && !queued) break;
-    int rc = e->flush();
-    if (rc) return rc;
-    for (int k = 0; k < e->nmem; k++) {
-      if (!e->mnan[k]) continue;
-      e->mnan[k] = 0;
-      Member &m = e->mem[k];
-      if (m.failed) continue;
-      m.failed = true;
-      m.flag = PIHM_ERR_NAN;
-      m.waiting = nullptr;
-      if (live[k]) { tops[k] = Task<int>(); live[k] = 0; }
-      nfail++;
-    }
+    int nf = 0;
+    const int r = ens_advance(e, 1L << 40, false, &nf);
+    if (r < 0) return r;
+    nfail_total += nf;
+    bool running = false;
+    for (int k = 0; k < e->nmem; k++) running = running || e->status[k] == ST_RUNNING;
+    if (!running) break;
   }
-  return nfail;
+  return nfail_total;
+}
+
+int pihm_ens_advance(pihm_ensemble *e, long max_ticks, int *nfailed) {
+  if (!e || max_ticks < 1) return PIHM_ERR_ARG;
+  return ens_advance(e, max_ticks, true, nfailed);
+}
+
+int pihm_ens_member_status(const pihm_ensemble *e, int k) {
+  if (!e || k < 0 || k >= e->nmem) return PIHM_ERR_ARG;
+  return e->status[k];
 }
 
 int pihm_ens_member_stats(pihm_ensemble *e, int k, pihm_run_stats *s) {

@@ -86,6 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
   if (!ix.active) return;
   const double *y = ix.y;
   const int lp = p - ix.eoff;
+  if (B && lp >= ix.ne) return;     // unused capacity of a slot
   const ElemDev &e = a.e;
 
   const double s = clamp0(y[lp]);
@@ -135,6 +136,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const Ix ix = ix_elem<B>(a, p);
   if (!ix.active) return;
   const int lp = p - ix.eoff;
+  if (B && lp >= ix.ne) return;     // unused capacity of a slot
   const double *y = ix.y;
   double *ydot = ix.ydot;
   const NbrRec me = e.rec[p];
@@ -311,6 +313,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   const RiverDev &R = a.r;
   const Ix ix = ix_river<B>(a, r);
   if (!ix.active) return;
+  if (B && r - ix.roff >= ix.nr) return;            // unused capacity of a slot
   const double *ys = ix.y + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
   const double *yg = ys + ix.nr;                           // yg[r']: river groundwater
   const double stage = clamp0(ys[r]);
@@ -495,9 +498,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
     const int r = t;
     const Ix ix = ix_river<B>(a, r);
     if (!ix.active) return;
+    if (B && r - ix.roff >= ix.nr) return;
     // src/river_flow.c:96-110 in ascending upstream index
     double up = R.upinit[r], upa = 0.0;
-    for (int k = R.up_ptr[r]; k < R.up_ptr[r + 1]; k++) {
+    for (int k = R.up_beg[r]; k < R.up_end[r]; k++) {
       const int u = R.up_idx[k];
       up -= R.rf[1 * nr + u];
       upa -= R.rf[9 * nr + u];
@@ -523,6 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
   } else if (t < nr + a.nbank) {
     const int k = t - nr;
     const int p = a.e.bank_elem[k];
+    if (p < 0) return;                              // unused bank record of a slot
     const Ix ix = ix_elem<B>(a, p);
     if (!ix.active) return;
     const BankRec b = a.e.bank[k];
@@ -643,110 +648,56 @@ int validate(const pihm_mesh_desc *m) {
 
 namespace {
 
-// Build a context for one or more solves (members).  Arrays of all members
-// are concatenated; each member's elements are renumbered along the Hilbert
-// curve of its own centroids and its state occupies one contiguous range.
-int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
+inline int bank_cap(const SlotCap &s) { return std::min(s.ne, 2 * s.nr); }
+
+// Allocate a context with one slot per entry of caps.  Element, river, bank,
+// upstream-list and state ranges of the slots are concatenated at their
+// capacities; slots start empty (no elements, no rivers) until a mesh is
+// loaded into them (ctx_load).
+int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pihm_ctx **out) {
   *out = nullptr;
-  const int nmem = (int)ms.size();
+  const int nmem = (int)caps.size();
   if (nmem < 1) return PIHM_ERR_ARG;
-  for (auto m : ms) {
-    int rc = validate(m);
-    if (rc != PIHM_OK) return rc;
-    if (m->surf_mode != ms[0]->surf_mode || m->riv_mode != ms[0]->riv_mode) return PIHM_ERR_ARG;
+  if (surf_mode != KINEMATIC && surf_mode != DIFF_WAVE) return PIHM_ERR_ARG;
+  if (riv_mode != KINEMATIC && riv_mode != DIFF_WAVE) return PIHM_ERR_ARG;
+  long ne = 0, nr = 0, neq = 0, nbank = 0, nup = 0;
+  for (const auto &s : caps) {
+    if (s.ne < 1 || s.nr < 0) return PIHM_ERR_ARG;
+    ne += s.ne; nr += s.nr; neq += 3L * s.ne + 2L * s.nr; nbank += bank_cap(s); nup += s.nr;
   }
+  if (ne > INT32_MAX / 4 || nr > INT32_MAX / 16 || neq > INT32_MAX) return PIHM_ERR_TOO_LARGE;
   pihm_ctx *c = new pihm_ctx();
   c->nmem = nmem;
+  c->caps = caps;
   c->members.resize(nmem);
-  long ne = 0, nr = 0, neq = 0;
+  c->boff.resize(nmem);
+  c->uoff.resize(nmem);
+  long e0 = 0, r0 = 0, y0 = 0, b0 = 0, u0 = 0;
   for (int k = 0; k < nmem; k++) {
     auto &M = c->members[k];
-    M.eoff = (int)ne; M.ne = ms[k]->nelem;
-    M.roff = (int)nr; M.nr = ms[k]->nriver;
-    M.yoff = neq; M.neq = 3 * M.ne + 2 * M.nr;
-    M.dt = ms[k]->stepsize;
-    ne += M.ne; nr += M.nr; neq += M.neq;
+    M.eoff = (int)e0; M.ne = 0;
+    M.roff = (int)r0; M.nr = 0;
+    M.yoff = y0; M.neq = 0;
+    M.ceoff = 0; M.croff = 0; M.cyoff = 0;
+    M.dt = 60.0;
+    c->boff[k] = (int)b0;
+    c->uoff[k] = (int)u0;
+    e0 += caps[k].ne; r0 += caps[k].nr; y0 += 3L * caps[k].ne + 2L * caps[k].nr;
+    b0 += bank_cap(caps[k]); u0 += caps[k].nr;
   }
-  if (ne > INT32_MAX / 4 || nr > INT32_MAX / 16) { delete c; return PIHM_ERR_TOO_LARGE; }
-  c->ne = (int)ne; c->nr = (int)nr; c->neq = (int)neq;
-  c->surf_mode = ms[0]->surf_mode; c->riv_mode = ms[0]->riv_mode; c->dt = ms[0]->stepsize;
+  c->ne = (int)ne; c->nr = (int)nr; c->neq = (int)neq; c->nbank = (int)nbank;
+  c->surf_mode = surf_mode; c->riv_mode = riv_mode; c->dt = 60.0;
   c->stream = nullptr; c->nan_check = 1; c->record = 0; c->ncalls = 0;
+  c->arena = nullptr;
+  c->nanflag_host = nullptr;
   PIHM_HIP_TRY(hipGetDevice(&c->device));
-
-  // per-member Hilbert order
-  c->perm.resize(ne);
-  c->iperm.resize(ne);
+  c->perm.assign(ne, -1);
+  c->iperm.assign(ne, -1);
   c->emem_host.resize(ne);
-  for (int k = 0; k < nmem; k++) {
-    const pihm_mesh_desc *m = ms[k];
-    const int off = c->members[k].eoff, n = m->nelem;
-    double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
-    for (int i = 1; i < n; i++) {
-      xmin = std::min(xmin, m->x[i]); xmax = std::max(xmax, m->x[i]);
-      ymin = std::min(ymin, m->y[i]); ymax = std::max(ymax, m->y[i]);
-    }
-    const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
-    std::vector<uint64_t> key(n);
-    for (int i = 0; i < n; i++) {
-      const double fx = (m->x[i] - xmin) / span, fy = (m->y[i] - ymin) / span;
-      const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fx * 65535.0)));
-      const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fy * 65535.0)));
-      key[i] = hilbert_key(qx, qy);
-    }
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-    for (int lp = 0; lp < n; lp++) {
-      c->perm[off + lp] = order[lp];
-      c->iperm[off + order[lp]] = off + lp;
-      c->emem_host[off + lp] = k;
-    }
-  }
+  c->nb_host.assign(3L * ne, NB_BOUND);
+  for (int k = 0; k < nmem; k++)
+    for (int lp = 0; lp < caps[k].ne; lp++) c->emem_host[c->members[k].eoff + lp] = k;
 
-  // neighbour codes, bank records, upstream lists (global indices)
-  c->nb_host.assign(3L * ne, 0);
-  std::vector<int32_t> bank_id(ne, -1), bank_elem;
-  std::vector<int32_t> rmem(std::max<long>(nr, 1), 0);
-  std::vector<int32_t> up_ptr(nr + 1, 0), up_idx;
-  for (int k = 0; k < nmem; k++) {
-    const pihm_mesh_desc *m = ms[k];
-    const auto &M = c->members[k];
-    for (int lp = 0; lp < M.ne; lp++) {
-      const int p = M.eoff + lp, i = c->perm[p];
-      bool bank = false;
-      for (int j = 0; j < 3; j++) {
-        const int v = m->nabr[3 * i + j];
-        int32_t code;
-        if (v > 0) code = c->iperm[M.eoff + v - 1];
-        else if (v == 0) code = NB_BOUND;
-        else {
-          const int r = -v - 1;
-          const int side = (m->leftele[r] == i + 1) ? 0 : 1;
-          code = -(2 * (M.roff + r) + side + 1);
-          bank = true;
-        }
-        c->nb_host[(long)j * ne + p] = code;
-      }
-      if (bank) { bank_id[p] = (int32_t)bank_elem.size(); bank_elem.push_back(p); }
-    }
-    for (int r = 0; r < M.nr; r++) {
-      rmem[M.roff + r] = k;
-      if (m->down[r] > 0) up_ptr[M.roff + m->down[r]]++;
-    }
-  }
-  c->nbank = (int)bank_elem.size();
-  for (long r = 0; r < nr; r++) up_ptr[r + 1] += up_ptr[r];
-  up_idx.assign(up_ptr[nr] > 0 ? up_ptr[nr] : 1, 0);
-  {
-    std::vector<int32_t> fill(up_ptr.begin(), up_ptr.end() - 1);
-    for (int k = 0; k < nmem; k++) {
-      const auto &M = c->members[k];
-      for (int r = 0; r < M.nr; r++)   // ascending caller index within the member
-        if (ms[k]->down[r] > 0) up_idx[fill[M.roff + ms[k]->down[r] - 1]++] = M.roff + r;
-    }
-  }
-
-  // device allocation
   RhsArgs &A = c->args;
   ElemDev &E = A.e;
   RiverDev &R = A.r;
@@ -761,13 +712,13 @@ int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
   ar.add(&E.bc_type, 3L * ne);
   ar.add(&E.bank_id, ne);
   ar.add(&E.rec, ne);
-  ar.add(&E.bank, std::max(c->nbank, 1));
-  ar.add(&E.bank_elem, std::max(c->nbank, 1));
+  const long nb1 = std::max<long>(nbank, 1);
+  ar.add(&E.bank, nb1);
+  ar.add(&E.bank_elem, nb1);
   const long nr1 = std::max<long>(nr, 1);
-  int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord};
+  int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord, &R.up_beg, &R.up_end};
   for (auto p : ri) ar.add(p, nr1);
-  ar.add(&R.up_ptr, nr + 1);
-  ar.add(&R.up_idx, up_idx.size());
+  ar.add(&R.up_idx, std::max<long>(nup, 1));
   double **rd[] = {&R.area, &R.zmin, &R.zmax, &R.zbed, &R.node_zmax, &R.dist_l, &R.dist_r,
                    &R.depth, &R.coeff, &R.length, &R.width, &R.rough, &R.cwr, &R.ksath,
                    &R.ksatv, &R.bedthick, &R.porosity, &R.bc, &R.upinit};
@@ -785,115 +736,41 @@ int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
   ar.add(&emem_dev, ne);
   ar.add(&rmem_dev, nr1);
   c->arena_bytes = ar.total();
-  PIHM_HIP_TRY(hipMalloc(&c->arena, c->arena_bytes));
-  PIHM_HIP_TRY(hipMemset(c->arena, 0, c->arena_bytes));
+  int rc = PIHM_OK;
+  if (hipMalloc(&c->arena, c->arena_bytes) != hipSuccess) rc = PIHM_ERR_HIP;
+  if (!rc && hipMemset(c->arena, 0, c->arena_bytes) != hipSuccess) rc = PIHM_ERR_HIP;
+  if (!rc && hipHostMalloc((void **)&c->nanflag_host, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    rc = PIHM_ERR_HIP;
+  if (rc) { pihm_ctx_destroy(c); return rc; }
   ar.carve((char *)c->arena);
-  PIHM_HIP_TRY(hipHostMalloc((void **)&c->nanflag_host, sizeof(int), hipHostMallocDefault));
   *c->nanflag_host = 0;
-
-  // element statics in device order
-  std::vector<double> hd(3L * ne);
-  auto cat1 = [&](int field) {
-    for (int k = 0; k < nmem; k++) {
-      const pihm_mesh_desc *m = ms[k];
-      const double *es[] = {m->area, m->zmin, m->zmax, m->depth, m->ksath, m->ksatv, m->kinfv,
-                            m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
-                            m->areafv, m->areafh, m->rough, m->rzd};
-      const auto &M = c->members[k];
-      for (int lp = 0; lp < M.ne; lp++) hd[M.eoff + lp] = es[field][c->perm[M.eoff + lp]];
-    }
-  };
-  for (int f = 0; f < 18; f++) {
-    cat1(f);
-    PIHM_HIP_TRY(hipMemcpy(*ed[f], hd.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+  // empty slots: no banks, empty upstream lists, boundary codes
+  std::vector<int32_t> neg(nb1, -1);
+  std::vector<int32_t> rm(nr1, 0), ub(nr1, 0);
+  for (int k = 0; k < nmem; k++)
+    for (int r = 0; r < caps[k].nr; r++) { rm[c->members[k].roff + r] = k; ub[c->members[k].roff + r] = c->uoff[k]; }
+  std::vector<int32_t> bid(ne, -1);
+  if (hipMemcpy(E.bank_elem, neg.data(), nb1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(E.bank_id, bid.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(E.nb, c->nb_host.data(), 3L * ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(emem_dev, c->emem_host.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(rmem_dev, rm.data(), nr1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(R.up_beg, ub.data(), nr1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(R.up_end, ub.data(), nr1 * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    pihm_ctx_destroy(c);
+    return PIHM_ERR_HIP;
   }
-  auto cat3 = [&](int field) {
-    for (int k = 0; k < nmem; k++) {
-      const pihm_mesh_desc *m = ms[k];
-      const double *es[] = {m->edge, m->nabrdist, m->nabr_x, m->nabr_y};
-      const auto &M = c->members[k];
-      for (int j = 0; j < 3; j++)
-        for (int lp = 0; lp < M.ne; lp++)
-          hd[(long)j * ne + M.eoff + lp] = es[field][3L * c->perm[M.eoff + lp] + j];
-    }
-  };
-  double *e3d[] = {E.edge, E.nabrdist, E.nabr_x, E.nabr_y};
-  for (int f = 0; f < 4; f++) {
-    cat3(f);
-    PIHM_HIP_TRY(hipMemcpy(e3d[f], hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice));
-  }
-  {
-    std::vector<int32_t> bt(3L * ne);
-    for (int k = 0; k < nmem; k++) {
-      const auto &M = c->members[k];
-      for (int j = 0; j < 3; j++)
-        for (int lp = 0; lp < M.ne; lp++)
-          bt[(long)j * ne + M.eoff + lp] = ms[k]->bc_type[3L * c->perm[M.eoff + lp] + j];
-    }
-    PIHM_HIP_TRY(hipMemcpy(E.bc_type, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(E.nb, c->nb_host.data(), c->nb_host.size() * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(E.bank_id, bank_id.data(), ne * 4, hipMemcpyHostToDevice));
-    if (c->nbank)
-      PIHM_HIP_TRY(hipMemcpy(E.bank_elem, bank_elem.data(), c->nbank * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(emem_dev, c->emem_host.data(), ne * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(rmem_dev, rmem.data(), nr1 * 4, hipMemcpyHostToDevice));
-  }
-  {
-    std::vector<NbrRec> rec(ne);
-    std::memset(rec.data(), 0, ne * sizeof(NbrRec));
-    for (int k = 0; k < nmem; k++) {
-      const auto &M = c->members[k];
-      for (int lp = 0; lp < M.ne; lp++) {
-        rec[M.eoff + lp].rough = ms[k]->rough[c->perm[M.eoff + lp]];
-        rec[M.eoff + lp].zmax = ms[k]->zmax[c->perm[M.eoff + lp]];
-      }
-    }
-    PIHM_HIP_TRY(hipMemcpy(E.rec, rec.data(), ne * sizeof(NbrRec), hipMemcpyHostToDevice));
-  }
-  if (nr > 0) {
-    std::vector<int32_t> down(nr), left(nr), right(nr), bct(nr), ordv(nr);
-    std::vector<double> rv(nr);
-    for (int k = 0; k < nmem; k++) {
-      const pihm_mesh_desc *m = ms[k];
-      const auto &M = c->members[k];
-      for (int r = 0; r < M.nr; r++) {
-        const int g = M.roff + r;
-        down[g] = (m->down[r] > 0) ? M.roff + m->down[r] - 1 : m->down[r];
-        left[g] = c->iperm[M.eoff + m->leftele[r] - 1];
-        right[g] = c->iperm[M.eoff + m->rightele[r] - 1];
-        bct[g] = m->riverbc_type[r];
-        ordv[g] = m->intrpl_ord[r];
-      }
-    }
-    PIHM_HIP_TRY(hipMemcpy(R.down, down.data(), nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.left, left.data(), nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.right, right.data(), nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.bctype, bct.data(), nr * 4, hipMemcpyHostToDevice));
-    PIHM_HIP_TRY(hipMemcpy(R.ord, ordv.data(), nr * 4, hipMemcpyHostToDevice));
-    for (int f = 0; f < 17; f++) {
-      for (int k = 0; k < nmem; k++) {
-        const pihm_mesh_desc *m = ms[k];
-        const double *rs[] = {m->rarea, m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left,
-                              m->dist_right, m->rdepth, m->coeff, m->length, m->width, m->rrough,
-                              m->cwr, m->rksath, m->rksatv, m->bedthick, m->rporosity};
-        const auto &M = c->members[k];
-        for (int r = 0; r < M.nr; r++) rv[M.roff + r] = rs[f][r];
-      }
-      PIHM_HIP_TRY(hipMemcpy(*rd[f], rv.data(), nr * sizeof(double), hipMemcpyHostToDevice));
-    }
-  }
-  PIHM_HIP_TRY(hipMemcpy(R.up_ptr, up_ptr.data(), (nr + 1) * 4, hipMemcpyHostToDevice));
-  PIHM_HIP_TRY(hipMemcpy(R.up_idx, up_idx.data(), up_idx.size() * 4, hipMemcpyHostToDevice));
-
   c->mem_host.resize(nmem);
   for (int k = 0; k < nmem; k++) {
     const auto &M = c->members[k];
-    c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, 0};
+    c->mem_host[k] = MemberDev{M.yoff, M.eoff, 0, M.roff, 0, M.dt, 0, 0};
   }
-  PIHM_HIP_TRY(hipMemcpy(c->mem_dev, c->mem_host.data(), nmem * sizeof(MemberDev), hipMemcpyHostToDevice));
-
+  if (hipMemcpy(c->mem_dev, c->mem_host.data(), nmem * sizeof(MemberDev), hipMemcpyHostToDevice) != hipSuccess) {
+    pihm_ctx_destroy(c);
+    return PIHM_ERR_HIP;
+  }
   A.ne = (int)ne; A.nr = (int)nr; A.nbank = c->nbank;
-  A.surf_mode = c->surf_mode; A.riv_mode = c->riv_mode; A.dt = c->dt;
+  A.surf_mode = surf_mode; A.riv_mode = riv_mode; A.dt = c->dt;
   A.cur = 0; A.record = 0;
   A.nanflag = c->nanflag_dev;
   A.nmem = nmem;
@@ -907,13 +784,229 @@ int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
   return PIHM_OK;
 }
 
+// Caller-order offsets of the loaded slots: members are concatenated in slot
+// order at their actual sizes.
+void update_caller_offsets(pihm_ctx *c) {
+  long e = 0, r = 0, y = 0;
+  for (auto &M : c->members) {
+    M.ceoff = (int)e; M.croff = (int)r; M.cyoff = y;
+    e += M.ne; r += M.nr; y += M.neq;
+  }
+  c->ne_loaded = (int)e; c->nr_loaded = (int)r; c->neq_loaded = y;
+}
+
+// Load one mesh into slot k: its elements are renumbered along the Hilbert
+// curve of its own centroids and every array of the slot is written.  The
+// carried river-edge fluxes of the slot restart at 0.  Asynchronous uploads
+// on the context's stream; returns after they complete.
+int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
+  if (!c || k < 0 || k >= c->nmem) return PIHM_ERR_ARG;
+  int rc = validate(m);
+  if (rc != PIHM_OK) return rc;
+  if (m->surf_mode != c->surf_mode || m->riv_mode != c->riv_mode) return PIHM_ERR_ARG;
+  const SlotCap cap = c->caps[k];
+  const int n = m->nelem, nrk = m->nriver;
+  if (n > cap.ne || nrk > cap.nr) return PIHM_ERR_TOO_LARGE;
+  auto &M = c->members[k];
+  const int off = M.eoff, roff = M.roff;
+  const long ne = c->ne, nr = c->nr;
+
+  // Hilbert order of the centroids
+  double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
+  for (int i = 1; i < n; i++) {
+    xmin = std::min(xmin, m->x[i]); xmax = std::max(xmax, m->x[i]);
+    ymin = std::min(ymin, m->y[i]); ymax = std::max(ymax, m->y[i]);
+  }
+  const double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
+  std::vector<uint64_t> key(n);
+  for (int i = 0; i < n; i++) {
+    const double fx = (m->x[i] - xmin) / span, fy = (m->y[i] - ymin) / span;
+    const uint32_t qx = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fx * 65535.0)));
+    const uint32_t qy = (uint32_t)std::min(65535.0, std::max(0.0, std::floor(fy * 65535.0)));
+    key[i] = hilbert_key(qx, qy);
+  }
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  for (int lp = 0; lp < cap.ne; lp++) {
+    c->iperm[off + lp] = -1;
+    c->perm[off + lp] = (lp < n) ? order[lp] : -1;
+  }
+  for (int lp = 0; lp < n; lp++) c->iperm[off + order[lp]] = off + lp;
+
+  // neighbour codes and banks
+  const int nbc = bank_cap(cap);
+  std::vector<int32_t> nbk(3L * cap.ne, NB_BOUND), bank_id(cap.ne, -1), bank_elem(std::max(nbc, 1), -1);
+  int nb_used = 0;
+  for (int lp = 0; lp < n; lp++) {
+    const int i = order[lp];
+    bool bank = false;
+    for (int j = 0; j < 3; j++) {
+      const int v = m->nabr[3 * i + j];
+      int32_t code;
+      if (v > 0) code = c->iperm[off + v - 1];
+      else if (v == 0) code = NB_BOUND;
+      else {
+        const int r = -v - 1;
+        const int side = (m->leftele[r] == i + 1) ? 0 : 1;
+        code = -(2 * (roff + r) + side + 1);
+        bank = true;
+      }
+      nbk[(long)j * cap.ne + lp] = code;
+    }
+    if (bank) {
+      if (nb_used >= nbc) return PIHM_ERR_TOO_LARGE;
+      bank_id[lp] = c->boff[k] + nb_used;
+      bank_elem[nb_used] = off + lp;
+      nb_used++;
+    }
+  }
+  for (int j = 0; j < 3; j++)
+    for (int lp = 0; lp < cap.ne; lp++) c->nb_host[(long)j * ne + off + lp] = nbk[(long)j * cap.ne + lp];
+  // upstream lists in ascending caller index (src/river_flow.c:96-110)
+  std::vector<int32_t> ub(std::max(cap.nr, 1)), ue(std::max(cap.nr, 1)), uidx(std::max(cap.nr, 1), 0);
+  {
+    std::vector<int32_t> cnt(cap.nr + 1, 0);
+    for (int r = 0; r < nrk; r++)
+      if (m->down[r] > 0) cnt[m->down[r]]++;
+    for (int r = 0; r < cap.nr; r++) cnt[r + 1] += cnt[r];
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < nrk; r++)
+      if (m->down[r] > 0) uidx[fill[m->down[r] - 1]++] = roff + r;
+    for (int r = 0; r < cap.nr; r++) {
+      ub[r] = c->uoff[k] + cnt[r];
+      ue[r] = c->uoff[k] + cnt[r + 1];
+    }
+  }
+
+  RhsArgs &A = c->args;
+  ElemDev &E = A.e;
+  RiverDev &R = A.r;
+  hipStream_t st = c->stream;
+  std::vector<double> hd(3L * cap.ne, 0.0);
+  const double *es[] = {m->area, m->zmin, m->zmax, m->depth, m->ksath, m->ksatv, m->kinfv,
+                        m->dinf, m->alpha, m->beta, m->porosity, m->dmac, m->kmach, m->kmacv,
+                        m->areafv, m->areafh, m->rough, m->rzd};
+  double *ed[] = {E.area, E.zmin, E.zmax, E.depth, E.ksath, E.ksatv, E.kinfv, E.dinf,
+                  E.alpha, E.beta, E.porosity, E.dmac, E.kmach, E.kmacv, E.areafv,
+                  E.areafh, E.rough, E.rzd};
+  for (int f = 0; f < 18; f++) {
+    for (int lp = 0; lp < n; lp++) hd[lp] = es[f][order[lp]];
+    PIHM_HIP_TRY(hipMemcpyAsync(ed[f] + off, hd.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipStreamSynchronize(st));
+  }
+  // per-step inputs of the slot start at 0
+  std::fill(hd.begin(), hd.end(), 0.0);
+  double *z1[] = {E.pcpdrp, E.edir, E.ett, E.ws0s};
+  for (auto p : z1) PIHM_HIP_TRY(hipMemcpyAsync(p + off, hd.data(), cap.ne * sizeof(double), hipMemcpyHostToDevice, st));
+  for (int j = 0; j < 3; j++)
+    PIHM_HIP_TRY(hipMemcpyAsync(E.ebc + (long)j * ne + off, hd.data(), cap.ne * sizeof(double), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipStreamSynchronize(st));
+  const double *e3s[] = {m->edge, m->nabrdist, m->nabr_x, m->nabr_y};
+  double *e3d[] = {E.edge, E.nabrdist, E.nabr_x, E.nabr_y};
+  for (int f = 0; f < 4; f++) {
+    for (int j = 0; j < 3; j++) {
+      for (int lp = 0; lp < n; lp++) hd[lp] = e3s[f][3L * order[lp] + j];
+      PIHM_HIP_TRY(hipMemcpyAsync(e3d[f] + (long)j * ne + off, hd.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+      PIHM_HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  std::vector<int32_t> bt(cap.ne, 0);
+  for (int j = 0; j < 3; j++) {
+    for (int lp = 0; lp < n; lp++) bt[lp] = m->bc_type[3L * order[lp] + j];
+    PIHM_HIP_TRY(hipMemcpyAsync(E.bc_type + (long)j * ne + off, bt.data(), n * 4, hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipMemcpyAsync(E.nb + (long)j * ne + off, nbk.data() + (long)j * cap.ne, cap.ne * 4,
+                                hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipStreamSynchronize(st));
+  }
+  PIHM_HIP_TRY(hipMemcpyAsync(E.bank_id + off, bank_id.data(), cap.ne * 4, hipMemcpyHostToDevice, st));
+  if (nbc > 0)
+    PIHM_HIP_TRY(hipMemcpyAsync(E.bank_elem + c->boff[k], bank_elem.data(), nbc * 4, hipMemcpyHostToDevice, st));
+  {
+    std::vector<NbrRec> rec(cap.ne);
+    std::memset(rec.data(), 0, cap.ne * sizeof(NbrRec));
+    for (int lp = 0; lp < n; lp++) {
+      rec[lp].rough = m->rough[order[lp]];
+      rec[lp].zmax = m->zmax[order[lp]];
+    }
+    PIHM_HIP_TRY(hipMemcpyAsync(E.rec + off, rec.data(), cap.ne * sizeof(NbrRec), hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipStreamSynchronize(st));
+  }
+  if (cap.nr > 0) {
+    std::vector<int32_t> down(cap.nr, -3), left(cap.nr, 0), right(cap.nr, 0), bct(cap.nr, 0), ordv(cap.nr, 1);
+    for (int r = 0; r < nrk; r++) {
+      down[r] = (m->down[r] > 0) ? roff + m->down[r] - 1 : m->down[r];
+      left[r] = c->iperm[off + m->leftele[r] - 1];
+      right[r] = c->iperm[off + m->rightele[r] - 1];
+      bct[r] = m->riverbc_type[r];
+      ordv[r] = m->intrpl_ord[r];
+    }
+    int32_t *rdst[] = {R.down, R.left, R.right, R.bctype, R.ord, R.up_beg, R.up_end, R.up_idx + c->uoff[k]};
+    const int32_t *rsrc[] = {down.data(), left.data(), right.data(), bct.data(), ordv.data(), ub.data(), ue.data(),
+                             uidx.data()};
+    for (int f = 0; f < 8; f++) {
+      int32_t *dst = (f < 7) ? rdst[f] + roff : rdst[f];
+      PIHM_HIP_TRY(hipMemcpyAsync(dst, rsrc[f], cap.nr * 4, hipMemcpyHostToDevice, st));
+    }
+    const double *rs[] = {m->rarea, m->rzmin, m->rzmax, m->zbed, m->node_zmax, m->dist_left,
+                          m->dist_right, m->rdepth, m->coeff, m->length, m->width, m->rrough,
+                          m->cwr, m->rksath, m->rksatv, m->bedthick, m->rporosity};
+    double *rdd[] = {R.area, R.zmin, R.zmax, R.zbed, R.node_zmax, R.dist_l, R.dist_r,
+                     R.depth, R.coeff, R.length, R.width, R.rough, R.cwr, R.ksath,
+                     R.ksatv, R.bedthick, R.porosity};
+    std::vector<double> rv(cap.nr, 1.0);
+    for (int f = 0; f < 17; f++) {
+      for (int r = 0; r < nrk; r++) rv[r] = rs[f][r];
+      PIHM_HIP_TRY(hipMemcpyAsync(rdd[f] + roff, rv.data(), cap.nr * sizeof(double), hipMemcpyHostToDevice, st));
+      PIHM_HIP_TRY(hipStreamSynchronize(st));
+    }
+    std::vector<double> zr(2L * cap.nr, 0.0);
+    PIHM_HIP_TRY(hipMemcpyAsync(R.bc + roff, zr.data(), cap.nr * sizeof(double), hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipMemcpyAsync(R.hid_ovl + 2L * roff, zr.data(), 2L * cap.nr * sizeof(double), hipMemcpyHostToDevice, st));
+    PIHM_HIP_TRY(hipMemcpyAsync(R.hid_ovl + 2L * nr + 2L * roff, zr.data(), 2L * cap.nr * sizeof(double),
+                                hipMemcpyHostToDevice, st));
+  }
+  M.ne = n;
+  M.nr = nrk;
+  M.neq = 3 * n + 2 * nrk;
+  M.dt = m->stepsize;
+  c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, 0};
+  PIHM_HIP_TRY(hipMemcpyAsync(c->mem_dev + k, &c->mem_host[k], sizeof(MemberDev), hipMemcpyHostToDevice, st));
+  PIHM_HIP_TRY(hipStreamSynchronize(st));
+  if (k == 0) { c->dt = m->stepsize; c->args.dt = m->stepsize; }
+  update_caller_offsets(c);
+  return PIHM_OK;
+}
+
+// Build a context for one or more solves (members), each slot exactly the
+// size of its mesh.
+int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
+  *out = nullptr;
+  if (ms.empty()) return PIHM_ERR_ARG;
+  std::vector<SlotCap> caps;
+  for (auto m : ms) {
+    int rc = validate(m);
+    if (rc != PIHM_OK) return rc;
+    if (m->surf_mode != ms[0]->surf_mode || m->riv_mode != ms[0]->riv_mode) return PIHM_ERR_ARG;
+    caps.push_back(SlotCap{m->nelem, m->nriver});
+  }
+  pihm_ctx *c = nullptr;
+  int rc = ctx_alloc(caps, ms[0]->surf_mode, ms[0]->riv_mode, &c);
+  if (rc) return rc;
+  for (int k = 0; k < (int)ms.size(); k++) {
+    if ((rc = ctx_load(c, k, ms[k]))) { pihm_ctx_destroy(c); return rc; }
+  }
+  *out = c;
+  return PIHM_OK;
+}
+
 // Apply f(member, host_base, device_base) for the member-major caller-order
 // layout of a concatenated state: perm maps device -> caller within members.
 void state_permute(const pihm_ctx *c, const double *src, double *dst, bool to_device) {
   for (int k = 0; k < c->nmem; k++) {
     const auto &M = c->members[k];
-    const double *s = src + M.yoff;
-    double *d = dst + M.yoff;
+    const double *s = src + (to_device ? M.cyoff : M.yoff);
+    double *d = dst + (to_device ? M.yoff : M.cyoff);
     for (int b = 0; b < 3; b++)
       for (int lp = 0; lp < M.ne; lp++) {
         const long dv = (long)b * M.ne + lp, hv = (long)b * M.ne + c->perm[M.eoff + lp];
@@ -936,6 +1029,18 @@ extern "C" int pihm_ctx_create_batch(int nmem, const pihm_mesh_desc *const *mesh
   if (!out || nmem < 1 || !meshes) return PIHM_ERR_ARG;
   std::vector<const pihm_mesh_desc *> ms(meshes, meshes + nmem);
   return build_ctx(ms, out);
+}
+
+extern "C" int pihm_ctx_create_slots(int nslots, const int32_t *ne_cap, const int32_t *nr_cap, int surf_mode,
+                                     int riv_mode, pihm_ctx **out) {
+  if (!out || nslots < 1 || !ne_cap || !nr_cap) return PIHM_ERR_ARG;
+  std::vector<SlotCap> caps(nslots);
+  for (int k = 0; k < nslots; k++) caps[k] = SlotCap{ne_cap[k], nr_cap[k]};
+  return ctx_alloc(caps, surf_mode, riv_mode, out);
+}
+
+extern "C" int pihm_ctx_load_member(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
+  return ctx_load(c, k, m);
 }
 
 extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
@@ -990,6 +1095,20 @@ extern "C" int pihm_ctx_get_perm(const pihm_ctx *c, int32_t *out) {
   return PIHM_OK;
 }
 
+// Caller-order index of device element p / device river r (the loaded slots
+// concatenated at their actual sizes), or -1 for unused slot capacity.
+static inline long caller_elem(const pihm_ctx *c, int p) {
+  if (c->perm[p] < 0) return -1;
+  return (long)c->members[c->emem_host[p]].ceoff + c->perm[p];
+}
+static inline long caller_river(const pihm_ctx *c, int r) {
+  for (int k = 0; k < c->nmem; k++) {
+    const auto &M = c->members[k];
+    if (r >= M.roff && r < M.roff + c->caps[k].nr) return (r - M.roff < M.nr) ? M.croff + (r - M.roff) : -1;
+  }
+  return -1;
+}
+
 extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in) {
   if (!c || !in) return PIHM_ERR_ARG;
   const int ne = c->ne, nr = c->nr;
@@ -997,7 +1116,10 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   // element arrays are concatenated over members in caller order: member k's
   // element i sits at members[k].eoff + i
   auto up1 = [&](double *dst, const double *src) -> int {
-    for (int p = 0; p < ne; p++) hd[p] = src[c->members[c->emem_host[p]].eoff + c->perm[p]];
+    for (int p = 0; p < ne; p++) {
+      const long ci = caller_elem(c, p);
+      hd[p] = (ci >= 0) ? src[ci] : 0.0;
+    }
     PIHM_HIP_TRY(hipMemcpyAsync(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
     return PIHM_OK;
@@ -1011,12 +1133,17 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   if (in->elem_bc) {
     for (int j = 0; j < 3; j++)
       for (int p = 0; p < ne; p++)
-        hd[(long)j * ne + p] = in->elem_bc[3L * (c->members[c->emem_host[p]].eoff + c->perm[p]) + j];
+        hd[(long)j * ne + p] = (caller_elem(c, p) >= 0) ? in->elem_bc[3L * caller_elem(c, p) + j] : 0.0;
     PIHM_HIP_TRY(hipMemcpyAsync(E.ebc, hd.data(), 3L * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
   }
   if (in->river_bc && nr > 0) {
-    PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.bc, in->river_bc, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> rb(nr, 0.0);
+    for (int r = 0; r < nr; r++) {
+      const long cr = caller_river(c, r);
+      if (cr >= 0) rb[r] = in->river_bc[cr];
+    }
+    PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.bc, rb.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return PIHM_OK;
@@ -1110,10 +1237,6 @@ extern "C" int PIHM_ODE_Hip(realtype t, N_Vector y, N_Vector ydot, void *user_da
   return PIHM_OK;
 }
 
-// element index in the concatenated caller order of device element p
-static inline long caller_elem(const pihm_ctx *c, int p) {
-  return (long)c->members[c->emem_host[p]].eoff + c->perm[p];
-}
 
 extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
   if (!c || !ovl3) return PIHM_ERR_ARG;
@@ -1128,7 +1251,7 @@ extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
       const int code = c->nb_host[(long)j * ne + p];
       double v = 0.0;
       if (code < 0 && code != NB_BOUND) v = hid[-code - 1];
-      ovl3[3L * caller_elem(c, p) + j] = v;
+      if (caller_elem(c, p) >= 0) ovl3[3L * caller_elem(c, p) + j] = v;
     }
   return PIHM_OK;
 }
@@ -1141,7 +1264,7 @@ extern "C" int pihm_ctx_set_river_edge_ovlflow(pihm_ctx *c, const double *ovl3) 
   for (int p = 0; p < ne; p++)
     for (int j = 0; j < 3; j++) {
       const int code = c->nb_host[(long)j * ne + p];
-      if (code < 0 && code != NB_BOUND) hid[-code - 1] = ovl3[3L * caller_elem(c, p) + j];
+      if (code < 0 && code != NB_BOUND && caller_elem(c, p) >= 0) hid[-code - 1] = ovl3[3L * caller_elem(c, p) + j];
     }
   PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.hid_ovl, hid.data(), 2L * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1163,25 +1286,32 @@ extern "C" int pihm_ctx_get_fluxes(pihm_ctx *c, double *ovl, double *sub, double
   if (ovl) {
     if ((rc = get(c->args.rec.ovl, 3L * ne))) return rc;
     for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) ovl[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
+      for (int p = 0; p < ne; p++)
+        if (caller_elem(c, p) >= 0) ovl[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
   }
   if (sub) {
     if ((rc = get(c->args.rec.sub, 3L * ne))) return rc;
     for (int j = 0; j < 3; j++)
-      for (int p = 0; p < ne; p++) sub[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
+      for (int p = 0; p < ne; p++)
+        if (caller_elem(c, p) >= 0) sub[3L * caller_elem(c, p) + j] = h[(long)j * ne + p];
   }
   if (infil) {
     if ((rc = get(c->args.rec.infil, ne))) return rc;
-    for (int p = 0; p < ne; p++) infil[caller_elem(c, p)] = h[p];
+    for (int p = 0; p < ne; p++)
+      if (caller_elem(c, p) >= 0) infil[caller_elem(c, p)] = h[p];
   }
   if (rechg) {
     if ((rc = get(c->args.rec.rechg, ne))) return rc;
-    for (int p = 0; p < ne; p++) rechg[caller_elem(c, p)] = h[p];
+    for (int p = 0; p < ne; p++)
+      if (caller_elem(c, p) >= 0) rechg[caller_elem(c, p)] = h[p];
   }
   if (rivflow && nr > 0) {
     if ((rc = get(c->args.r.rf, 11L * nr))) return rc;
-    for (int r = 0; r < nr; r++)
-      for (int k = 0; k < 11; k++) rivflow[11L * r + k] = h[(long)k * nr + r];
+    for (int r = 0; r < nr; r++) {
+      const long cr = caller_river(c, r);
+      if (cr < 0) continue;
+      for (int k = 0; k < 11; k++) rivflow[11L * cr + k] = h[(long)k * nr + r];
+    }
   }
   return PIHM_OK;
 }

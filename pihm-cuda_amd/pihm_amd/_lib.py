@@ -99,6 +99,8 @@ _PROTOS = [
     ("pihm_ctx_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     ("pihm_ctx_destroy", None, [C.c_void_p]),
     ("pihm_ctx_create_batch", C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshDesc)), C.POINTER(C.c_void_p)]),
+    ("pihm_ctx_create_slots", C.c_int, [C.c_int, c_int32_p, c_int32_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("pihm_ctx_load_member", C.c_int, [C.c_void_p, C.c_int, C.POINTER(MeshDesc)]),
     ("pihm_ctx_nmembers", C.c_int, [C.c_void_p]),
     ("pihm_ctx_member_info", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -200,6 +202,11 @@ _PROTOS = [
     ("pihm_ens_run", C.c_int, [C.c_void_p, C.c_double]),
     ("pihm_ens_member_stats", C.c_int, [C.c_void_p, C.c_int, C.POINTER(RunStats)]),
     ("pihm_ens_counters", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("pihm_ens_create_slots", C.c_int, [C.c_void_p, c_int32_p, c_int32_p, C.POINTER(RunConfig), C.POINTER(C.c_void_p)]),
+    ("pihm_ens_load_member", C.c_int, [C.c_void_p, C.c_int, C.POINTER(MeshDesc), C.POINTER(Forcing), c_double_p,
+                                       C.c_double]),
+    ("pihm_ens_advance", C.c_int, [C.c_void_p, C.c_long, C.POINTER(C.c_int)]),
+    ("pihm_ens_member_status", C.c_int, [C.c_void_p, C.c_int]),
 ]
 
 
