@@ -39,6 +39,9 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     if rc:
         raise RuntimeError(f"pihm_ctx_create: {rc}")
     L.pihm_ctx_set_nan_check(ctx, 0)
+    stream = C.c_void_p()
+    assert L.pihm_stream_create(C.byref(stream)) == 0
+    L.pihm_ctx_set_stream(ctx, stream)
     fs = forcing.struct()
     gb = C.c_void_p()
     assert L.pihm_gpu_backend_create(ctx, C.byref(fs), C.byref(gb)) == 0
@@ -46,6 +49,8 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     L.pihm_gpu_backend_get(gb, C.byref(be))
     y = L.N_VNew_Hip(mesh.neq)
     yd = L.N_VNew_Hip(mesh.neq)
+    L.N_VSetStream_Hip(y, stream)
+    L.N_VSetStream_Hip(yd, stream)
     y0 = mesh.initial_state()
     assert L.pihm_ctx_state_to_device(ctx, y0.ctypes.data_as(_lib.c_double_p), y) == 0
     _lib.SummaryFn(be.summary)(be.user, y)        # ws0 = initial surface state
@@ -79,6 +84,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     L.N_VDestroy_Hip(yd)
     L.pihm_gpu_backend_destroy(gb)
     L.pihm_ctx_destroy(ctx)
+    L.pihm_stream_destroy(stream)
     return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t3],
                 yend=yend)
 
@@ -138,6 +144,71 @@ def ensemble_run(nmem, hours, sample, seed):
         speedup_vs_one_at_a_time=(member_days / wall) / (member_days / est_single_total))
 
 
+SERVE_CLASSES = [(8192, 48), (32768, 32), (131072, 12)]   # slot capacity (elements) x count
+
+
+def _serve_payload(spec, hours):
+    from pihm_amd.workloads import _build, member_forcing
+    m = _build(spec)
+    f = member_forcing(spec, m, hours + 2, p_wet=0.1)
+    return m, f, m.initial_state()
+
+
+def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
+    """C4: a Poisson stream of solves served with continuous batching on a slot ensemble."""
+    from concurrent.futures import ProcessPoolExecutor
+    from pihm_amd.serve import Request, Scheduler, SlotEngine, latency_summary, slot_bytes
+    from pihm_amd.workloads import request_stream, tree_segments
+    stream = request_stream(duration_s=duration_s, rate=rate, seed=seed, hmin_days=1.0, hmax_days=30.0)
+    # horizon: U(1, 30) units of horizon_unit_s, rounded to whole 60 s model steps
+    for rq in stream:
+        rq["t_end"] = max(60.0, round(rq["horizon_days"] * horizon_unit_s / 60.0) * 60.0)
+    max_hours = max(rq["t_end"] for rq in stream) / 3600.0
+    nrec_cap = int(max_hours) + 4
+    caps = []
+    for ne, cnt in SERVE_CLASSES:
+        caps += [(ne, tree_segments(ne), nrec_cap)] * cnt
+    hbm = sum(slot_bytes(ne, nr) for ne, nr, _ in caps)
+    workers = min(16, len(os.sched_getaffinity(0)))
+    pool = ProcessPoolExecutor(max_workers=workers)
+    futs = [pool.submit(_serve_payload, rq, rq["t_end"] / 3600.0) for rq in stream]
+    reqs = [Request(i, rq["arrival"], rq["t_end"], futs[i]) for i, rq in enumerate(stream)]
+    eng = SlotEngine(caps, run_config(3600.0))
+    t0 = time.perf_counter()
+    clock = lambda: time.perf_counter() - t0
+    sched = Scheduler(eng, clock, ticks_per_round=8)
+    sched.run(reqs, sleep=time.sleep)
+    wall = clock()
+    cnt = eng.counters()
+    done = [r for r in reqs if r.t_done is not None and not r.failed]
+    member_days = sum(r.t_end for r in done) / 86400.0
+    lat = latency_summary(done)
+    # alone vs in the mix, bit for bit, for a few requests of different sizes
+    eng.close()
+    same = 0
+    order = sorted(range(len(reqs)), key=lambda i: stream[i]["n_tri"])
+    pick = [order[int(round(q * (len(order) - 1)))] for q in np.linspace(0.0, 1.0, check_alone)] if check_alone else []
+    for i in pick:
+        m, f, y0 = futs[i].result()
+        cap = [c for c in caps if c[0] >= m.nelem][0]
+        e1 = SlotEngine([cap], run_config(3600.0))
+        r1 = Request(0, 0.0, reqs[i].t_end, (m, f, y0))
+        Scheduler(e1, time.perf_counter, ticks_per_round=64).run([r1])
+        e1.close()
+        same += int(np.array_equal(r1.result, reqs[i].result))
+    pool.shutdown(cancel_futures=True)
+    return dict(
+        metric="request latency p50/p99 (s) and member-days/s under continuous batching",
+        p50_s=lat.get("p50_s"), p99_s=lat.get("p99_s"), mean_s=lat.get("mean_s"), max_s=lat.get("max_s"),
+        value=member_days / wall, unit="member-days/s", requests=len(reqs), completed=len(done),
+        failed=sum(1 for r in reqs if r.failed), wall_s=wall,
+        stream={"rate_per_s": rate, "duration_s": duration_s, "sizes": "log-uniform 1,000-100,000 triangles",
+                "horizon": f"U(1, 30) x {horizon_unit_s:g} s (the request spec's days scaled to "
+                           f"{horizon_unit_s:g} s so the offered load fits one GPU)"},
+        slots={"classes": [{"elements": ne, "count": n} for ne, n in SERVE_CLASSES], "hbm_bytes": hbm},
+        alone_equals_mix=f"{same}/{len(pick)} bit-identical", **cnt)
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -194,6 +265,12 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-hours", type=float, default=1.0)
     ap.add_argument("--ensemble-sample", type=int, default=16)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-horizon-unit", type=float, default=60.0,
+                    help="seconds of simulated time per horizon unit (the request spec's unit is a day)")
+    ap.add_argument("--serve-check", type=int, default=3)
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only run N RHS evaluations and N copy-kernel launches (counter passes)")
     args = ap.parse_args()
@@ -258,6 +335,9 @@ def main():
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(args.ensemble_members, args.ensemble_hours, args.ensemble_sample, 2024)
+    if not args.no_serve:
+        out["serving"] = serve_run(args.serve_seconds, args.serve_rate, args.serve_horizon_unit,
+                                   args.serve_check, 77)
     print(json.dumps(out))
 
 

@@ -62,6 +62,14 @@ int   PihmCVodeSetMaxNumSteps(void *mem, long int mxsteps);
 int   PihmCVodeSetMaxOrd(void *mem, int maxord);
 int   PihmCVodeSetStopTime(void *mem, realtype tstop);
 int   PihmCVodeSetStabLimDet(void *mem, int onoff);
+/* Declares that the RHS only enqueues device work on the vectors' stream (no
+ * host-side effects, no synchronisation, result independent of t), as
+ * PIHM_ODE_Hip with the NaN check off does.  The integrator then captures
+ * each Krylov iteration (basis vector, RHS, Arnoldi step) once in a HIP graph
+ * and replays it; device vectors on a non-null stream are needed, otherwise
+ * (or if capture fails) it runs the same steps launch by launch.  Results are
+ * the same bits either way.  No counterpart in CVODE. */
+int   PihmCVodeSetGraphRhs(void *mem, int onoff);
 int   PihmCVSpgmr(void *mem, int pretype, int maxl);
 int   PihmCVode(void *mem, realtype tout, N_Vector yout, realtype *tret, int itask);
 int   PihmCVodeGetDky(void *mem, realtype t, int k, N_Vector dky);

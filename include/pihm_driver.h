@@ -35,6 +35,7 @@ typedef struct pihm_run_config {
   double nncfn, nnimax, nnimin, decr, incr, stmin;   /* NUM_NONCOV_FAIL .. MIN_MAXSTEP */
   int maxl;                     /* SPGMR dimension (0 = 5) */
   long max_steps_per_call;      /* CVodeSetMaxNumSteps; 0 = stepsize*10 as the reference */
+  int graphs;                   /* pihm_run_gpu: replay Krylov iterations from HIP graphs */
 } pihm_run_config;
 
 typedef struct pihm_forcing {

@@ -127,6 +127,10 @@ int  pihm_ctx_create_slots(int nslots, const int32_t *ne_cap, const int32_t *nr_
                            int riv_mode, pihm_ctx **out);
 int  pihm_ctx_load_member(pihm_ctx *ctx, int slot, const pihm_mesh_desc *mesh);
 int  pihm_ctx_set_stream(pihm_ctx *ctx, void *hip_stream);
+/* A HIP stream of default priority for callers without a HIP toolchain
+ * (graph replay needs a stream other than the null stream). */
+int  pihm_stream_create(void **hip_stream);
+int  pihm_stream_destroy(void *hip_stream);
 int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
 int  pihm_ctx_neq(const pihm_ctx *ctx);
 

@@ -10,6 +10,8 @@
 // the vectors are device vectors (pihm_nvector.h), groups of operations are
 // fused into one launch each; the fused kernels round exactly like the
 // operation sequences they replace.
+#include <hip/hip_runtime_api.h>
+
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -39,6 +41,12 @@ struct Mem {
   int maxl = 5;
   double eplifac = 0.05;
   bool fused = false;
+  // Krylov iterations replayed from graphs (device vectors on a non-null
+  // stream and an RHS declared capturable, PihmCVodeSetGraphRhs)
+  bool graph_rhs = false, graph_off = false;
+  hipGraphExec_t gx[MAXL_CAP] = {};
+  double *kry = nullptr;        // device: copies of the Krylov sums and gamma
+  double *gamma_h = nullptr;    // pinned staging of gamma
 
   N_Vector zn[6] = {};
   N_Vector ewt = nullptr, y = nullptr, acor = nullptr, tempv = nullptr, ftemp = nullptr;
@@ -75,7 +83,19 @@ inline void VDestroy(N_Vector v) { if (v) v->ops->nvdestroy(v); }
 
 bool is_hip(N_Vector v) { return v->ops->nvgetvectorid == N_VGetVectorID_Hip; }
 
+void free_graphs(Mem *m) {
+  for (auto &g : m->gx) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
 void free_vectors(Mem *m) {
+  free_graphs(m);
+  if (m->kry) (void)hipFree(m->kry);
+  if (m->gamma_h) (void)hipHostFree(m->gamma_h);
+  m->kry = nullptr;
+  m->gamma_h = nullptr;
   for (auto &v : m->zn) { VDestroy(v); v = nullptr; }
   N_Vector *vs[] = {&m->ewt, &m->y, &m->acor, &m->tempv, &m->ftemp, &m->xcor, &m->vtemp, &m->ytemp, &m->xs};
   for (auto p : vs) { VDestroy(*p); *p = nullptr; }
@@ -93,6 +113,11 @@ bool alloc_spgmr(Mem *m) {
   for (int j = 0; j <= m->maxl; j++) if (!(m->V[j] = VClone(m->zn[0]))) return false;
   N_Vector *vs[] = {&m->xcor, &m->vtemp, &m->ytemp, &m->xs};
   for (auto p : vs) if (!(*p = VClone(m->zn[0]))) return false;
+  if (m->fused && !m->kry) {
+    if (hipMalloc((void **)&m->kry, PIHM_KRY_SIZE * sizeof(double)) != hipSuccess) return false;
+    if (hipMemset(m->kry, 0, PIHM_KRY_SIZE * sizeof(double)) != hipSuccess) return false;
+    if (hipHostMalloc((void **)&m->gamma_h, sizeof(double), hipHostMallocDefault) != hipSuccess) return false;
+  }
   return true;
 }
 
@@ -317,6 +342,30 @@ int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
 // sig = 1/||V_l/s||_WRMS(s) = sqrt(N) for the unit basis vector V_l.
 enum { SP_ZERO = 10 };
 
+// Capture Krylov iteration l (prep from the device sums, the RHS at y + sig*v,
+// the Arnoldi step) on the vectors' stream.  False (and graphs off for this
+// integrator) when the stream is the null stream or capture fails.
+bool build_graph(Mem *m, int l) {
+  hipStream_t st = (hipStream_t)pihm_nv_stream(m->ewt);
+  if (!st || !m->kry) { m->graph_off = true; return false; }
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) { m->graph_off = true; return false; }
+  int bad = pihm_gm_prep_dev(l, m->V, m->kry, m->y, m->ewt, m->sqrtN, m->ytemp);
+  if (!bad) bad = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
+  if (!bad) bad = pihm_gm_arnoldi(l, m->V, m->ewt, m->vtemp, m->ftemp, 1.0 / m->sqrtN, 0.0, nullptr, m->kry);
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  if (bad || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    m->graph_off = true;
+    return false;
+  }
+  const hipError_t ei = hipGraphInstantiate(&m->gx[l], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) { m->gx[l] = nullptr; m->graph_off = true; return false; }
+  return true;
+}
+
 int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
   N_Vector *V = m->V;
   N_Vector s = m->ewt;
@@ -333,15 +382,28 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
   const double siginv = 1.0 / sig;
   bool converged = false;
   int krydim = 0;
+  bool graph = m->graph_rhs && !m->graph_off && m->kry;
+  if (graph && pihm_gm_set_gamma(m->kry, m->gamma_h, m->gamma, s)) return SP_ATIMES_FAIL_UNREC;
   for (int l = 0; l < m->maxl; l++) {
     krydim = l + 1;
-    if (pihm_gm_prep(l, V, lazy, cscale, m->ycur, s, sig, m->ytemp)) return SP_ATIMES_FAIL_UNREC;
-    const int rv = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
-    m->nfes++;
-    if (rv < 0) return SP_ATIMES_FAIL_UNREC;
-    if (rv > 0) return SP_ATIMES_FAIL_REC;
     double sums[MAXL_CAP + 2];
-    if (pihm_gm_arnoldi(l, V, s, m->vtemp, m->fcur, siginv, m->gamma, sums)) return SP_ATIMES_FAIL_UNREC;
+    if (graph && !m->gx[l]) graph = build_graph(m, l);
+    if (graph) {
+      // the replayed iteration computes the same coefficients on the device
+      hipStream_t st = (hipStream_t)pihm_nv_stream(s);
+      if (hipGraphLaunch(m->gx[l], st) != hipSuccess) return SP_ATIMES_FAIL_UNREC;
+      m->nfes++;
+      pihm_redwork w = pihm_nv_redwork(s);
+      if (pihm_red_wait(w, st)) return SP_ATIMES_FAIL_UNREC;
+      for (int k = 0; k <= l + 1; k++) sums[k] = w.result[k];
+    } else {
+      if (pihm_gm_prep(l, V, lazy, cscale, m->ycur, s, sig, m->ytemp)) return SP_ATIMES_FAIL_UNREC;
+      const int rv = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
+      m->nfes++;
+      if (rv < 0) return SP_ATIMES_FAIL_UNREC;
+      if (rv > 0) return SP_ATIMES_FAIL_REC;
+      if (pihm_gm_arnoldi(l, V, s, m->vtemp, m->fcur, siginv, m->gamma, sums, m->kry)) return SP_ATIMES_FAIL_UNREC;
+    }
     const double ww = sums[l + 1];
     double hsq = 0.0;
     for (int k = 0; k <= l; k++) {
@@ -352,7 +414,7 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
     double nn = ww - hsq;
     if (!(nn > 1e-6 * ww)) {
       double s2[MAXL_CAP + 2];
-      if (pihm_gm_orth(l, V, lazy, s2)) return SP_GS_FAIL;
+      if (pihm_gm_orth(l, V, lazy, s2, m->kry)) return SP_GS_FAIL;
       double hsq2 = 0.0;
       for (int k = 0; k <= l; k++) {
         m->Hes[k][l] += s2[k];
@@ -385,7 +447,7 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
   const double N = (double)m->neq;
   for (;;) {
     double sumsq = 0.0;
-    if (pihm_gm_start(m->rl1, m->zn[1], m->acor, m->gamma, m->ftemp, m->ewt, m->tempv, m->V[0], &sumsq))
+    if (pihm_gm_start(m->rl1, m->zn[1], m->acor, m->gamma, m->ftemp, m->ewt, m->tempv, m->V[0], &sumsq, m->kry))
       return PIHM_CV_LSOLVE_FAIL;
     const double deltar = m->eplifac * m->tq[4];
     const double bnorm = std::sqrt(sumsq / N);
@@ -756,6 +818,14 @@ int PihmCVodeSetStopTime(void *mem, realtype tstop) {
   m->tstop = tstop; m->tstopset = true;
   return 0;
 }
+int PihmCVodeSetGraphRhs(void *mem, int onoff) {
+  if (!mem) return PIHM_CV_MEM_NULL;
+  Mem *m = (Mem *)mem;
+  m->graph_rhs = onoff != 0;
+  m->graph_off = false;
+  free_graphs(m);
+  return PIHM_CV_SUCCESS;
+}
 int PihmCVodeSetStabLimDet(void *mem, int onoff) { if (!mem) return PIHM_CV_MEM_NULL; ((Mem *)mem)->sldet = onoff != 0; return 0; }
 
 int PihmCVSpgmr(void *mem, int pretype, int maxl) {
@@ -765,6 +835,7 @@ int PihmCVSpgmr(void *mem, int pretype, int maxl) {
   m->maxl = (maxl <= 0) ? 5 : (maxl > MAXL_CAP ? MAXL_CAP : maxl);
   m->ls_set = true;
   if (m->zn[0]) {
+    free_graphs(m);
     for (auto &v : m->V) { VDestroy(v); v = nullptr; }
     N_Vector *vs[] = {&m->xcor, &m->vtemp, &m->ytemp, &m->xs};
     for (auto p : vs) { VDestroy(*p); *p = nullptr; }

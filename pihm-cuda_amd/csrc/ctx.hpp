@@ -84,6 +84,7 @@ struct MemberDev {
   int roff, nr;         // river range
   double dt;            // ctrl.stepsize of this solve (Infil)
   int active;           // evaluated by the next RHS launch
+  int boff, nbank;      // bank-record range
   int pad;
 };
 
@@ -112,6 +113,11 @@ struct RhsArgs {
   const int32_t *emem, *rmem;  // member of each device element / river
   const MemberIO *mio;         // [nmem] per-call member table, or nullptr
   int *mnan;                   // [nmem] per-member NaN flags, or nullptr
+  // Compact launch over the active members only (nullptr: every position of
+  // the batch): bm[0..nact] element-block, bm[nact+1..2nact+1] river-block and
+  // bm[2nact+2..3nact+2] bank-block prefixes, bm_mem[i] the i-th active member.
+  const int *bm, *bm_mem;
+  int nact;
 };
 
 }  // namespace pihm
@@ -132,6 +138,7 @@ struct pihm_member_host {
   int ceoff, croff;
   long cyoff;
   double dt;
+  int nbank;
 };
 
 // Launch one batched RHS evaluation whose member states come from the device
@@ -139,6 +146,10 @@ struct pihm_member_host {
 // mnan[member].  Asynchronous on the context's stream.
 struct pihm_ctx;
 int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
+// The same over the nact members of a block map (see RhsArgs::bm); nbe, nbr,
+// nbb are the map's element, river and bank block totals.
+int pihm_rhs_launch_compact(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan, const int *bm,
+                            const int *bm_mem, int nact, int nbe, int nbr, int nbb);
 
 struct pihm_ctx {
   int ne, nr, nbank, neq;       // totals over the batch, at slot capacity

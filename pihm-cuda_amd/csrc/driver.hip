@@ -13,6 +13,9 @@
 
 using namespace pihm;
 
+static int gpu_rhs(realtype t, N_Vector y, N_Vector ydot, void *ud);
+static PihmRhsFn gpu_rhs_fn() { return gpu_rhs; }
+
 extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_config *cfg,
                           pihm_run_stats *st) {
   if (!be || !y || !cfg || !be->rhs) return PIHM_ERR_ARG;
@@ -34,6 +37,7 @@ extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_con
   if (!flag) flag = PihmCVodeSetMaxNumSteps(cv, cfg->max_steps_per_call > 0 ? cfg->max_steps_per_call
                                                                             : (long)(cfg->stepsize * 10));
   if (!flag) flag = PihmCVSpgmr(cv, 0, cfg->maxl);
+  if (!flag && cfg->graphs && be->rhs == gpu_rhs_fn()) flag = PihmCVodeSetGraphRhs(cv, 1);
   if (flag) { PihmCVodeFree(&cv); return flag; }
 
   long nst0 = 0, ncfn0 = 0, nni0 = 0;
@@ -125,6 +129,8 @@ struct pihm_gpu_backend {
 static int gpu_rhs(realtype t, N_Vector y, N_Vector ydot, void *ud) {
   return PIHM_ODE_Hip(t, y, ydot, ud);
 }
+// gpu_rhs only enqueues work on the context's stream once the NaN check is off
+// (pihm_run_gpu), so the integrator may replay it from a graph.
 
 static int gpu_forcing(void *user, double t) {
   pihm_gpu_backend *b = (pihm_gpu_backend *)user;
@@ -193,8 +199,20 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   pihm_gpu_backend *b = nullptr;
   int rc = pihm_gpu_backend_create(c, f, &b);
   if (rc) return rc;
+  // graphs need a stream of their own (capture is not possible on the null stream)
+  hipStream_t own = nullptr;
+  void *saved_stream = c->stream;
+  if (cfg->graphs && !c->stream) {
+    if (hipStreamCreate(&own) != hipSuccess) { pihm_gpu_backend_destroy(b); return PIHM_ERR_HIP; }
+    c->stream = own;
+  }
   N_Vector y = N_VNew_Hip(c->neq);
-  if (!y) { pihm_gpu_backend_destroy(b); return PIHM_ERR_HIP; }
+  if (!y) {
+    pihm_gpu_backend_destroy(b);
+    c->stream = (hipStream_t)saved_stream;
+    if (own) (void)hipStreamDestroy(own);
+    return PIHM_ERR_HIP;
+  }
   N_VSetStream_Hip(y, c->stream);
   const int saved_check = c->nan_check;
   c->nan_check = 0;   // NaN shows up in the norms; polled once at the end
@@ -214,5 +232,10 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   c->nan_check = saved_check;
   N_VDestroy_Hip(y);
   pihm_gpu_backend_destroy(b);
+  if (own) {
+    (void)hipStreamSynchronize(own);
+    c->stream = (hipStream_t)saved_stream;
+    (void)hipStreamDestroy(own);
+  }
   return rc;
 }

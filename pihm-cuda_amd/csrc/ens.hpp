@@ -103,6 +103,8 @@ struct Round {
 // per-member totals of its reductions (a second launch, one workgroup per
 // member with a reduction).
 void launch_ops(const Dev &d, const Round &r, int nblocks, hipStream_t st);
+// One-lane kernel writing ++(*ctr) to *flag (host memory) after the work queued before it.
+void launch_signal(unsigned *ctr, volatile unsigned *flag, hipStream_t st);
 
 }  // namespace ens
 }  // namespace pihm

@@ -469,6 +469,22 @@ __global__ __launch_bounds__(kChunkThreads) void k_ens_final(Dev d, Round r) {
 
 }  // namespace
 
+// End-of-tick signal: the running count of ticks goes to host memory after
+// everything queued before it on the stream.
+__global__ void k_ens_signal(unsigned *ctr, volatile unsigned *flag) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    const unsigned s = *ctr + 1u;
+    *ctr = s;
+    *flag = s;
+    __threadfence_system();
+  }
+}
+
+void launch_signal(unsigned *ctr, volatile unsigned *flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_signal, dim3(1), dim3(64), 0, st, ctr, flag);
+}
+
 void launch_ops(const Dev &d, const Round &r, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL(k_ens_ops, dim3(nblocks), dim3(kChunkThreads), 0, st, d, r);
   if (r.nred > 0) hipLaunchKernelGGL(k_ens_final, dim3(r.nred), dim3(kChunkThreads), 0, st, d, r);

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <coroutine>
 #include <cstring>
@@ -711,6 +713,9 @@ struct pihm_ensemble {
   std::vector<long> k_end;     // model steps each member runs to
   std::vector<Task<int>> tops; // running model-step loops
   long ticks = 0, op_launches = 0, rhs_launches = 0;
+  unsigned *sig_ctr = nullptr;           // device tick count
+  volatile unsigned *sig_flag = nullptr; // pinned host copy
+  unsigned sig_expect = 0;
 
   double *P(int v, int m) const { return vecs + (long)v * stride + info[m].yoff; }
   int ensure_stage(size_t bytes);
@@ -741,8 +746,9 @@ int pihm_ensemble::upload_info(int k) {
 namespace {
 struct RoundPlan {
   bool rhs;
-  size_t off_part, off_cpre, off_obeg, off_ops, off_io, off_rmem, off_rinfo;
+  size_t off_part, off_cpre, off_obeg, off_ops, off_io, off_rmem, off_rinfo, off_bm, off_bmm;
   int np, nblocks, nred;
+  int nact, nbe, nbr, nbb;
 };
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 }  // namespace
@@ -807,7 +813,7 @@ int pihm_ensemble::flush() {
     // RHS evaluations of every member whose next item is one
     std::vector<MemberIO> io(nmem);
     std::memset(io.data(), 0, nmem * sizeof(MemberIO));
-    bool anyr = false;
+    std::vector<int> act;
     for (int m = 0; m < nmem; m++) {
       auto &Q = mem[m].q_items;
       if (pos[m] >= Q.size() || !Q[pos[m]].rhs) continue;
@@ -815,13 +821,30 @@ int pihm_ensemble::flush() {
       io[m].ydot = P(Q[pos[m]].out, m);
       io[m].active = 1;
       pos[m]++;
-      anyr = true;
+      act.push_back(m);
     }
-    if (anyr) {
+    if (!act.empty()) {
+      // block map of the active members: element, river and bank-record blocks
+      const int na = (int)act.size();
+      std::vector<int> bm(3 * (na + 1), 0);
+      for (int i = 0; i < na; i++) {
+        const auto &M = ctx->members[act[i]];
+        bm[i + 1] = bm[i] + div_up(M.ne, kBlock);
+        bm[(na + 1) + i + 1] = bm[(na + 1) + i] + div_up(M.nr, kBlock);
+        bm[2 * (na + 1) + i + 1] = bm[2 * (na + 1) + i] + div_up(M.nbank, kBlock);
+      }
       RoundPlan r{};
       r.rhs = true;
+      r.nact = na;
+      r.nbe = bm[na];
+      r.nbr = bm[(na + 1) + na];
+      r.nbb = bm[2 * (na + 1) + na];
       r.off_io = reserve(nmem * sizeof(MemberIO));
       std::memcpy(buf.data() + r.off_io, io.data(), nmem * sizeof(MemberIO));
+      r.off_bm = reserve(bm.size() * sizeof(int));
+      std::memcpy(buf.data() + r.off_bm, bm.data(), bm.size() * sizeof(int));
+      r.off_bmm = reserve(act.size() * sizeof(int));
+      std::memcpy(buf.data() + r.off_bmm, act.data(), act.size() * sizeof(int));
       plans.push_back(r);
       any = true;
     }
@@ -835,7 +858,9 @@ int pihm_ensemble::flush() {
   PIHM_HIP_TRY(hipMemcpyAsync(stage_d, stage_h, buf.size(), hipMemcpyHostToDevice, st));
   for (const auto &r : plans) {
     if (r.rhs) {
-      rc = pihm_rhs_launch_io(ctx, (const MemberIO *)(stage_d + r.off_io), mnan);
+      rc = pihm_rhs_launch_compact(ctx, (const MemberIO *)(stage_d + r.off_io), mnan,
+                                   (const int *)(stage_d + r.off_bm), (const int *)(stage_d + r.off_bmm),
+                                   r.nact, r.nbe, r.nbr, r.nbb);
       if (rc) return rc;
       rhs_launches++;
     } else {
@@ -853,7 +878,21 @@ int pihm_ensemble::flush() {
       op_launches++;
     }
   }
-  PIHM_HIP_TRY(hipStreamSynchronize(st));
+  // wait for the tick: spin on the count the signal kernel writes to host
+  // memory, handing over to the runtime's wait after 2 ms
+  launch_signal(sig_ctr, sig_flag, st);
+  PIHM_HIP_TRY(hipGetLastError());
+  const unsigned want = ++sig_expect;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 1; *sig_flag != want; it++) {
+    if ((it & 255u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      PIHM_HIP_TRY(hipStreamSynchronize(st));
+      if (*sig_flag != want) return PIHM_ERR_HIP;
+      break;
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
   ticks++;
   return PIHM_OK;
 }
@@ -960,10 +999,18 @@ int ens_alloc(pihm_ctx *c, const int *nrec_cap, const int *nst_cap, const pihm_r
   if (hipMalloc((void **)&e->vecs, (size_t)NVEC * e->stride * sizeof(double)) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMemsetAsync(e->vecs, 0, (size_t)NVEC * e->stride * sizeof(double), e->st) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMalloc((void **)&e->partials, (size_t)std::max(cb, 1) * kSums * sizeof(double)) != hipSuccess) return fail(PIHM_ERR_HIP);
-  if (hipHostMalloc((void **)&e->result, (size_t)nmem * kRes * sizeof(double), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&e->result, (size_t)nmem * kRes * sizeof(double), hipHostMallocCoherent) != hipSuccess)
     return fail(PIHM_ERR_HIP);
   std::memset(e->result, 0, (size_t)nmem * kRes * sizeof(double));
-  if (hipHostMalloc((void **)&e->mnan, nmem * sizeof(int), hipHostMallocDefault) != hipSuccess) return fail(PIHM_ERR_HIP);
+  if (hipHostMalloc((void **)&e->mnan, nmem * sizeof(int), hipHostMallocCoherent) != hipSuccess) return fail(PIHM_ERR_HIP);
+  if (hipMalloc((void **)&e->sig_ctr, sizeof(unsigned)) != hipSuccess) return fail(PIHM_ERR_HIP);
+  if (hipMemset(e->sig_ctr, 0, sizeof(unsigned)) != hipSuccess) return fail(PIHM_ERR_HIP);
+  {
+    unsigned *f = nullptr;
+    if (hipHostMalloc((void **)&f, sizeof(unsigned), hipHostMallocCoherent) != hipSuccess) return fail(PIHM_ERR_HIP);
+    *f = 0u;
+    e->sig_flag = f;
+  }
   std::memset(e->mnan, 0, nmem * sizeof(int));
   if (hipMalloc((void **)&e->info_dev, nmem * sizeof(MemberInfo)) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMemcpy(e->info_dev, e->info.data(), nmem * sizeof(MemberInfo), hipMemcpyHostToDevice) != hipSuccess)
@@ -1138,6 +1185,8 @@ void pihm_ens_destroy(pihm_ensemble *e) {
   (void)hipFree(e->partials);
   if (e->result) (void)hipHostFree(e->result);
   if (e->mnan) (void)hipHostFree(e->mnan);
+  if (e->sig_ctr) (void)hipFree(e->sig_ctr);
+  if (e->sig_flag) (void)hipHostFree((void *)e->sig_flag);
   (void)hipFree(e->info_dev);
   (void)hipFree(e->station);
   (void)hipFree(e->prcp);

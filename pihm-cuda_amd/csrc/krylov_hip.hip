@@ -56,17 +56,45 @@ __global__ void k_prep(long n, PrepArgs a) {
   a.work[i] = (a.sig * vt) + a.y[i];                // LinearSum(sig, v, 1, y)
 }
 
+// k_prep with its coefficients read from the device copy of the previous
+// step's sums (kry): the same values the host computes for pihm_gm_prep, in
+// the same order, so a captured Krylov iteration needs no host arguments.
+__global__ void k_prep_dev(long n, int l, const double *kry, PrepArgs a) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double c;
+  double coef[kMaxBasis];
+  if (l == 0) {
+    c = 1.0 / sqrt(kry[PIHM_KRY_START]);            // 1 / r_norm
+  } else {
+    const double *h = kry + (long)(l - 1) * PIHM_KRY_LD;
+    double hsq = 0.0;
+    for (int k = 0; k < l; k++) { coef[k] = h[k]; hsq += h[k] * h[k]; }
+    double nn = h[l] - hsq;
+    if (nn < 0.0) nn = 0.0;
+    c = 1.0 / sqrt(nn);                             // 1 / H[l][l-1]
+  }
+  double v = a.vl[i];
+  for (int k = 0; k < l; k++) v = (-coef[k] * a.vk[k][i]) + v;
+  v = c * v;
+  a.vl[i] = v;
+  const double vt = v / a.s[i];
+  a.work[i] = (a.sig * vt) + a.y[i];
+}
+
 template <int NV>
 struct FArnoldi {
   const double *vl, *s, *jf, *fy;
   double siginv, gamma;
+  const double *gp;                                 // gamma from device memory when set
   double *w;
   const double *vk[kMaxBasis];
   __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const {
+    const double g = gp ? *gp : gamma;
     const double si = s[i];
     const double vt = vl[i] / si;
     const double jv = siginv * (jf[i] - fy[i]);     // LinearSum(siginv, Jv, -siginv, fy)
-    const double z = (-gamma * jv) + vt;            // LinearSum(1, v, -gamma, Jv)
+    const double z = (-g * jv) + vt;                // LinearSum(1, v, -gamma, Jv)
     const double wi = si * z;                       // Prod(s1, z)
     w[i] = wi;
 #pragma unroll
@@ -119,23 +147,31 @@ struct FFinish {
   }
 };
 
+// sums == nullptr: launch only (gamma from kry, nothing awaited)
 template <int NV>
 int arnoldi_n(N_Vector *V, int l, N_Vector s, N_Vector jf, N_Vector fy, double siginv, double gamma,
-              double *sums) {
+              double *sums, double *kry) {
   FArnoldi<NV> f;
   f.vl = D(V[l]); f.s = D(s); f.jf = D(jf); f.fy = D(fy);
   f.siginv = siginv; f.gamma = gamma;
+  f.gp = sums ? nullptr : kry + PIHM_KRY_GAMMA;
   f.w = D(V[l + 1]);
   for (int k = 0; k <= l; k++) f.vk[k] = D(V[k]);
-  return fused_sum<NV>(L(s), f, pihm_nv_redwork(s), S(s), sums);
+  double *dc = kry ? kry + (long)l * PIHM_KRY_LD : nullptr;
+  if (!sums) {
+    fused_sum_launch<NV>(L(s), f, pihm_nv_redwork(s), S(s), dc);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  return fused_sum<NV>(L(s), f, pihm_nv_redwork(s), S(s), sums, dc);
 }
 
 template <int NV>
-int orth_n(N_Vector *V, int l, const double *coef, double *sums) {
+int orth_n(N_Vector *V, int l, const double *coef, double *sums, double *kry) {
   FOrth<NV> f;
   f.w = D(V[l + 1]);
   for (int k = 0; k <= l; k++) { f.vk[k] = D(V[k]); f.coef[k] = coef[k]; }
-  return fused_sum<NV>(L(V[0]), f, pihm_nv_redwork(V[0]), S(V[0]), sums);
+  double *dc = kry ? kry + (long)l * PIHM_KRY_LD : nullptr;
+  return fused_sum<NV>(L(V[0]), f, pihm_nv_redwork(V[0]), S(V[0]), sums, dc);
 }
 
 }  // namespace
@@ -143,9 +179,27 @@ int orth_n(N_Vector *V, int l, const double *coef, double *sums) {
 extern "C" {
 
 int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f, N_Vector s,
-                  N_Vector b, N_Vector v0, double *sumsq) {
+                  N_Vector b, N_Vector v0, double *sumsq, double *kry) {
   FStart k{rl1, gamma, D(zn1), D(acor), D(f), D(s), D(b), D(v0)};
-  return fused_sum<1>(L(s), k, pihm_nv_redwork(s), S(s), sumsq);
+  return fused_sum<1>(L(s), k, pihm_nv_redwork(s), S(s), sumsq, kry ? kry + PIHM_KRY_START : nullptr);
+}
+
+int pihm_gm_set_gamma(double *kry, double *gamma_pinned, double gamma, N_Vector s) {
+  *gamma_pinned = gamma;
+  return hipMemcpyAsync(kry + PIHM_KRY_GAMMA, gamma_pinned, sizeof(double), hipMemcpyHostToDevice, S(s)) ==
+                 hipSuccess ? 0 : -2;
+}
+
+int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig, N_Vector work) {
+  if (l < 0 || l >= kMaxBasis) return -1;
+  PrepArgs a;
+  a.l = l; a.c = 0.0; a.sig = sig;
+  a.vl = D(V[l]);
+  for (int k = 0; k < l; k++) { a.vk[k] = D(V[k]); a.coef[k] = 0.0; }
+  a.y = D(y); a.s = D(s); a.work = D(work);
+  const long n = L(s);
+  hipLaunchKernelGGL(k_prep_dev, dim3(div_up(n, kBlock)), dim3(kBlock), 0, S(s), n, l, kry, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s, double sig,
@@ -162,36 +216,37 @@ int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N
 }
 
 int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
-                    double gamma, double *sums) {
+                    double gamma, double *sums, double *kry) {
+  if (!sums && !kry) return -1;
   switch (l + 2) {
-    case 2: return arnoldi_n<2>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 3: return arnoldi_n<3>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 4: return arnoldi_n<4>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 5: return arnoldi_n<5>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 6: return arnoldi_n<6>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 7: return arnoldi_n<7>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 8: return arnoldi_n<8>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 9: return arnoldi_n<9>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 10: return arnoldi_n<10>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 11: return arnoldi_n<11>(V, l, s, jf, fy, siginv, gamma, sums);
-    case 12: return arnoldi_n<12>(V, l, s, jf, fy, siginv, gamma, sums);
+    case 2: return arnoldi_n<2>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 3: return arnoldi_n<3>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 4: return arnoldi_n<4>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 5: return arnoldi_n<5>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 6: return arnoldi_n<6>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 7: return arnoldi_n<7>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 8: return arnoldi_n<8>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 9: return arnoldi_n<9>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 10: return arnoldi_n<10>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 11: return arnoldi_n<11>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 12: return arnoldi_n<12>(V, l, s, jf, fy, siginv, gamma, sums, kry);
     default: return -1;
   }
 }
 
-int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums) {
+int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums, double *kry) {
   switch (l + 2) {
-    case 2: return orth_n<2>(V, l, coef, sums);
-    case 3: return orth_n<3>(V, l, coef, sums);
-    case 4: return orth_n<4>(V, l, coef, sums);
-    case 5: return orth_n<5>(V, l, coef, sums);
-    case 6: return orth_n<6>(V, l, coef, sums);
-    case 7: return orth_n<7>(V, l, coef, sums);
-    case 8: return orth_n<8>(V, l, coef, sums);
-    case 9: return orth_n<9>(V, l, coef, sums);
-    case 10: return orth_n<10>(V, l, coef, sums);
-    case 11: return orth_n<11>(V, l, coef, sums);
-    case 12: return orth_n<12>(V, l, coef, sums);
+    case 2: return orth_n<2>(V, l, coef, sums, kry);
+    case 3: return orth_n<3>(V, l, coef, sums, kry);
+    case 4: return orth_n<4>(V, l, coef, sums, kry);
+    case 5: return orth_n<5>(V, l, coef, sums, kry);
+    case 6: return orth_n<6>(V, l, coef, sums, kry);
+    case 7: return orth_n<7>(V, l, coef, sums, kry);
+    case 8: return orth_n<8>(V, l, coef, sums, kry);
+    case 9: return orth_n<9>(V, l, coef, sums, kry);
+    case 10: return orth_n<10>(V, l, coef, sums, kry);
+    case 11: return orth_n<11>(V, l, coef, sums, kry);
+    case 12: return orth_n<12>(V, l, coef, sums, kry);
     default: return -1;
   }
 }

@@ -12,7 +12,9 @@
 // only on the length, so results are reproducible bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -284,12 +286,25 @@ __global__ void k_lincomb(long n, int nvec, LcArgs a, double *z) {
 }
 
 // ---------------------------------------------------------------- helpers
+// Pinned host block of a workspace: results, then the completion count.
+struct HostBlock {
+  double result[kMaxMulti];
+  unsigned flag;
+  unsigned expect;
+};
+
 int aux_new(Aux **out) {
   Aux *x = (Aux *)std::calloc(1, sizeof(Aux));
   if (!x) return -1;
+  HostBlock *hb = nullptr;
   if (hipMalloc(&x->w.partials, sizeof(double) * kMaxRedBlocks * kMaxMulti) != hipSuccess) return -1;
-  if (hipHostMalloc((void **)&x->w.result, sizeof(double) * kMaxMulti, hipHostMallocDefault) != hipSuccess)
-    return -1;
+  if (hipMalloc(&x->w.ctr, sizeof(unsigned)) != hipSuccess) return -1;
+  if (hipMemset(x->w.ctr, 0, sizeof(unsigned)) != hipSuccess) return -1;
+  if (hipHostMalloc((void **)&hb, sizeof(HostBlock), hipHostMallocCoherent) != hipSuccess) return -1;
+  std::memset(hb, 0, sizeof(HostBlock));
+  x->w.result = hb->result;
+  x->w.flag = &hb->flag;
+  x->w.expect = &hb->expect;
   x->refs = 1;
   *out = x;
   return 0;
@@ -298,7 +313,8 @@ void aux_release(Aux *x) {
   if (!x) return;
   if (--x->refs > 0) return;
   (void)hipFree(x->w.partials);
-  (void)hipHostFree(x->w.result);
+  (void)hipFree(x->w.ctr);
+  (void)hipHostFree(x->w.result);   // first member of the host block
   std::free(x);
 }
 
@@ -308,9 +324,8 @@ bool reduce(N_Vector v, RedArgs &a, double *out, int nout) {
   hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kRedBlock), 0, S(v), a);
   unsigned combs = 0;
   for (int k = 0; k < a.nvec; k++) combs |= red_comb(a.kind) << (2 * k);
-  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedBlock), 0, S(v), nb, a.nvec, combs,
-                     a.work.partials, a.work.result);
-  if (hipStreamSynchronize(S(v)) != hipSuccess) return false;
+  launch_final(nb, a.nvec, combs, a.work, S(v), nullptr);
+  if (pihm_red_wait(a.work, S(v))) return false;
   for (int k = 0; k < nout; k++) out[k] = a.work.result[k];
   return true;
 }
@@ -344,6 +359,23 @@ extern "C" {
 N_Vector_ID N_VGetVectorID_Hip(N_Vector v) { (void)v; return PIHM_NVEC_HIP; }
 
 pihm_redwork pihm_nv_redwork(N_Vector v) { return X(v)->w; }
+
+int pihm_red_wait(pihm_redwork w, void *stream) {
+  const unsigned want = ++(*w.expect);
+  // Spin on the count the second pass writes to host memory; after 2 ms hand
+  // over to the runtime's wait, which also reports a failed launch.
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 1; *w.flag != want; it++) {
+    if ((it & 255u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return -2;
+      if (*w.flag != want) return -2;
+      break;
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return 0;
+}
 void *pihm_nv_stream(N_Vector v) { return C(v)->stream; }
 
 N_Vector N_VNewEmpty_Hip(long int length) {
@@ -576,9 +608,8 @@ int pihm_nv_ewt_ss(N_Vector y, double rtol, double atol, N_Vector ewt, double *m
   RedWork w = X(y)->w;
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_ewt, dim3(nb), dim3(kRedBlock), 0, S(y), n, D(y), rtol, atol, D(ewt), w);
-  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedBlock), 0, S(y), nb, 2, COMB_MIN | (COMB_SUM << 2),
-                     w.partials, w.result);
-  if (hipStreamSynchronize(S(y)) != hipSuccess) return -2;
+  launch_final(nb, 2, COMB_MIN | (COMB_SUM << 2), w, S(y), nullptr);
+  if (pihm_red_wait(w, S(y))) return -2;
   *mn = w.result[0];
   *ynrm = std::sqrt(w.result[1] / (double)n);
   return 0;

@@ -7,14 +7,23 @@ extern "C" {
 #endif
 
 // Reduction workspace shared by a family of device vectors (clones share it).
+// The second pass of every reduction writes its results to `result` and then
+// the running count of second passes to `flag`; the host waits for that count
+// (pihm_red_wait) instead of synchronising the stream.
 typedef struct pihm_redwork {
-  double *partials;   // device, [PIHM_RED_MAXBLOCKS * PIHM_RED_MAXSUMS]
-  double *result;     // pinned host, [PIHM_RED_MAXSUMS]
+  double *partials;          // device, [PIHM_RED_MAXBLOCKS * PIHM_RED_MAXSUMS]
+  double *result;            // pinned host (coherent), [PIHM_RED_MAXSUMS]
+  unsigned *ctr;             // device: second passes completed
+  volatile unsigned *flag;   // pinned host (coherent): copy of *ctr
+  unsigned *expect;          // host: second passes launched and awaited
 } pihm_redwork;
 #define PIHM_RED_MAXBLOCKS 1024
 #define PIHM_RED_MAXSUMS 16
 pihm_redwork pihm_nv_redwork(N_Vector v);
 void *pihm_nv_stream(N_Vector v);
+// Wait until the most recently launched second pass of w has written its
+// results (spins on the pinned flag; falls back to synchronising the stream).
+int pihm_red_wait(pihm_redwork w, void *stream);
 
 // Fused Newton / SPGMR operations (krylov_hip.hip).  All vectors are device
 // vectors of one family; s is the error-weight vector (the SPGMR scalings
@@ -28,13 +37,28 @@ void *pihm_nv_stream(N_Vector v);
 //   orth:    V_{l+1} <- V_{l+1} - sum_{k<=l} coef[k] V_k; sums as arnoldi
 //   finish:  x = (sum_k ycoef[k] V_k)/s (mode 0) | b (mode 1) | 0 (mode 2);
 //            acor += x; y = zn0 + acor; sums: (x*s)^2, (acor*s)^2
+//
+// kry (optional, device, PIHM_KRY_SIZE doubles) receives a device copy of the
+// sums: start -> kry[PIHM_KRY_START], arnoldi/orth step l -> row l
+// (kry + l*PIHM_KRY_LD).  prep_dev forms V_l and the work vector with the
+// coefficients of row l-1 (or the start sum for l = 0) read from kry, as the
+// host computes them, and arnoldi with sums == NULL only launches, reading
+// gamma from kry[PIHM_KRY_GAMMA]: a Krylov iteration can then be captured in
+// a graph and replayed with no host arguments.
+#define PIHM_KRY_LD 12
+#define PIHM_KRY_START (11 * PIHM_KRY_LD)
+#define PIHM_KRY_GAMMA (PIHM_KRY_START + 1)
+#define PIHM_KRY_SIZE (PIHM_KRY_START + 4)
 int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f,
-                  N_Vector s, N_Vector b, N_Vector v0, double *sumsq);
+                  N_Vector s, N_Vector b, N_Vector v0, double *sumsq, double *kry);
 int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s,
                  double sig, N_Vector work);
+int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig,
+                     N_Vector work);
+int pihm_gm_set_gamma(double *kry, double *gamma_pinned, double gamma, N_Vector s);
 int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
-                    double gamma, double *sums);
-int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums);
+                    double gamma, double *sums, double *kry);
+int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums, double *kry);
 int pihm_gm_finish(int mode, int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector b,
                    N_Vector zn0, N_Vector acor, N_Vector y, double *sums);
 

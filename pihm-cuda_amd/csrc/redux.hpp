@@ -70,7 +70,9 @@ __device__ __forceinline__ double comb(unsigned c, double a, double b) {
 // device-wide fence in the first pass.
 namespace {
 __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, unsigned combs,
-                                                                 const double *partials, double *result) {
+                                                                 const double *partials, double *result,
+                                                                 double *devcopy, unsigned *ctr,
+                                                                 volatile unsigned *flag) {
   __shared__ double lds[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int k = 0; k < nv; k++) {
@@ -85,20 +87,41 @@ __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, 
       double r = comb_ident(c);
       for (int w2 = 0; w2 < (int)(blockDim.x >> 6); w2++) r = comb(c, r, lds[w2]);
       result[k] = r;
+      if (devcopy) devcopy[k] = r;
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // results reach host memory before the count that announces them
+    __threadfence_system();
+    const unsigned s = *ctr + 1u;
+    *ctr = s;
+    *flag = s;
+    __threadfence_system();
   }
 }
 }  // namespace
 
-// Launch both passes and wait; results in out[0..NV).
+// Launch the second pass of a reduction whose first pass wrote nb partials.
+inline void launch_final(int nb, int nv, unsigned combs, pihm_redwork w, hipStream_t st, double *devcopy) {
+  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedThreads), 0, st, nb, nv, combs, w.partials, w.result,
+                     devcopy, w.ctr, w.flag);
+}
+
+// Launch both passes without waiting (devcopy: optional device copy of the results).
 template <int NV, class F>
-int fused_sum(long n, const F &f, pihm_redwork w, hipStream_t st, double *out) {
+void fused_sum_launch(long n, const F &f, pihm_redwork w, hipStream_t st, double *devcopy) {
   const int nb = fused_blocks(n);
   hipLaunchKernelGGL((k_fused_sum<NV, F>), dim3(nb), dim3(kRedThreads), 0, st, n, f, w.partials);
-  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedThreads), 0, st, nb, NV, 0u, w.partials, w.result);
+  launch_final(nb, NV, 0u, w, st, devcopy);
+}
+
+// Launch both passes and wait; results in out[0..NV).
+template <int NV, class F>
+int fused_sum(long n, const F &f, pihm_redwork w, hipStream_t st, double *out, double *devcopy = nullptr) {
+  fused_sum_launch<NV>(n, f, w, st, devcopy);
   if (hipGetLastError() != hipSuccess) return -2;
-  if (hipStreamSynchronize(st) != hipSuccess) return -2;
+  if (pihm_red_wait(w, st)) return -2;
   for (int k = 0; k < NV; k++) out[k] = w.result[k];
   return 0;
 }

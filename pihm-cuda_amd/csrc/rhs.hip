@@ -77,11 +77,35 @@ __device__ __forceinline__ Ix ix_elem(const RhsArgs &a, int p) { return ix_of<B>
 template <bool B>
 __device__ __forceinline__ Ix ix_river(const RhsArgs &a, int r) { return ix_of<B>(a, B ? a.rmem[r] : 0); }
 
+// Largest i in [0, n) with pre[i] <= b (pre ascending).
+__device__ __forceinline__ int find_seg(const int *pre, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Position handled by this thread of block `b` of a compact launch: kind 0
+// elements, 1 rivers, 2 bank records; -1 past the member's range.
+__device__ __forceinline__ int compact_index(const RhsArgs &a, int kind, int b) {
+  const int *pre = a.bm + kind * (a.nact + 1);
+  const int i = find_seg(pre, a.nact, b);
+  const MemberDev M = a.mem[a.bm_mem[i]];
+  const int l = (b - pre[i]) * kBlock + threadIdx.x;
+  if (kind == 0) return l < M.ne ? M.eoff + l : -1;
+  if (kind == 1) return l < M.nr ? M.roff + l : -1;
+  return l < M.nbank ? M.boff + l : -1;
+}
+
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
-  const int p = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
   const int ne = a.ne;
-  if (p >= ne) return;
+  if (p < 0 || p >= ne) return;
   const Ix ix = ix_elem<B>(a, p);
   if (!ix.active) return;
   const double *y = ix.y;
@@ -481,19 +505,32 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
   if ((int)blockIdx.x < nbe) {
-    const int p = xcd_tile(blockIdx.x, nbe) * kBlock + threadIdx.x;
-    if (p < a.ne) elem_flux<B>(a, p);
+    const int tile = xcd_tile(blockIdx.x, nbe);
+    const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
+    if (p >= 0 && p < a.ne) elem_flux<B>(a, p);
   } else {
-    const int r = (blockIdx.x - nbe) * kBlock + threadIdx.x;
-    if (r < a.nr) river_flux<B>(a, r);
+    const int rb = blockIdx.x - nbe;
+    const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
+    if (r >= 0 && r < a.nr) river_flux<B>(a, r);
   }
 }
 
 template <bool B>
-__global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a, int nbr) {
+  int t = blockIdx.x * kBlock + threadIdx.x;
   const int ne = a.ne, nr = a.nr;
   const RiverDev &R = a.r;
+  if (B && a.bm) {
+    // compact launch: river blocks [0, nbr), then bank-record blocks
+    if ((int)blockIdx.x < nbr) {
+      t = compact_index(a, 1, blockIdx.x);
+      if (t < 0) return;
+    } else {
+      t = compact_index(a, 2, blockIdx.x - nbr);
+      if (t < 0) return;
+      t += nr;
+    }
+  }
   if (t < nr) {
     const int r = t;
     const Ix ix = ix_river<B>(a, r);
@@ -680,6 +717,7 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
     M.yoff = y0; M.neq = 0;
     M.ceoff = 0; M.croff = 0; M.cyoff = 0;
     M.dt = 60.0;
+    M.nbank = 0;
     c->boff[k] = (int)b0;
     c->uoff[k] = (int)u0;
     e0 += caps[k].ne; r0 += caps[k].nr; y0 += 3L * caps[k].ne + 2L * caps[k].nr;
@@ -763,7 +801,7 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   c->mem_host.resize(nmem);
   for (int k = 0; k < nmem; k++) {
     const auto &M = c->members[k];
-    c->mem_host[k] = MemberDev{M.yoff, M.eoff, 0, M.roff, 0, M.dt, 0, 0};
+    c->mem_host[k] = MemberDev{M.yoff, M.eoff, 0, M.roff, 0, M.dt, 0, c->boff[k], 0, 0};
   }
   if (hipMemcpy(c->mem_dev, c->mem_host.data(), nmem * sizeof(MemberDev), hipMemcpyHostToDevice) != hipSuccess) {
     pihm_ctx_destroy(c);
@@ -779,6 +817,9 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   A.rmem = rmem_dev;
   A.mio = nullptr;
   A.mnan = nullptr;
+  A.bm = nullptr;
+  A.bm_mem = nullptr;
+  A.nact = 0;
   c->stage_host = nullptr;
   *out = c;
   return PIHM_OK;
@@ -970,7 +1011,8 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
   M.nr = nrk;
   M.neq = 3 * n + 2 * nrk;
   M.dt = m->stepsize;
-  c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, 0};
+  M.nbank = nb_used;
+  c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, c->boff[k], nb_used, 0};
   PIHM_HIP_TRY(hipMemcpyAsync(c->mem_dev + k, &c->mem_host[k], sizeof(MemberDev), hipMemcpyHostToDevice, st));
   PIHM_HIP_TRY(hipStreamSynchronize(st));
   if (k == 0) { c->dt = m->stepsize; c->args.dt = m->stepsize; }
@@ -1053,6 +1095,21 @@ extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
 extern "C" int pihm_ctx_set_stream(pihm_ctx *c, void *s) {
   if (!c) return PIHM_ERR_ARG;
   c->stream = (hipStream_t)s;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_stream_create(void **out) {
+  if (!out) return PIHM_ERR_ARG;
+  hipStream_t s = nullptr;
+  PIHM_HIP_TRY(hipStreamCreate(&s));
+  *out = s;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_stream_destroy(void *s) {
+  if (!s) return PIHM_ERR_ARG;
+  PIHM_HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  PIHM_HIP_TRY(hipStreamDestroy((hipStream_t)s));
   return PIHM_OK;
 }
 
@@ -1183,11 +1240,11 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
   if (c->nmem > 1) {
     hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
     hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+    if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
   } else {
     hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
     hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+    if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
   }
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
@@ -1207,7 +1264,27 @@ int pihm_rhs_launch_io(pihm_ctx *c, const MemberIO *mio, int *mnan) {
   const int nas = div_up((long)c->nr + c->nbank, kBlock);
   hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
   hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-  if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+  if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
+  PIHM_HIP_TRY(hipGetLastError());
+  c->ncalls++;
+  return PIHM_OK;
+}
+
+int pihm_rhs_launch_compact(pihm_ctx *c, const MemberIO *mio, int *mnan, const int *bm, const int *bm_mem,
+                            int nact, int nbe, int nbr, int nbb) {
+  if (!c || !mio || !bm || !bm_mem || nact < 1) return PIHM_ERR_ARG;
+  RhsArgs a = c->args;
+  a.y = nullptr;
+  a.ydot = nullptr;
+  a.mio = mio;
+  a.mnan = mnan;
+  a.record = 0;
+  a.bm = bm;
+  a.bm_mem = bm_mem;
+  a.nact = nact;
+  if (nbe > 0) hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+  if (nbe + nbr > 0) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  if (nbr + nbb > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nbr + nbb), dim3(kBlock), 0, c->stream, a, nbr);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
   return PIHM_OK;
@@ -1346,8 +1423,8 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
     else hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[2], c->stream));
     if (nas > 0) {
-      if (c->nmem > 1) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a);
-      else hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a);
+      if (c->nmem > 1) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
+      else hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
     }
     PIHM_HIP_TRY(hipEventRecord(ev[3], c->stream));
     PIHM_HIP_TRY(hipEventSynchronize(ev[3]));

@@ -57,7 +57,8 @@ class RunConfig(C.Structure):
                 ("etstep", C.c_double), ("reltol", C.c_double), ("abstol", C.c_double),
                 ("initstep", C.c_double), ("nncfn", C.c_double), ("nnimax", C.c_double),
                 ("nnimin", C.c_double), ("decr", C.c_double), ("incr", C.c_double),
-                ("stmin", C.c_double), ("maxl", C.c_int), ("max_steps_per_call", C.c_long)]
+                ("stmin", C.c_double), ("maxl", C.c_int), ("max_steps_per_call", C.c_long),
+                ("graphs", C.c_int)]
 
 
 class Forcing(C.Structure):
@@ -106,6 +107,8 @@ _PROTOS = [
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("pihm_ctx_set_active", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("pihm_ctx_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("pihm_stream_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("pihm_stream_destroy", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
     ("pihm_ctx_neq", C.c_int, [C.c_void_p]),
     ("pihm_ctx_get_perm", C.c_int, [C.c_void_p, c_int32_p]),
@@ -173,6 +176,7 @@ _PROTOS = [
     ("PihmCVodeSetMaxOrd", C.c_int, [C.c_void_p, C.c_int]),
     ("PihmCVodeSetStopTime", C.c_int, [C.c_void_p, C.c_double]),
     ("PihmCVodeSetStabLimDet", C.c_int, [C.c_void_p, C.c_int]),
+    ("PihmCVodeSetGraphRhs", C.c_int, [C.c_void_p, C.c_int]),
     ("PihmCVSpgmr", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("PihmCVode", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     ("PihmCVodeGetDky", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_void_p]),

@@ -56,7 +56,7 @@ class StormForcing:
 
 
 def run_config(t_end, t_start=0.0, stepsize=60.0, etstep=900.0, reltol=1e-3, abstol=1e-4,
-               initstep=5e-5, maxl=0, max_steps_per_call=0):
+               initstep=5e-5, maxl=0, max_steps_per_call=0, graphs=1):
     """Driver settings of the reference example (benchmark/input/example/example.para):
     60 s model step, 900 s LSM step, reltol 1e-3, abstol 1e-4, initial step 5e-5,
     NUM_NONCOV_FAIL 0, MAX_NONLIN_ITER 3, MIN_NONLIN_ITER 1, DECR/INCR 1.2, MIN_MAXSTEP 1."""
@@ -66,4 +66,5 @@ def run_config(t_end, t_start=0.0, stepsize=60.0, etstep=900.0, reltol=1e-3, abs
     c.nncfn, c.nnimax, c.nnimin, c.decr, c.incr, c.stmin = 0.0, 3.0, 1.0, 1.2, 1.2, 1.0
     c.maxl = maxl
     c.max_steps_per_call = max_steps_per_call
+    c.graphs = graphs
     return c
