@@ -209,6 +209,26 @@ def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
         alone_equals_mix=f"{same}/{len(pick)} bit-identical", **cnt)
 
 
+def closure_run(days):
+    """Water-balance closure of the 535-triangle / 20-segment watershed over `days` simulated
+    days on the device, with the closure accumulators (pihm_ctx_set_closure)."""
+    from pihm_amd.balance import closure
+    from pihm_amd.rhs import run_gpu
+    m = config_mesh("c0", seed=1)
+    f = StormForcing(int(days * 24) + 4, seed=11, nelem=m.nelem)
+    y0 = m.initial_state()
+    t0 = time.perf_counter()
+    rc, y1, st = run_gpu(m, f, run_config(days * 86400.0), y0, closure=True)
+    wall = time.perf_counter() - t0
+    if rc:
+        raise RuntimeError(f"closure run failed: {rc}")
+    c = closure(m, y0, y1)
+    return {"workload": f"c0: 535 triangles, 20 segments, {days:g} simulated days, storm forcing",
+            "residual_rel_to_precip": c["rel"], "precip_m3": c["precip_m3"],
+            "storage_change_m3": c["storage_change_m3"], "residual_m3": c["residual_m3"],
+            "sim_days_per_s": days / wall, "nst": st.nst}
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -265,6 +285,8 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-hours", type=float, default=1.0)
     ap.add_argument("--ensemble-sample", type=int, default=16)
+    ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--closure-days", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -333,6 +355,8 @@ def main():
             gs = g["marks"][S] - g["marks"][0.0]
             out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
+    if not args.no_closure:
+        out["closure"] = closure_run(args.closure_days)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_run(args.ensemble_members, args.ensemble_hours, args.ensemble_sample, 2024)
     if not args.no_serve:

@@ -156,6 +156,16 @@ int  pihm_ctx_poll_nan(pihm_ctx *ctx);
 /* Recording mode keeps every flux of the last call for pihm_ctx_get_fluxes
  * (extra HBM writes; off by default). */
 int  pihm_ctx_set_record(pihm_ctx *ctx, int on);
+/* Water-balance closure (single-solve contexts, set before creating
+ * vectors): the state gains 4 entries after the reference's 3*nelem+2*nriver,
+ * the cumulative volumes [m3] of precipitation in, evapotranspiration out,
+ * outflow across the mesh boundary edges, and outflow through river
+ * boundaries and outlets.  The RHS writes their rates, so the integrator
+ * integrates them with the same steps as the state and
+ *   sum_i area_i (surf_i + porosity_i (unsat_i + gw_i))
+ *   + sum_r area_r (stage_r + porosity_r rivgw_r) - Q_in + Q_et + Q_bnd + Q_riv
+ * is conserved up to rounding.  pihm_ctx_neq includes the 4 entries. */
+int  pihm_ctx_set_closure(pihm_ctx *ctx, int on);
 
 /* Wait for all work queued on the context's stream. */
 int  pihm_ctx_sync(pihm_ctx *ctx);

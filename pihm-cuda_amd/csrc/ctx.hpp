@@ -113,6 +113,12 @@ struct RhsArgs {
   const int32_t *emem, *rmem;  // member of each device element / river
   const MemberIO *mio;         // [nmem] per-call member table, or nullptr
   int *mnan;                   // [nmem] per-member NaN flags, or nullptr
+  // Water-balance accumulators (single solves): per-element precipitation,
+  // evapotranspiration and boundary-edge outflow volume rates cl_e[3][ne],
+  // per-river external outflow cl_r[nr]; summed into the 4 state entries
+  // that follow the reference's state when closure is on.
+  int closure;
+  double *cl_e, *cl_r, *cl_part;
   // Compact launch over the active members only (nullptr: every position of
   // the batch): bm[0..nact] element-block, bm[nact+1..2nact+1] river-block and
   // bm[2nact+2..3nact+2] bank-block prefixes, bm_mem[i] the i-th active member.
@@ -168,6 +174,7 @@ struct pihm_ctx {
   hipStream_t stream;
   int nan_check;
   int record;
+  int closure;                  // 4 water-balance accumulators after the state
   long long ncalls;
   std::vector<int32_t> perm;     // device position -> caller index within its member
   std::vector<int32_t> iperm;    // (member eoff + caller index) -> device position

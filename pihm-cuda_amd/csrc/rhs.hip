@@ -182,9 +182,11 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   // LateralFlow: src/lat_flow.c:29-63
   double ovl[3], sub[3];
   bool bank = false;
+  bool bnd[3] = {false, false, false};
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const int c = e.nb[j * ne + p];
+    bnd[j] = (c == NB_BOUND);
     const double dist = e.nabrdist[j * ne + p];
     const double edge = e.edge[j * ne + p];
     if (c >= 0) {
@@ -329,6 +331,16 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     for (int j = 0; j < 3; j++) { b->ovl[j] = ovl[j]; b->sub[j] = sub[j]; }
   }
   if (nan) flag_nan<B>(a, ix);
+  if (!B && a.closure) {
+    // volume rates crossing the element's outer boundary
+    double out = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (bnd[j]) out += ovl[j] + sub[j];
+    a.cl_e[p] = e.pcpdrp[p] * area;
+    a.cl_e[ne + p] = ((((edir_s + edir_u) + edir_g) + ett_u) + ett_g) * area;
+    a.cl_e[2L * ne + p] = out;
+  }
 }
 
 template <bool B>
@@ -484,6 +496,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
     rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
   }
 
+  if (!B && a.closure) a.cl_r[r] = upinit + ((dcode < 0) ? rf1 : 0.0);   // leaves through river boundaries
   double *rf = R.rf;
   rf[1 * nr + r] = rf1;
   rf[2 * nr + r] = rf2;
@@ -512,6 +525,51 @@ __global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
     const int rb = blockIdx.x - nbe;
     const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
     if (r >= 0 && r < a.nr) river_flux<B>(a, r);
+  }
+}
+
+// Water-balance accumulators: the four volume rates summed over the solve in
+// a fixed order (strided per thread, wavefront butterfly, waves in order,
+// then workgroups in order), written into the derivative entries that follow
+// the reference's state.
+constexpr int kClosureBlocks = 256;
+
+__global__ __launch_bounds__(kBlock) void k_closure_part(RhsArgs a) {
+  __shared__ double lds[4 * 4];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int stride = gridDim.x * kBlock;
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < a.ne; p += stride) {
+    acc[0] += a.cl_e[p];
+    acc[1] += a.cl_e[a.ne + p];
+    acc[2] += a.cl_e[2L * a.ne + p];
+  }
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < a.nr; r += stride) acc[3] += a.cl_r[r];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) lds[wave * 4 + k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    a.cl_part[k * kClosureBlocks + blockIdx.x] = ((lds[k] + lds[4 + k]) + lds[8 + k]) + lds[12 + k];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_closure_final(RhsArgs a, int nb, double *out) {
+  __shared__ double lds[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 0; k < 4; k++) {
+    double v = (threadIdx.x < nb) ? a.cl_part[k * kClosureBlocks + threadIdx.x] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    __syncthreads();
   }
 }
 
@@ -764,6 +822,9 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   ar.add(&R.rf, 11L * nr1);
   ar.add(&R.hid_ovl, 4L * nr1);
   ar.add(&R.bank_sub, 2L * nr1);
+  ar.add(&A.cl_e, 3L * ne);
+  ar.add(&A.cl_r, nr1);
+  ar.add(&A.cl_part, 4L * kClosureBlocks);
   ar.add(&A.rec.ovl, 3L * ne);
   ar.add(&A.rec.sub, 3L * ne);
   ar.add(&A.rec.infil, ne);
@@ -820,6 +881,8 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   A.bm = nullptr;
   A.bm_mem = nullptr;
   A.nact = 0;
+  A.closure = 0;
+  c->closure = 0;
   c->stage_host = nullptr;
   *out = c;
   return PIHM_OK;
@@ -1140,6 +1203,20 @@ extern "C" int pihm_ctx_set_nan_check(pihm_ctx *c, int on) {
   return PIHM_OK;
 }
 
+extern "C" int pihm_ctx_set_closure(pihm_ctx *c, int on) {
+  if (!c || c->nmem != 1 || c->members[0].ne != c->caps[0].ne || c->members[0].nr != c->caps[0].nr)
+    return PIHM_ERR_ARG;
+  on = on ? 1 : 0;
+  if (on == c->closure) return PIHM_OK;
+  const int d = on ? 4 : -4;
+  c->closure = on;
+  c->args.closure = on;
+  c->neq += d;
+  c->members[0].neq += d;
+  update_caller_offsets(c);
+  return PIHM_OK;
+}
+
 extern "C" int pihm_ctx_set_record(pihm_ctx *c, int on) {
   if (!c) return PIHM_ERR_ARG;
   c->record = on ? 1 : 0;
@@ -1245,6 +1322,11 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
     hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
     hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
+    if (a.closure) {
+      const int nb = std::max(1, std::min(kClosureBlocks, div_up(std::max(c->ne, c->nr), kBlock)));
+      hipLaunchKernelGGL(k_closure_part, dim3(nb), dim3(kBlock), 0, c->stream, a);
+      hipLaunchKernelGGL(k_closure_final, dim3(1), dim3(kBlock), 0, c->stream, a, nb, ydot + (c->neq - 4));
+    }
   }
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;

@@ -119,6 +119,7 @@ _PROTOS = [
     ("pihm_rhs_launch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pihm_ctx_poll_nan", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_record", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_ctx_set_closure", C.c_int, [C.c_void_p, C.c_int]),
     ("pihm_ctx_sync", C.c_int, [C.c_void_p]),
     ("pihm_ctx_time_kernels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
                                         c_double_p, c_double_p]),

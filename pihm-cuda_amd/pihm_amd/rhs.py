@@ -18,7 +18,7 @@ def _dp(a):
 
 
 class HipRHS:
-    def __init__(self, mesh, nan_check=True, record=False):
+    def __init__(self, mesh, nan_check=True, record=False, closure=False):
         self.L = _lib.lib()
         self.mesh = mesh
         self.desc = mesh.desc()
@@ -27,6 +27,8 @@ class HipRHS:
         if rc != 0:
             raise RuntimeError(f"pihm_ctx_create failed ({rc})")
         self.ctx = ctx
+        if closure and self.L.pihm_ctx_set_closure(ctx, 1):
+            raise RuntimeError("pihm_ctx_set_closure failed")
         self.neq = self.L.pihm_ctx_neq(ctx)
         self.y = self.L.N_VNew_Hip(self.neq)
         self.yd = self.L.N_VNew_Hip(self.neq)
@@ -71,6 +73,8 @@ class HipRHS:
     def ode(self, y, t=0.0):
         """ydot = f(t, y) (caller order).  Returns (rc, ydot); rc 1 = NaN in ydot."""
         y = np.ascontiguousarray(y, dtype=np.float64)
+        if len(y) < self.neq:
+            y = np.concatenate([y, np.zeros(self.neq - len(y))])
         if self.L.pihm_ctx_state_to_device(self.ctx, _dp(y), self.y):
             raise RuntimeError("state_to_device failed")
         rc = self.L.PIHM_ODE_Hip(t, self.y, self.yd, self.ctx)
@@ -99,8 +103,10 @@ class HipRHS:
         return dict(ovlflow=ovl, subsurf=sub, infil=infil, rechg=rechg, rivflow=rf[:11 * nr])
 
 
-def run_gpu(mesh, forcing, cfg, y0):
-    """Integrate on the device with the model-step driver (pihm_run_gpu); returns (rc, y, stats)."""
+def run_gpu(mesh, forcing, cfg, y0, closure=False):
+    """Integrate on the device with the model-step driver (pihm_run_gpu); returns (rc, y, stats).
+    With closure=True the state carries the 4 water-balance accumulators after the
+    reference's entries (y0 may include them or not; they start at 0 if not)."""
     L = _lib.lib()
     desc = mesh.desc()
     ctx = C.c_void_p()
@@ -109,6 +115,11 @@ def run_gpu(mesh, forcing, cfg, y0):
         raise RuntimeError(f"pihm_ctx_create failed ({rc})")
     try:
         y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+        if closure:
+            if L.pihm_ctx_set_closure(ctx, 1):
+                raise RuntimeError("pihm_ctx_set_closure failed")
+            if len(y) == mesh.neq:
+                y = np.concatenate([y, np.zeros(4)])
         st = _lib.RunStats()
         fs = forcing.struct()
         rc = L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), _dp(y), C.byref(st))

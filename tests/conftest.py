@@ -51,3 +51,14 @@ def assert_rhs_close(got, ref, mag, rtol=1e-12, what="ydot"):
         i = np.nonzero(bad)[0]
         raise AssertionError(f"{what}: {len(i)} components outside {rtol} (first {i[:5]}: "
                              f"got {got[i[:5]]} ref {ref[i[:5]]} scale {scale[i[:5]]})")
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "example")
+
+
+@pytest.fixture(scope="session")
+def example():
+    """The reference's own 535-element / 20-segment example watershed
+    (benchmark/input/example), read from the committed copy of its input files."""
+    from pihm_amd.inputs import read_project
+    return mesh_cached("example", lambda: read_project(GOLDEN, "example", os.path.join(GOLDEN, "vegprmt.tbl")))

@@ -49,7 +49,7 @@ def test_state_permutation_round_trip(c0):
     h.close()
 
 
-@pytest.mark.parametrize("which", ["c0", "tree_small"])
+@pytest.mark.parametrize("which", ["c0", "tree_small", "example"])
 def test_rhs_parity_states(which, request):
     m = request.getfixturevalue(which)
     print(f"[parity] {which} ne={m.nelem} nr={m.nriver}", flush=True)

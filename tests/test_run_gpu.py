@@ -72,3 +72,22 @@ def test_graph_replay_matches_eager(tree_small):
     assert rg == 0 and re == 0
     assert np.array_equal(yg, ye)
     assert sg.as_dict() == {**se.as_dict(), "wall_s": sg.wall_s}
+
+
+def test_gpu_run_matches_cpu_run_on_reference_example(example):
+    """The reference's own example watershed, with its .para solver settings, under storm
+    forcing: device run within the solver tolerance of the CPU run."""
+    m = example
+    p = m.para
+    f = StormForcing(10, seed=9, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    cfg = run_config(6 * 3600.0, stepsize=p["stepsize"], etstep=p["lsm_step"], reltol=p["reltol"],
+                     abstol=p["abstol"], initstep=p["initstep"])
+    print("[run] reference example 6 h gpu", flush=True)
+    rc_g, y_g, st_g = run_gpu(m, f, cfg, y0)
+    assert rc_g == 0, st_g.as_dict()
+    rc_c, y_c, st_c = O.OracleRun(m, f, nthreads=1).run(y0, cfg)
+    assert rc_c == 0
+    print("gpu", st_g.as_dict(), "\ncpu", st_c.as_dict(), flush=True)
+    assert _wrms(y_g, y_c) < 1.0
+    assert abs(st_g.nst - st_c.nst) <= 0.1 * st_c.nst
