@@ -46,7 +46,6 @@ struct Mem {
   bool graph_rhs = false, graph_off = false;
   hipGraphExec_t gx[MAXL_CAP] = {};
   double *kry = nullptr;        // device: copies of the Krylov sums and gamma
-  double *gamma_h = nullptr;    // pinned staging of gamma
 
   N_Vector zn[6] = {};
   N_Vector ewt = nullptr, y = nullptr, acor = nullptr, tempv = nullptr, ftemp = nullptr;
@@ -93,9 +92,7 @@ void free_graphs(Mem *m) {
 void free_vectors(Mem *m) {
   free_graphs(m);
   if (m->kry) (void)hipFree(m->kry);
-  if (m->gamma_h) (void)hipHostFree(m->gamma_h);
   m->kry = nullptr;
-  m->gamma_h = nullptr;
   for (auto &v : m->zn) { VDestroy(v); v = nullptr; }
   N_Vector *vs[] = {&m->ewt, &m->y, &m->acor, &m->tempv, &m->ftemp, &m->xcor, &m->vtemp, &m->ytemp, &m->xs};
   for (auto p : vs) { VDestroy(*p); *p = nullptr; }
@@ -116,7 +113,6 @@ bool alloc_spgmr(Mem *m) {
   if (m->fused && !m->kry) {
     if (hipMalloc((void **)&m->kry, PIHM_KRY_SIZE * sizeof(double)) != hipSuccess) return false;
     if (hipMemset(m->kry, 0, PIHM_KRY_SIZE * sizeof(double)) != hipSuccess) return false;
-    if (hipHostMalloc((void **)&m->gamma_h, sizeof(double), hipHostMallocDefault) != hipSuccess) return false;
   }
   return true;
 }
@@ -382,8 +378,7 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
   const double siginv = 1.0 / sig;
   bool converged = false;
   int krydim = 0;
-  bool graph = m->graph_rhs && !m->graph_off && m->kry;
-  if (graph && pihm_gm_set_gamma(m->kry, m->gamma_h, m->gamma, s)) return SP_ATIMES_FAIL_UNREC;
+  bool graph = m->graph_rhs && !m->graph_off && m->kry;   // gamma was left in kry by pihm_gm_start
   for (int l = 0; l < m->maxl; l++) {
     krydim = l + 1;
     double sums[MAXL_CAP + 2];

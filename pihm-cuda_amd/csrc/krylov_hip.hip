@@ -181,13 +181,9 @@ extern "C" {
 int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f, N_Vector s,
                   N_Vector b, N_Vector v0, double *sumsq, double *kry) {
   FStart k{rl1, gamma, D(zn1), D(acor), D(f), D(s), D(b), D(v0)};
-  return fused_sum<1>(L(s), k, pihm_nv_redwork(s), S(s), sumsq, kry ? kry + PIHM_KRY_START : nullptr);
-}
-
-int pihm_gm_set_gamma(double *kry, double *gamma_pinned, double gamma, N_Vector s) {
-  *gamma_pinned = gamma;
-  return hipMemcpyAsync(kry + PIHM_KRY_GAMMA, gamma_pinned, sizeof(double), hipMemcpyHostToDevice, S(s)) ==
-                 hipSuccess ? 0 : -2;
+  // the second pass also leaves gamma in kry for the replayed Krylov iterations
+  return fused_sum<1>(L(s), k, pihm_nv_redwork(s), S(s), sumsq, kry ? kry + PIHM_KRY_START : nullptr,
+                      kry ? kry + PIHM_KRY_GAMMA : nullptr, gamma);
 }
 
 int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig, N_Vector work) {

@@ -43,8 +43,8 @@ int pihm_red_wait(pihm_redwork w, void *stream);
 // (kry + l*PIHM_KRY_LD).  prep_dev forms V_l and the work vector with the
 // coefficients of row l-1 (or the start sum for l = 0) read from kry, as the
 // host computes them, and arnoldi with sums == NULL only launches, reading
-// gamma from kry[PIHM_KRY_GAMMA]: a Krylov iteration can then be captured in
-// a graph and replayed with no host arguments.
+// gamma from kry[PIHM_KRY_GAMMA] (stored there by start): a Krylov iteration
+// can then be captured in a graph and replayed with no host arguments.
 #define PIHM_KRY_LD 12
 #define PIHM_KRY_START (11 * PIHM_KRY_LD)
 #define PIHM_KRY_GAMMA (PIHM_KRY_START + 1)
@@ -55,7 +55,6 @@ int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N
                  double sig, N_Vector work);
 int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig,
                      N_Vector work);
-int pihm_gm_set_gamma(double *kry, double *gamma_pinned, double gamma, N_Vector s);
 int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
                     double gamma, double *sums, double *kry);
 int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums, double *kry);

@@ -2,7 +2,8 @@
 //
 // Each workgroup accumulates NV partial sums over a fixed strided set of
 // entries (wavefront butterfly, then the four waves in order) and writes
-// them; a one-workgroup second launch adds the partials in workgroup order
+// them; a one-workgroup second launch adds each slot's partials (one
+// wavefront per slot, lane-strided in workgroup order, then the butterfly)
 // and writes the NV results to pinned host memory.  The summation tree
 // depends only on the length, so every result is reproducible bit for bit.
 // Two launches instead of a last-workgroup ticket: on a multi-XCD device the
@@ -69,29 +70,30 @@ __device__ __forceinline__ double comb(unsigned c, double a, double b) {
 // launch, so the partials of the first pass are visible without any
 // device-wide fence in the first pass.
 namespace {
+// One wavefront per slot (wave w takes slots w, w+4, ...): lane l adds partials
+// l, l+64, ... in order, then the wavefront butterfly.  Optionally stores one
+// extra scalar (xstore) for kernels that follow on the stream.
 __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, unsigned combs,
                                                                  const double *partials, double *result,
                                                                  double *devcopy, unsigned *ctr,
-                                                                 volatile unsigned *flag) {
-  __shared__ double lds[4];
+                                                                 volatile unsigned *flag, double *xstore,
+                                                                 double xval) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < nv; k++) {
+  for (int k = wave; k < nv; k += kRedThreads / 64) {
     const unsigned c = (combs >> (2 * k)) & 3u;
+    const double *pk = partials + (long)k * PIHM_RED_MAXBLOCKS;
     double v = comb_ident(c);
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v = comb(c, v, partials[(long)k * PIHM_RED_MAXBLOCKS + b]);
+    for (int b = lane; b < nb; b += 64) v = comb(c, v, pk[b]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = comb(c, v, __shfl_xor(v, off, 64));
-    if (lane == 0) lds[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double r = comb_ident(c);
-      for (int w2 = 0; w2 < (int)(blockDim.x >> 6); w2++) r = comb(c, r, lds[w2]);
-      result[k] = r;
-      if (devcopy) devcopy[k] = r;
+    if (lane == 0) {
+      result[k] = v;
+      if (devcopy) devcopy[k] = v;
     }
-    __syncthreads();
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    if (xstore) *xstore = xval;
     // results reach host memory before the count that announces them
     __threadfence_system();
     const unsigned s = *ctr + 1u;
@@ -103,23 +105,27 @@ __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, 
 }  // namespace
 
 // Launch the second pass of a reduction whose first pass wrote nb partials.
-inline void launch_final(int nb, int nv, unsigned combs, pihm_redwork w, hipStream_t st, double *devcopy) {
+inline void launch_final(int nb, int nv, unsigned combs, pihm_redwork w, hipStream_t st, double *devcopy,
+                         double *xstore = nullptr, double xval = 0.0) {
   hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedThreads), 0, st, nb, nv, combs, w.partials, w.result,
-                     devcopy, w.ctr, w.flag);
+                     devcopy, w.ctr, w.flag, xstore, xval);
 }
 
-// Launch both passes without waiting (devcopy: optional device copy of the results).
+// Launch both passes without waiting (devcopy: optional device copy of the
+// results; xstore: optional scalar store made by the second pass).
 template <int NV, class F>
-void fused_sum_launch(long n, const F &f, pihm_redwork w, hipStream_t st, double *devcopy) {
+void fused_sum_launch(long n, const F &f, pihm_redwork w, hipStream_t st, double *devcopy,
+                      double *xstore = nullptr, double xval = 0.0) {
   const int nb = fused_blocks(n);
   hipLaunchKernelGGL((k_fused_sum<NV, F>), dim3(nb), dim3(kRedThreads), 0, st, n, f, w.partials);
-  launch_final(nb, NV, 0u, w, st, devcopy);
+  launch_final(nb, NV, 0u, w, st, devcopy, xstore, xval);
 }
 
 // Launch both passes and wait; results in out[0..NV).
 template <int NV, class F>
-int fused_sum(long n, const F &f, pihm_redwork w, hipStream_t st, double *out, double *devcopy = nullptr) {
-  fused_sum_launch<NV>(n, f, w, st, devcopy);
+int fused_sum(long n, const F &f, pihm_redwork w, hipStream_t st, double *out, double *devcopy = nullptr,
+              double *xstore = nullptr, double xval = 0.0) {
+  fused_sum_launch<NV>(n, f, w, st, devcopy, xstore, xval);
   if (hipGetLastError() != hipSuccess) return -2;
   if (pihm_red_wait(w, st)) return -2;
   for (int k = 0; k < NV; k++) out[k] = w.result[k];

@@ -21,8 +21,9 @@ from .meshgen import make_watershed
 
 def tree_segments(n_tri):
     """Segments of a binary-branching network for n_tri triangles: 2^k - 1 closest (in
-    log) to n_tri / 48.8, at least 7."""
-    k = max(3, int(round(math.log2(max(n_tri / 48.8, 1.0) + 1.0))))
+    log) to n_tri / 48.8, at least 31 (the smallest tree the generator lays out on a
+    1,000-triangle mesh: main stem plus two primaries)."""
+    k = max(5, int(round(math.log2(max(n_tri / 48.8, 1.0) + 1.0))))
     return (1 << k) - 1
 
 
@@ -38,8 +39,16 @@ def member_spec(nmem=256, seed=2024, nmin=2000, nmax=50000):
 
 
 def _build(spec):
-    return make_watershed(spec["n_tri"], seed=spec["mesh_seed"], n_seg=spec["n_seg"], network="tree",
-                          param_mult=spec["param_mult"])
+    # the network generator can fail to settle a tree of exactly n_seg segments on a given
+    # triangulation; the next seeds are tried in order (deterministic)
+    err = None
+    for k in range(8):
+        try:
+            return make_watershed(spec["n_tri"], seed=spec["mesh_seed"] + k, n_seg=spec["n_seg"], network="tree",
+                                  param_mult=spec["param_mult"])
+        except RuntimeError as e:
+            err = e
+    raise err
 
 
 def build_meshes(specs, workers=None):
