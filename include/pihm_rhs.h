@@ -190,6 +190,15 @@ int  pihm_ctx_set_river_edge_ovlflow(pihm_ctx *ctx, const double *ovl3);
 int  pihm_ctx_get_fluxes(pihm_ctx *ctx, double *ovlflow, double *subsurf,
                          double *infil, double *rechg, double *rivflow);
 
+/* The power routine of the flux functions (x^y as exp(y ln x), within a few
+ * ulp of the host libm pow; exact for 1^y and (+0)^y), evaluated on the host
+ * (on_device = 0) or by a device kernel (on_device = 1) over n host arrays.
+ * Diagnostic: no counterpart in the reference, which calls libm pow
+ * (src/soil.c:8-13, src/vert_flow.c:272-278, src/lat_flow.c:279-283,
+ * src/river_flow.c:520-583); it lets tests bound the device results against
+ * the host libm and check that host and device return the same bits. */
+int  pihm_math_pow(int n, const double *x, const double *y, double *out, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

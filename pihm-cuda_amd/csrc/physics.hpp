@@ -2,11 +2,14 @@
 //
 // Each function follows the reference serial C code in operation order (file
 // and lines at each definition) so that, compiled with -ffp-contract=off,
-// every add/mul/div/sqrt rounds exactly as the reference does on the host;
-// only the device pow/log may differ from glibc in the last bit.
+// every add/mul/div/sqrt rounds exactly as the reference does on the host.
+// Powers use xpow (exp(y ln x), within a few ulp of the host pow) and the
+// channel-geometry logarithm the device log.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include "xmath.hpp"
 
 namespace pihm {
 
@@ -43,14 +46,14 @@ PIHM_DEV double surf_h(double surfeqv) {
 // expression; it is evaluated once and squared in the written order
 // (a * b * b with b == b), which rounds identically.
 PIHM_DEV double kr_func(double beta, double satn) {
-  double f = 1.0 - pow(1.0 - pow(satn, beta / (beta - 1.0)), (beta - 1.0) / beta);
+  double f = 1.0 - xpow(1.0 - xpow(satn, beta / (beta - 1.0)), (beta - 1.0) / beta);
   return sqrt(satn) * f * f;
 }
 
 // Psi: src/vert_flow.c:272-278
 PIHM_DEV double psi_vg(double satn, double alpha, double beta) {
   satn = (satn < SATMIN) ? SATMIN : satn;
-  return -pow(pow(1.0 / satn, beta / (beta - 1.0)) - 1.0, 1.0 / beta) / alpha;
+  return -xpow(xpow(1.0 / satn, beta / (beta - 1.0)) - 1.0, 1.0 / beta) / alpha;
 }
 
 // EffKh: src/lat_flow.c:248-277
@@ -118,7 +121,7 @@ PIHM_DEV double dh_by_dl(double a0, double a1, double a2, double b0, double b1,
 // OverLandFlow: src/lat_flow.c:279-283 (literal exponent 0.6666667)
 PIHM_DEV double overland_flow(double avg_h, double grad_h, double avg_sf,
                               double crossa, double avg_rough) {
-  return crossa * pow(avg_h, 0.6666667) * grad_h / (sqrt(avg_sf) * avg_rough);
+  return crossa * xpow(avg_h, 0.6666667) * grad_h / (sqrt(avg_sf) * avg_rough);
 }
 
 // AvgHsurf: src/lat_flow.c:187-215
@@ -143,7 +146,7 @@ PIHM_DEV double riv_cs_area(int order, double depth, double coeff) {
     case RECTANGLE: return depth * coeff;
     case TRIANGLE: return depth * depth / coeff;
     case QUADRATIC: return 4.0 * depth * sqrt(depth) / (3.0 * sqrt(coeff));
-    default: return 3.0 * pow(depth, 4.0 / 3.0) / (2.0 * pow(coeff, 1.0 / 3.0));
+    default: return 3.0 * xpow(depth, 4.0 / 3.0) / (2.0 * xpow(coeff, 1.0 / 3.0));
   }
 }
 
@@ -158,10 +161,10 @@ PIHM_DEV double riv_perim(int order, double depth, double coeff) {
              log(2.0 * sqrt(coeff * depth) + sqrt(1.0 + 4.0 * coeff * depth)) /
                  (2.0 * coeff);
     default:
-      return 2.0 * ((pow(depth * (1.0 + 9.0 * pow(coeff, 2.0 / 3.0) * depth), 0.5) / 3.0) +
-                    (log(3.0 * pow(coeff, 1.0 / 3.0) * sqrt(depth) +
-                         pow(1.0 + 9.0 * pow(coeff, 2.0 / 3.0) * depth, 0.5)) /
-                     (9.0 * pow(coeff, 1.0 / 3.0))));
+      return 2.0 * ((xpow(depth * (1.0 + 9.0 * xpow(coeff, 2.0 / 3.0) * depth), 0.5) / 3.0) +
+                    (log(3.0 * xpow(coeff, 1.0 / 3.0) * sqrt(depth) +
+                         xpow(1.0 + 9.0 * xpow(coeff, 2.0 / 3.0) * depth, 0.5)) /
+                     (9.0 * xpow(coeff, 1.0 / 3.0))));
   }
 }
 

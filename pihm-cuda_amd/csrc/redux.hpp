@@ -2,9 +2,9 @@
 //
 // Each workgroup accumulates NV partial sums over a fixed strided set of
 // entries (wavefront butterfly, then the four waves in order) and writes
-// them; a one-workgroup second launch adds each slot's partials (one
-// wavefront per slot, lane-strided in workgroup order, then the butterfly)
-// and writes the NV results to pinned host memory.  The summation tree
+// them; a one-workgroup second launch adds each slot's partials (per thread
+// in workgroup order, wavefront butterfly, waves in order) and writes the
+// NV results to pinned host memory.  The summation tree
 // depends only on the length, so every result is reproducible bit for bit.
 // Two launches instead of a last-workgroup ticket: on a multi-XCD device the
 // device-scope fence such a ticket needs writes back the whole L2 of the
@@ -70,26 +70,54 @@ __device__ __forceinline__ double comb(unsigned c, double a, double b) {
 // launch, so the partials of the first pass are visible without any
 // device-wide fence in the first pass.
 namespace {
-// One wavefront per slot (wave w takes slots w, w+4, ...): lane l adds partials
-// l, l+64, ... in order, then the wavefront butterfly.  Optionally stores one
-// extra scalar (xstore) for kernels that follow on the stream.
+// Every thread loads its partials of all nv slots (workgroups t, t+256,
+// t+512, t+768) before combining any, so the second pass waits for one
+// round of memory latency; each slot is then combined in the fixed order
+// thread (ascending workgroup), wavefront butterfly, waves in order.
+// Optionally stores one extra scalar (xstore) for kernels that follow on the
+// stream.
 __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, unsigned combs,
                                                                  const double *partials, double *result,
                                                                  double *devcopy, unsigned *ctr,
                                                                  volatile unsigned *flag, double *xstore,
                                                                  double xval) {
+  constexpr int kPer = PIHM_RED_MAXBLOCKS / kRedThreads;
+  static_assert(PIHM_RED_MAXBLOCKS % kRedThreads == 0, "whole rounds of partials per thread");
+  __shared__ double lds[(kRedThreads / 64) * PIHM_RED_MAXSUMS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = wave; k < nv; k += kRedThreads / 64) {
-    const unsigned c = (combs >> (2 * k)) & 3u;
-    const double *pk = partials + (long)k * PIHM_RED_MAXBLOCKS;
-    double v = comb_ident(c);
-    for (int b = lane; b < nb; b += 64) v = comb(c, v, pk[b]);
+  double v[PIHM_RED_MAXSUMS][kPer];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = comb(c, v, __shfl_xor(v, off, 64));
-    if (lane == 0) {
-      result[k] = v;
-      if (devcopy) devcopy[k] = v;
+  for (int k = 0; k < PIHM_RED_MAXSUMS; k++) {
+    if (k < nv) {
+      const unsigned c = (combs >> (2 * k)) & 3u;
+#pragma unroll
+      for (int j = 0; j < kPer; j++) {
+        const int b = (int)threadIdx.x + j * kRedThreads;
+        v[k][j] = (b < nb) ? partials[(long)k * PIHM_RED_MAXBLOCKS + b] : comb_ident(c);
+      }
     }
+  }
+#pragma unroll
+  for (int k = 0; k < PIHM_RED_MAXSUMS; k++) {
+    if (k < nv) {
+      const unsigned c = (combs >> (2 * k)) & 3u;
+      double x = v[k][0];
+#pragma unroll
+      for (int j = 1; j < kPer; j++) x = comb(c, x, v[k][j]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x = comb(c, x, __shfl_xor(x, off, 64));
+      if (lane == 0) lds[wave * PIHM_RED_MAXSUMS + k] = x;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    const int k = threadIdx.x;
+    const unsigned c = (combs >> (2 * k)) & 3u;
+    double r = lds[k];
+#pragma unroll
+    for (int w2 = 1; w2 < kRedThreads / 64; w2++) r = comb(c, r, lds[w2 * PIHM_RED_MAXSUMS + k]);
+    result[k] = r;
+    if (devcopy) devcopy[k] = r;
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -8,11 +8,11 @@
 //   k_prep     per element: clamp the state, surface depth, EffKh, and the
 //              FrictSlope gradient magnitude (neighbour heads recomputed
 //              from the state); writes one 64-byte neighbour record.
-//   k_flux     blocks [0, nbe): per element, lateral fluxes (one 64-byte
-//              gather per neighbour), EtExtract, Infil, Recharge and the
-//              element ydot; elements with a river edge leave a partial
-//              record.  Blocks [nbe, ..): per river segment, all segment
-//              fluxes and the two bank-edge fluxes.
+//   k_flux     river blocks first: per river segment, all segment fluxes
+//              and the two bank-edge fluxes; then the element blocks: per
+//              element, lateral fluxes (one 64-byte gather per neighbour),
+//              EtExtract, Infil, Recharge and the element ydot; elements
+//              with a river edge leave a partial record.
 //   k_assemble per river: upstream inflow as a gather over the upstream
 //              list in ascending caller index (the order of the reference's
 //              serial loop, src/river_flow.c:96-110), then river ydot;
@@ -265,7 +265,8 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     if (wet_bottom) {
       dh_by_dz = (surfh + zmax - (g + zmin)) / (0.5 * (surfh + dinf));
       dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
-      const double satkfunc = kr_func(e.beta[p], 1.0);
+      // KrFunc(beta, 1) is exactly 1 for beta > 1 (1^m = 1, 0^(1/m) = 0)
+      const double satkfunc = (e.beta[p] > 1.0) ? 1.0 : kr_func(e.beta[p], 1.0);
       kinf = eff_kinf(e.kinfv[p], e.kmacv[p], e.areafh[p], dh_by_dz, satkfunc, 1.0, applrate, surfh);
       infil = kinf * dh_by_dz;
     } else {
@@ -436,7 +437,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
         const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
         const double perim = riv_perim(ord, stage, coeff);
         const double crossa = riv_cs_area(ord, stage, coeff);
-        q = sqrt(grad_h) * crossa * ((perim > 0.0) ? pow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
+        q = sqrt(grad_h) * crossa * ((perim > 0.0) ? xpow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
         break;
       }
       default: {  // CRIT_DPTH (other codes rejected at creation)
@@ -515,14 +516,18 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   R.bank_sub[2 * r + 1] = -(rf5 + rf8);
 }
 
+// Grid: nbr river blocks first (their dependent gathers and channel geometry
+// are the longest per thread, so they start with the first wave of
+// workgroups instead of trailing the element blocks), then nbe element blocks.
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
-  if ((int)blockIdx.x < nbe) {
-    const int tile = xcd_tile(blockIdx.x, nbe);
+  const int nbr = (int)gridDim.x - nbe;
+  if ((int)blockIdx.x >= nbr) {
+    const int tile = xcd_tile((int)blockIdx.x - nbr, nbe);
     const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
     if (p >= 0 && p < a.ne) elem_flux<B>(a, p);
   } else {
-    const int rb = blockIdx.x - nbe;
+    const int rb = blockIdx.x;
     const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
     if (r >= 0 && r < a.nr) river_flux<B>(a, r);
   }
@@ -1521,4 +1526,36 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
   if (ms_flux) *ms_flux = t[1] / iters;
   if (ms_asm) *ms_asm = t[2] / iters;
   return PIHM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the flux functions' power routine, exposed for its accuracy tests
+// ---------------------------------------------------------------------------
+
+namespace {
+__global__ __launch_bounds__(kBlock) void k_math_pow(int n, const double *x, const double *y, double *out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = xpow(x[i], y[i]);
+}
+}  // namespace
+
+extern "C" int pihm_math_pow(int n, const double *x, const double *y, double *out, int on_device) {
+  if (n < 0 || (n > 0 && (!x || !y || !out))) return PIHM_ERR_ARG;
+  if (n == 0) return PIHM_OK;
+  if (!on_device) {
+    for (int i = 0; i < n; i++) out[i] = xpow(x[i], y[i]);
+    return PIHM_OK;
+  }
+  double *d = nullptr;
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (hipMalloc((void **)&d, 3 * bytes) != hipSuccess) return PIHM_ERR_HIP;
+  hipError_t e = hipMemcpy(d, x, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + n, y, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_math_pow, dim3(div_up(n, kBlock)), dim3(kBlock), 0, 0, n, d, d + n, d + 2L * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d + 2L * n, bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? PIHM_OK : PIHM_ERR_HIP;
 }

@@ -127,6 +127,7 @@ _PROTOS = [
     ("pihm_ctx_set_river_edge_ovlflow", C.c_int, [C.c_void_p, c_double_p]),
     ("pihm_ctx_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                       c_double_p]),
+    ("pihm_math_pow", C.c_int, [C.c_int, c_double_p, c_double_p, c_double_p, C.c_int]),
     ("N_VNew_Hip", C.c_void_p, [C.c_long]),
     ("N_VNewEmpty_Hip", C.c_void_p, [C.c_long]),
     ("N_VMake_Hip", C.c_void_p, [C.c_long, C.c_void_p]),
