@@ -27,9 +27,15 @@ from pihm_amd.meshgen import config_mesh  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ELEM = 344     # algorithmic bytes of one RHS per element
 BYTES_PER_SEG = 200      # ... per river segment
+# HBM bytes one RHS evaluation at the 200,000-triangle workload moves, from the rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (summed over
+# the RHS kernels, KiB per dispatch), each scaled by what the counter reads
+# for the calibration copy in the same run: FETCH_SIZE x 2.000 (262,157 KiB read back for
+# 524,288 KiB copied), WRITE_SIZE x 1.000.  (43,061.7 x 2 + 18,547.7) KiB.
+RHS_TRAFFIC_BYTES = (43061.7 * 2.0 + 18547.7) * 1024
 
 
-def gpu_run(mesh, forcing, hours_total, mark_hours):
+def gpu_run(mesh, forcing, hours_total, mark_hours, graphs=1):
     """Run the device path for hours_total simulated hours; return wall-clock marks at the
     requested hour boundaries (device synchronised at each), stats and kernel timings."""
     L = _lib.lib()
@@ -66,7 +72,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
 
     cb = _lib.StepEndFn(step_end)
     be.model_step_end = C.cast(cb, C.c_void_p)
-    cfg = run_config(hours_total * 3600.0)
+    cfg = run_config(hours_total * 3600.0, graphs=graphs)
     st = _lib.RunStats()
     L.pihm_ctx_sync(ctx)
     marks[0.0] = time.perf_counter()
@@ -75,9 +81,11 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     nan = L.pihm_ctx_poll_nan(ctx)
     ydev = L.N_VGetDeviceArrayPointer_Hip(y)
     yddev = L.N_VGetDeviceArrayPointer_Hip(yd)
-    t3 = [C.c_double(), C.c_double(), C.c_double()]
-    L.pihm_ctx_time_kernels(ctx, ydev, yddev, 5, *[C.byref(x) for x in t3])   # warm
-    L.pihm_ctx_time_kernels(ctx, ydev, yddev, 100, *[C.byref(x) for x in t3])
+    t2 = [C.c_double(), C.c_double()]
+    L.pihm_ctx_time_kernels(ctx, ydev, yddev, 5, *[C.byref(x) for x in t2])   # warm
+    L.pihm_ctx_time_kernels(ctx, ydev, yddev, 100, *[C.byref(x) for x in t2])
+    ms_eval = C.c_double()
+    L.pihm_ctx_time_rhs(ctx, ydev, yddev, 200, C.byref(ms_eval))
     yend = np.empty(mesh.neq)
     L.pihm_ctx_state_from_device(ctx, y, yend.ctypes.data_as(_lib.c_double_p))
     L.N_VDestroy_Hip(y)
@@ -85,8 +93,8 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     L.pihm_gpu_backend_destroy(gb)
     L.pihm_ctx_destroy(ctx)
     L.pihm_stream_destroy(stream)
-    return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t3],
-                yend=yend)
+    return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t2],
+                ms_eval=ms_eval.value, yend=yend)
 
 
 def ensemble_run(nmem, hours, sample, seed):
@@ -240,9 +248,10 @@ def cpu_run(mesh, forcing, hours, nthreads):
 
 
 def profile_kernels(iters, seed):
-    """Short fixed workload for hardware-counter passes: `iters` RHS evaluations at C1 on a
-    mid-storm state, then `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB
-    read and 512 MiB written per launch), the calibration kernel for the byte counters."""
+    """Short fixed workload for kernel traces and hardware-counter passes: `iters` RHS
+    evaluations at C1 on a mid-storm state (then `iters` more timed with HIP events), then
+    `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB read and 512 MiB
+    written per launch), the calibration kernel for the byte counters."""
     L = _lib.lib()
     mesh = config_mesh("c1", seed=seed)
     desc = mesh.desc()
@@ -258,6 +267,9 @@ def profile_kernels(iters, seed):
     for _ in range(iters):
         L.PIHM_ODE_Hip(0.0, y, yd, ctx)
     L.pihm_ctx_sync(ctx)
+    ms_eval = C.c_double()
+    L.pihm_ctx_time_rhs(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), iters,
+                        C.byref(ms_eval))
     n = 64 << 20
     a = L.N_VNew_Hip(n)
     b = L.N_VNew_Hip(n)
@@ -269,7 +281,7 @@ def profile_kernels(iters, seed):
         L.N_VDestroy_Hip(v)
     L.pihm_ctx_destroy(ctx)
     print(json.dumps({"profile_kernels": iters, "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
-                      "copy_bytes": 2 * n * 8}))
+                      "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8}))
 
 
 def main():
@@ -293,6 +305,8 @@ def main():
     ap.add_argument("--serve-horizon-unit", type=float, default=60.0,
                     help="seconds of simulated time per horizon unit (the request spec's unit is a day)")
     ap.add_argument("--serve-check", type=int, default=3)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch each Krylov iteration's kernels one by one instead of replaying graphs")
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only run N RHS evaluations and N copy-kernel launches (counter passes)")
     args = ap.parse_args()
@@ -308,13 +322,13 @@ def main():
     W, K = args.warmup, args.steps
     forcing = StormForcing(W + K + 4, seed=11, nelem=mesh.nelem)
     S = args.cpu_hours
-    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S])
+    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S], graphs=0 if args.no_graphs else 1)
     if g["rc"] != 0 or g["nan"] != 0:
         raise RuntimeError(f"device run failed: rc={g['rc']} nan={g['nan']} {g['stats']}")
     elapsed = g["marks"][float(W + K)] - g["marks"][float(W)]
     value = (K / 24.0) / elapsed
-    ms_prep, ms_flux, ms_asm = g["kernel_ms"]
-    rhs_ms = ms_prep + ms_flux + ms_asm
+    ms_prep, ms_flux = g["kernel_ms"]
+    rhs_ms = g["ms_eval"]
     alg_bytes = BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver
     achieved = alg_bytes / (rhs_ms * 1e-3) / 1e9
     st = g["stats"]
@@ -336,11 +350,11 @@ def main():
                                "step = 1 simulated hour; BDF+SPGMR reltol 1e-3 abstol 1e-4",
                    "nelem": mesh.nelem, "nriver": mesh.nriver, "neq": mesh.neq},
         "solver": {k: st[k] for k in ("nst", "nfe", "nfes", "nni", "nli", "ncfn", "netf")},
-        "rhs": {"ms_prep": ms_prep, "ms_flux": ms_flux, "ms_assemble": ms_asm, "ms_total": rhs_ms,
-                "evals_per_s_kernel": 1e3 / rhs_ms},
-        "roofline": {"kernel": "one RHS evaluation (k_prep + k_flux + k_assemble)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "rhs": {"ms_per_eval": rhs_ms, "evals_per_s": 1e3 / rhs_ms,
+                "ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}},
+        "roofline": {"kernel": "one RHS evaluation (k_prep + k_flux, back to back)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": RHS_TRAFFIC_BYTES,
                      "algorithmic_bytes": alg_bytes},
     }
     if not args.no_cpu:

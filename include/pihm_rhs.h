@@ -170,11 +170,18 @@ int  pihm_ctx_set_closure(pihm_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int  pihm_ctx_sync(pihm_ctx *ctx);
 
-/* Measurement: average device time [ms] of the three RHS kernels (neighbour
- * records + slopes, fluxes, river/bank assembly) over `iters` evaluations at
- * state y_dev, timed with HIP events on the context's stream. */
+/* Measurement: average device time [ms] of each of the two RHS kernels
+ * (neighbour records + slopes; all fluxes and the derivative) over `iters`
+ * evaluations at state y_dev, each between its own pair of HIP events on the
+ * context's stream. */
 int  pihm_ctx_time_kernels(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters,
-                           double *ms_prep, double *ms_flux, double *ms_assemble);
+                           double *ms_prep, double *ms_flux);
+
+/* Measurement: average device time [ms] of one whole RHS evaluation (both
+ * kernels issued back to back, as an integrator issues them) over `iters`
+ * evaluations at state y_dev: one HIP event before the first launch and one
+ * after the last, on the context's stream. */
+int  pihm_ctx_time_rhs(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters, double *ms_per_eval);
 
 /* Carried state: the overland flux on river edges written at the end of one
  * RHS call and read by Infil in the next (src/river_flow.c:159-182 then

@@ -7,9 +7,8 @@
 //   edge codes         2 x int32[3*ne]     nb (neighbour code), bc_type
 //   per-step inputs    4 x double[ne] + double[3*ne]
 //   neighbour record   NbrRec[ne]          values a neighbour reads, one 64-byte load
-//   bank records       BankRec[nbank]      partial sums of elements with a river edge
 //   river statics      17 x double[nr] + 6 x int32[nr], upstream CSR
-//   river fluxes       double[11*nr] (rf[k*nr+r]), carried river-edge fluxes
+//   river fluxes       double[11*nr] (rf[k*nr+r]), carried and new river-edge fluxes
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -35,26 +34,15 @@ struct alignas(64) NbrRec {
   double zmax;    // topo.zmax (static)
 };
 
-// Partial sums of an element that has at least one river edge; finished once
-// the river fluxes are known, in the reference's edge order.
-struct alignas(64) BankRec {
-  double dys0;    // pcpdrp - infil - edir_surf
-  double dyg0;    // rechg - edir_gw - ett_gw
-  double ovl[3];
-  double sub[3];
-};
 
 struct ElemDev {
   double *area, *zmin, *zmax, *depth, *ksath, *ksatv, *kinfv, *dinf, *alpha,
       *beta, *porosity, *dmac, *kmach, *kmacv, *areafv, *areafh, *rough, *rzd;
   double *edge, *nabrdist, *nabr_x, *nabr_y;  // [3*ne]
   int32_t *nb, *bc_type;                      // [3*ne]
-  int32_t *bank_id;                           // [ne], -1 if no river edge
   // per-step inputs
   double *pcpdrp, *edir, *ett, *ws0s, *ebc;   // ebc [3*ne]
   NbrRec *rec;
-  BankRec *bank;
-  int32_t *bank_elem;                         // [nbank] device element of each bank record
 };
 
 struct RiverDev {
@@ -65,10 +53,11 @@ struct RiverDev {
   double *area, *zmin, *zmax, *zbed, *node_zmax, *dist_l, *dist_r, *depth, *coeff,
       *length, *width, *rough, *cwr, *ksath, *ksatv, *bedthick, *porosity;
   double *bc;           // per-step river bc value
-  double *rf;           // [11*nr], rf[k*nr + r]
+  double *rf;           // [11*nr], rf[k*nr + r] (0 and 10 in recording mode)
   double *upinit;       // 0.0 + boundary flux of the upstream node
-  double *hid_ovl;      // [2][2*nr] carried river-edge overland flux, double-buffered
-  double *bank_sub;     // [2*nr] river-edge subsurface flux of the current call
+  double *hid_ovl;      // [4*nr]: carried river-edge overland flux at slot 2r + side,
+                        // this call's at 2*nr + slot
+  double *bank_sub;     // [2*nr] this call's river-edge subsurface flux, by slot
 };
 
 struct RecordDev {      // optional flux recording (device order)
@@ -84,7 +73,6 @@ struct MemberDev {
   int roff, nr;         // river range
   double dt;            // ctrl.stepsize of this solve (Infil)
   int active;           // evaluated by the next RHS launch
-  int boff, nbank;      // bank-record range
   int pad;
 };
 
@@ -97,10 +85,9 @@ struct MemberIO {
 };
 
 struct RhsArgs {
-  int ne, nr, nbank;
+  int ne, nr;
   int surf_mode, riv_mode;
   double dt;            // ctrl.stepsize
-  int cur;              // parity of the carried river-edge buffer read this call
   int record;
   ElemDev e;
   RiverDev r;
@@ -120,8 +107,8 @@ struct RhsArgs {
   int closure;
   double *cl_e, *cl_r, *cl_part;
   // Compact launch over the active members only (nullptr: every position of
-  // the batch): bm[0..nact] element-block, bm[nact+1..2nact+1] river-block and
-  // bm[2nact+2..3nact+2] bank-block prefixes, bm_mem[i] the i-th active member.
+  // the batch): bm[0..nact] element-block and bm[nact+1..2nact+1] river-block
+  // prefixes, bm_mem[i] the i-th active member.
   const int *bm, *bm_mem;
   int nact;
 };
@@ -144,7 +131,6 @@ struct pihm_member_host {
   int ceoff, croff;
   long cyoff;
   double dt;
-  int nbank;
 };
 
 // Launch one batched RHS evaluation whose member states come from the device
@@ -152,16 +138,16 @@ struct pihm_member_host {
 // mnan[member].  Asynchronous on the context's stream.
 struct pihm_ctx;
 int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
-// The same over the nact members of a block map (see RhsArgs::bm); nbe, nbr,
-// nbb are the map's element, river and bank block totals.
+// The same over the nact members of a block map (see RhsArgs::bm); nbe and
+// nbr are the map's element and river block totals.
 int pihm_rhs_launch_compact(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan, const int *bm,
-                            const int *bm_mem, int nact, int nbe, int nbr, int nbb);
+                            const int *bm_mem, int nact, int nbe, int nbr);
 
 struct pihm_ctx {
-  int ne, nr, nbank, neq;       // totals over the batch, at slot capacity
+  int ne, nr, neq;              // totals over the batch, at slot capacity
   int nmem;
   std::vector<pihm::SlotCap> caps;
-  std::vector<int> boff, uoff;  // first bank record / upstream entry of each slot
+  std::vector<int> uoff;        // first upstream entry of each slot
   int ne_loaded, nr_loaded;     // caller-order totals of the loaded meshes
   long neq_loaded;
   std::vector<pihm_member_host> members;

@@ -748,7 +748,7 @@ struct RoundPlan {
   bool rhs;
   size_t off_part, off_cpre, off_obeg, off_ops, off_io, off_rmem, off_rinfo, off_bm, off_bmm;
   int np, nblocks, nred;
-  int nact, nbe, nbr, nbb;
+  int nact, nbe, nbr;
 };
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 }  // namespace
@@ -824,21 +824,19 @@ int pihm_ensemble::flush() {
       act.push_back(m);
     }
     if (!act.empty()) {
-      // block map of the active members: element, river and bank-record blocks
+      // block map of the active members: element and river blocks
       const int na = (int)act.size();
-      std::vector<int> bm(3 * (na + 1), 0);
+      std::vector<int> bm(2 * (na + 1), 0);
       for (int i = 0; i < na; i++) {
         const auto &M = ctx->members[act[i]];
         bm[i + 1] = bm[i] + div_up(M.ne, kBlock);
         bm[(na + 1) + i + 1] = bm[(na + 1) + i] + div_up(M.nr, kBlock);
-        bm[2 * (na + 1) + i + 1] = bm[2 * (na + 1) + i] + div_up(M.nbank, kBlock);
       }
       RoundPlan r{};
       r.rhs = true;
       r.nact = na;
       r.nbe = bm[na];
       r.nbr = bm[(na + 1) + na];
-      r.nbb = bm[2 * (na + 1) + na];
       r.off_io = reserve(nmem * sizeof(MemberIO));
       std::memcpy(buf.data() + r.off_io, io.data(), nmem * sizeof(MemberIO));
       r.off_bm = reserve(bm.size() * sizeof(int));
@@ -860,7 +858,7 @@ int pihm_ensemble::flush() {
     if (r.rhs) {
       rc = pihm_rhs_launch_compact(ctx, (const MemberIO *)(stage_d + r.off_io), mnan,
                                    (const int *)(stage_d + r.off_bm), (const int *)(stage_d + r.off_bmm),
-                                   r.nact, r.nbe, r.nbr, r.nbb);
+                                   r.nact, r.nbe, r.nbr);
       if (rc) return rc;
       rhs_launches++;
     } else {
@@ -1212,7 +1210,6 @@ int pihm_ens_set_state(pihm_ensemble *e, int k, const double *y) {
   PIHM_HIP_TRY(hipMemcpyAsync(c->args.e.ws0s + M.eoff, d.data(), M.ne * sizeof(double), hipMemcpyHostToDevice, e->st));
   if (M.nr > 0) {
     PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
-    PIHM_HIP_TRY(hipMemsetAsync(c->args.r.hid_ovl + 2L * c->nr + 2L * M.roff, 0, 2L * M.nr * sizeof(double), e->st));
   }
   PIHM_HIP_TRY(hipStreamSynchronize(e->st));
   reset_member(e->mem[k], e->cfg);

@@ -199,4 +199,35 @@ PIHM_DEV double ovl_elem_to_river(double rzmax, double ezmax, double surfh, doub
   return flux;
 }
 
+// ChanFlowElemToRiver: src/river_flow.c:429-460 (bank element -> channel).
+// Evaluated by the river segment for both banks and by each bank element for
+// its own river edge, from the same operands, so both get the same bits.
+PIHM_DEV double chan_elem_to_river(double stage, double zbed, double length, double rksath, double ec_gw,
+                                   double ec_gwz, double ezmin, double effk, double distance) {
+  const double diff_h = (stage + zbed) - ec_gwz;
+  double avg_h;
+  if (ezmin > zbed) avg_h = ec_gw;
+  else if (ec_gwz > zbed) avg_h = ec_gwz - zbed;
+  else avg_h = 0.0;
+  avg_h = avg_h_sub(diff_h, stage, avg_h);
+  const double grad_h = diff_h / distance;
+  const double avg_ksat = 0.5 * (effk + rksath);
+  return length * avg_ksat * grad_h * avg_h;
+}
+
+// SubFlowElemToRiver (_ARITH_): src/river_flow.c:462-498 (bank aquifer ->
+// river aquifer), shared in the same way.
+PIHM_DEV double sub_elem_to_river(double rgw, double rzmin, double zbed, double length, double ec_gw,
+                                  double ec_gwz, double ezmin, double effk, double effk_riv, double distance) {
+  const double diff_h = (rgw + rzmin) - ec_gwz;
+  double avg_h;
+  if (ezmin > zbed) avg_h = 0.0;
+  else if (ec_gwz > zbed) avg_h = zbed - ezmin;
+  else avg_h = ec_gw;
+  avg_h = avg_h_sub(diff_h, rgw, avg_h);
+  const double avg_ksat = 0.5 * (effk + effk_riv);
+  const double grad_h = diff_h / distance;
+  return length * avg_ksat * grad_h * avg_h;
+}
+
 }  // namespace pihm

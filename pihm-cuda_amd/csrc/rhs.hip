@@ -4,22 +4,23 @@
 // (src/hydrol.c), LateralFlow/FrictSlope (src/lat_flow.c), VerticalFlow
 // (src/vert_flow.c) and RiverFlow (src/river_flow.c).
 //
-// One RHS call is three launches on one stream:
+// One RHS call is two launches on one stream:
 //   k_prep     per element: clamp the state, surface depth, EffKh, and the
 //              FrictSlope gradient magnitude (neighbour heads recomputed
 //              from the state); writes one 64-byte neighbour record.
-//   k_flux     river blocks first: per river segment, all segment fluxes
-//              and the two bank-edge fluxes; then the element blocks: per
-//              element, lateral fluxes (one 64-byte gather per neighbour),
-//              EtExtract, Infil, Recharge and the element ydot; elements
-//              with a river edge leave a partial record.
-//   k_assemble per river: upstream inflow as a gather over the upstream
-//              list in ascending caller index (the order of the reference's
-//              serial loop, src/river_flow.c:96-110), then river ydot;
-//              per bank element: ydot with the river-edge fluxes in edge
-//              order.  No atomics: every output has one writer, so results
-//              are deterministic and, apart from device pow/log, equal to
-//              the reference's bit for bit.
+//   k_flux     river blocks first: per river segment, all segment fluxes,
+//              the upstream inflow (each upstream segment's outflow
+//              evaluated again here, subtracted in ascending caller index
+//              as the reference's serial loop does, src/river_flow.c:96-110)
+//              and the river ydot; then the element blocks: per element,
+//              lateral fluxes (one 64-byte gather per neighbour), EtExtract,
+//              Infil, Recharge, the element's own river-edge fluxes (the
+//              same expressions and operands as the river's) and the
+//              element ydot.
+// No atomics and no cross-thread hand-off inside k_flux: a flux needed by
+// two threads is evaluated by both from the same operands, so every output
+// has one writer and results are deterministic.  Apart from the power
+// routine (xmath.hpp), every operation rounds as the reference's does.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,20 +90,33 @@ __device__ __forceinline__ int find_seg(const int *pre, int n, int b) {
 }
 
 // Position handled by this thread of block `b` of a compact launch: kind 0
-// elements, 1 rivers, 2 bank records; -1 past the member's range.
+// elements, 1 rivers; -1 past the member's range.
 __device__ __forceinline__ int compact_index(const RhsArgs &a, int kind, int b) {
   const int *pre = a.bm + kind * (a.nact + 1);
   const int i = find_seg(pre, a.nact, b);
   const MemberDev M = a.mem[a.bm_mem[i]];
   const int l = (b - pre[i]) * kBlock + threadIdx.x;
   if (kind == 0) return l < M.ne ? M.eoff + l : -1;
-  if (kind == 1) return l < M.nr ? M.roff + l : -1;
-  return l < M.nbank ? M.boff + l : -1;
+  return l < M.nr ? M.roff + l : -1;
 }
 
+// Loads come first, in two rounds (the element's own entries and edge codes,
+// then what the codes point at), so a wavefront waits for memory twice
+// instead of once per neighbour; the arithmetic follows them unchanged.
 template <bool B>
-__global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void river_flux(const RhsArgs &a, int r);
+
+// Grid: nbr river blocks first (all river segment fluxes, the longest
+// threads of the launch), then nbe element blocks (the neighbour records).
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
+  const int nbr = (int)gridDim.x - nbe;
+  if ((int)blockIdx.x < nbr) {
+    const int r = (B && a.bm) ? compact_index(a, 1, blockIdx.x) : (int)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= 0 && r < a.nr) river_flux<B>(a, r);
+    return;
+  }
+  const int tile = xcd_tile((int)blockIdx.x - nbr, nbe);
   const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
   const int ne = a.ne;
   if (p < 0 || p >= ne) return;
@@ -112,45 +126,78 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a) {
   const int lp = p - ix.eoff;
   if (B && lp >= ix.ne) return;     // unused capacity of a slot
   const ElemDev &e = a.e;
+  const bool diff = a.surf_mode == DIFF_WAVE;
 
-  const double s = clamp0(y[lp]);
-  const double g = clamp0(y[2 * ix.ne + lp]);
+  // round 1
+  const double ys = y[lp], yg = y[2 * ix.ne + lp];
+  const double zmax = e.zmax[p], zmin = e.zmin[p], depth = e.depth[p], dmac = e.dmac[p];
+  const double kmach = e.kmach[p], areafv = e.areafv[p], ksath = e.ksath[p];
+  int c[3] = {0, 0, 0}, bt[3] = {0, 0, 0};
+  double nx[3] = {0.0, 0.0, 0.0}, ny[3] = {0.0, 0.0, 0.0};
+  if (diff) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      c[j] = e.nb[j * ne + p];
+      bt[j] = e.bc_type[j * ne + p];
+      nx[j] = e.nabr_x[j * ne + p];
+      ny[j] = e.nabr_y[j * ne + p];
+    }
+  }
+  // round 2: element neighbour (zmax, surface storage), boundary value,
+  // river edge (stage, depth, zbed, zmax).  Each case loads into
+  // its own registers, left unset in the other cases (each is read only in
+  // its own case): a register shared by two cases, or a default written into
+  // it, would make a load wait for another lane's load to the same register.
+  double ez[3], es[3], bv[3], rs[3], rd[3], rb[3], rz[3];
+  if (diff) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (c[j] >= 0) {
+        ez[j] = e.zmax[c[j]];
+        es[j] = y[c[j] - ix.eoff];
+      } else if (c[j] == NB_BOUND) {
+        bv[j] = e.ebc[j * ne + p];
+      } else {
+        const int r = (-c[j] - 1) >> 1;
+        rs[j] = y[3 * ix.ne + (r - ix.roff)];
+        rd[j] = a.r.depth[r];
+        rb[j] = a.r.zbed[r];
+        rz[j] = a.r.zmax[r];
+      }
+    }
+  }
+
+  const double s = clamp0(ys);
+  const double g = clamp0(yg);
   const double surfh = surf_h(s);
-  const double zmax = e.zmax[p];
   const double head = zmax + surfh;
-  const double gwz = g + e.zmin[p];
-  const double effkh = eff_kh(e.depth[p], e.dmac[p], e.kmach[p], e.areafv[p], e.ksath[p], g);
+  const double gwz = g + zmin;
+  const double effkh = eff_kh(depth, dmac, kmach, areafv, ksath, g);
 
   double sfmag = 0.0;
-  if (a.surf_mode == DIFF_WAVE) {
+  if (diff) {
     // FrictSlope: src/lat_flow.c:130-185
     double h[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      const int c = e.nb[j * ne + p];
-      if (c >= 0) {
-        h[j] = e.zmax[c] + surf_h(clamp0(y[c - ix.eoff]));
-      } else if (c == NB_BOUND) {
-        h[j] = (e.bc_type[j * ne + p] == 0) ? zmax + surfh : e.ebc[j * ne + p];
+      if (c[j] >= 0) {
+        h[j] = ez[j] + surf_h(clamp0(es[j]));
+      } else if (c[j] == NB_BOUND) {
+        h[j] = (bt[j] == 0) ? zmax + surfh : bv[j];
       } else {
-        const int r = (-c - 1) >> 1;
-        const double st = clamp0(y[3 * ix.ne + (r - ix.roff)]);
-        h[j] = (st > a.r.depth[r]) ? a.r.zbed[r] + st : a.r.zmax[r];
+        const double st = clamp0(rs[j]);
+        h[j] = (st > rd[j]) ? rb[j] + st : rz[j];
       }
     }
-    const double x0 = e.nabr_x[p], x1 = e.nabr_x[ne + p], x2 = e.nabr_x[2 * ne + p];
-    const double y0 = e.nabr_y[p], y1 = e.nabr_y[ne + p], y2 = e.nabr_y[2 * ne + p];
-    const double dhdx = dh_by_dl(y0, y1, y2, x0, x1, x2, h[0], h[1], h[2]);
-    const double dhdy = dh_by_dl(x0, x1, x2, y0, y1, y2, h[0], h[1], h[2]);
+    const double dhdx = dh_by_dl(ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], h[0], h[1], h[2]);
+    const double dhdy = dh_by_dl(nx[0], nx[1], nx[2], ny[0], ny[1], ny[2], h[0], h[1], h[2]);
     sfmag = sqrt(dhdx * dhdx + dhdy * dhdy);
   }
-  NbrRec *rp = &a.e.rec[p];
-  rp->surfh = surfh;
-  rp->head = head;
-  rp->gw = g;
-  rp->gwz = gwz;
-  rp->effkh = effkh;
-  rp->sfmag = sfmag;
+  // the six per-call fields of the record, as three 16-byte stores
+  double2 *rp = reinterpret_cast<double2 *>(&a.e.rec[p]);
+  rp[0] = make_double2(surfh, head);
+  rp[1] = make_double2(g, gwz);
+  rp[2] = make_double2(effkh, sfmag);
 }
 
 template <bool B>
@@ -163,20 +210,49 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   if (B && lp >= ix.ne) return;     // unused capacity of a slot
   const double *y = ix.y;
   double *ydot = ix.ydot;
+
+  // Loads first, in two rounds as in k_prep: the element's own entries and
+  // edge codes, then the neighbours' records, boundary values and carried
+  // river-edge fluxes the codes point at.
   const NbrRec me = e.rec[p];
-  const double u = clamp0(y[ix.ne + lp]);
+  const double yu = y[ix.ne + lp];
+  const double area = e.area[p], zmax = e.zmax[p], zmin = e.zmin[p];
+  const double depth = e.depth[p], dinf = e.dinf[p], rzd = e.rzd[p];
+  const double edir = e.edir[p], ett = e.ett[p], pcpdrp = e.pcpdrp[p], ws0 = e.ws0s[p];
+  const double alpha = e.alpha[p], beta = e.beta[p], kinfv = e.kinfv[p], kmacv = e.kmacv[p];
+  const double areafh = e.areafh[p], dmac = e.dmac[p], ksatv = e.ksatv[p], porosity = e.porosity[p];
+  int c[3], bt[3];
+  double dist[3], edge[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    c[j] = e.nb[j * ne + p];
+    bt[j] = e.bc_type[j * ne + p];
+    dist[j] = e.nabrdist[j * ne + p];
+    edge[j] = e.edge[j * ne + p];
+  }
+  NbrRec nrec[3];           // each read only in its own case, as in k_prep
+  double bcv[3], hov[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    if (c[j] >= 0) {
+      nrec[j] = e.rec[c[j]];
+    } else if (c[j] == NB_BOUND) {
+      bcv[j] = e.ebc[j * ne + p];
+    } else {
+      hov[j] = a.r.hid_ovl[-c[j] - 1];     // the carried overland flux
+    }
+  }
+
+  const double u = clamp0(yu);
   const double g = me.gw;
   const double surfh = me.surfh;
-  const double area = e.area[p], zmax = e.zmax[p], zmin = e.zmin[p];
-  const double depth = e.depth[p], dinf = e.dinf[p];
 
   // EtExtract: src/hydrol.c:33-97
-  const double edir = e.edir[p], ett = e.ett[p];
   double edir_s, edir_u, edir_g, ett_u, ett_g;
   if (surfh >= DEPRSTG) { edir_s = edir; edir_u = 0.0; edir_g = 0.0; }
   else if (g > depth - dinf) { edir_s = 0.0; edir_u = 0.0; edir_g = edir; }
   else { edir_s = 0.0; edir_u = edir; edir_g = 0.0; }
-  if (g > depth - e.rzd[p]) { ett_u = 0.0; ett_g = ett; }
+  if (g > depth - rzd) { ett_u = 0.0; ett_g = ett; }
   else { ett_u = ett; ett_g = 0.0; }
 
   // LateralFlow: src/lat_flow.c:29-63
@@ -185,52 +261,47 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   bool bnd[3] = {false, false, false};
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    const int c = e.nb[j * ne + p];
-    bnd[j] = (c == NB_BOUND);
-    const double dist = e.nabrdist[j * ne + p];
-    const double edge = e.edge[j * ne + p];
-    if (c >= 0) {
-      const NbrRec nr = e.rec[c];
+    bnd[j] = (c[j] == NB_BOUND);
+    if (c[j] >= 0) {
+      const NbrRec &nr = nrec[j];
       // SubFlowElemToElem: src/lat_flow.c:285-309
       {
         const double diff_h = me.gwz - nr.gwz;
         const double avg_h = avg_h_sub(diff_h, g, nr.gw);
-        const double grad_h = diff_h / dist;
+        const double grad_h = diff_h / dist[j];
         const double avg_ksat = 0.5 * (me.effkh + nr.effkh);
-        sub[j] = avg_ksat * grad_h * avg_h * edge;
+        sub[j] = avg_ksat * grad_h * avg_h * edge[j];
       }
       // OvlFlowElemToElem: src/lat_flow.c:311-338
       {
         double avg_sf = 0.5 * (me.sfmag + nr.sfmag);
         const double diff_h = (a.surf_mode == KINEMATIC) ? zmax - nr.zmax : me.head - nr.head;
         const double avg_h = avg_hsurf(diff_h, surfh, nr.surfh);
-        const double grad_h = diff_h / dist;
+        const double grad_h = diff_h / dist[j];
         if (a.surf_mode == KINEMATIC) avg_sf = (grad_h > 0.0) ? grad_h : GRADMIN;
         else avg_sf = (avg_sf > GRADMIN) ? avg_sf : GRADMIN;
         const double avg_rough = 0.5 * (me.rough + nr.rough);
-        const double crossa = avg_h * edge;
+        const double crossa = avg_h * edge[j];
         ovl[j] = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
       }
-    } else if (c == NB_BOUND) {
+    } else if (c[j] == NB_BOUND) {
       // BoundFluxElem: src/lat_flow.c:340-382
-      const int bt = e.bc_type[j * ne + p];
-      const double bcv = e.ebc[j * ne + p];
       ovl[j] = 0.0;
-      if (bt == 0) {
+      if (bt[j] == 0) {
         sub[j] = 0.0;
-      } else if (bt > 0) {
-        const double diff_h = me.gwz - bcv;
-        const double avg_h = avg_h_sub(diff_h, g, bcv - zmin);
-        const double grad_h = diff_h / dist;
-        sub[j] = me.effkh * grad_h * avg_h * edge;
+      } else if (bt[j] > 0) {
+        const double diff_h = me.gwz - bcv[j];
+        const double avg_h = avg_h_sub(diff_h, g, bcv[j] - zmin);
+        const double grad_h = diff_h / dist[j];
+        sub[j] = me.effkh * grad_h * avg_h * edge[j];
       } else {
-        sub[j] = -bcv;
+        sub[j] = -bcv[j];
       }
     } else {
       // river edge: the overland flux carried from the previous call is what
       // Infil reads (src/vert_flow.c:59-62); both fluxes are replaced later.
       bank = true;
-      ovl[j] = a.r.hid_ovl[-c - 1];
+      ovl[j] = hov[j];
       sub[j] = 0.0;
     }
   }
@@ -244,8 +315,8 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     satn = u / deficit;
     satn = (satn > 1.0) ? 1.0 : satn;
     satn = (satn < SATMIN) ? SATMIN : satn;
-    psi_raw = psi_vg(satn, e.alpha[p], e.beta[p]);
-    kr = kr_func(e.beta[p], satn);
+    psi_raw = psi_vg(satn, alpha, beta);
+    kr = kr_func(beta, satn);
   }
 
   // Infil: src/vert_flow.c:33-130
@@ -257,7 +328,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
 #pragma unroll
     for (int j = 0; j < 3; j++) applrate += -ovl[j] / area;
     applrate = (applrate > 0.0) ? applrate : 0.0;
-    applrate += e.pcpdrp[p];
+    applrate += pcpdrp;
     double wetfrac = surfh / DEPRSTG;
     wetfrac = (wetfrac > 0.0) ? wetfrac : 0.0;
     wetfrac = (wetfrac < 1.0) ? wetfrac : 1.0;
@@ -266,19 +337,18 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
       dh_by_dz = (surfh + zmax - (g + zmin)) / (0.5 * (surfh + dinf));
       dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
       // KrFunc(beta, 1) is exactly 1 for beta > 1 (1^m = 1, 0^(1/m) = 0)
-      const double satkfunc = (e.beta[p] > 1.0) ? 1.0 : kr_func(e.beta[p], 1.0);
-      kinf = eff_kinf(e.kinfv[p], e.kmacv[p], e.areafh[p], dh_by_dz, satkfunc, 1.0, applrate, surfh);
+      const double satkfunc = (beta > 1.0) ? 1.0 : kr_func(beta, 1.0);
+      kinf = eff_kinf(kinfv, kmacv, areafh, dh_by_dz, satkfunc, 1.0, applrate, surfh);
       infil = kinf * dh_by_dz;
     } else {
       const double psi_u = (psi_raw > PSIMIN) ? psi_raw : PSIMIN;
       const double h_u = psi_u + zmax - 0.5 * dinf;
       dh_by_dz = (surfh + zmax - h_u) / (0.5 * (surfh + dinf));
       dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
-      kinf = eff_kinf(e.kinfv[p], e.kmacv[p], e.areafh[p], dh_by_dz, kr, satn, applrate, surfh);
+      kinf = eff_kinf(kinfv, kmacv, areafh, dh_by_dz, kr, satn, applrate, surfh);
       infil = kinf * dh_by_dz;
       infil = (infil > 0.0) ? infil : 0.0;
     }
-    const double ws0 = e.ws0s[p];
     const double infil_max = applrate + ((ws0 > 0.0) ? ws0 / ix.dt : 0.0);
     infil = (infil > infil_max) ? infil_max : infil;
     infil *= wetfrac;
@@ -290,10 +360,26 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     rechg = infil;
   } else {
     const double dh_by_dz = (0.5 * deficit + psi_raw) / (0.5 * (deficit + g));
-    const double kavg = avg_kv(e.dmac[p], e.ksatv[p], e.kmacv[p], e.areafh[p], deficit, g, kr);
+    const double kavg = avg_kv(dmac, ksatv, kmacv, areafh, deficit, g, kr);
     rechg = kavg * dh_by_dz;
     rechg = (rechg > 0.0 && u <= 0.0) ? 0.0 : rechg;
     rechg = (rechg < 0.0 && g <= 0.0) ? 0.0 : rechg;
+  }
+
+  // River edges: this call's bank fluxes (src/river_flow.c:113-183, from the
+  // river blocks of k_prep) replace the carried overland flux that Infil has
+  // used, and the new overland flux is carried to the next call
+  // (src/river_flow.c:159-182).
+  if (bank) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (c[j] < 0 && c[j] != NB_BOUND) {
+        const int slot = -c[j] - 1;
+        ovl[j] = a.r.hid_ovl[2 * a.nr + slot];
+        sub[j] = a.r.bank_sub[slot];
+        a.r.hid_ovl[slot] = ovl[j];
+      }
+    }
   }
 
   if (a.record) {
@@ -308,42 +394,81 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
 
   // element ODEs: src/ode.c:121-149
   double dys = 0.0, dyu = 0.0, dyg = 0.0;
-  dys += e.pcpdrp[p] - infil - edir_s;
+  dys += pcpdrp - infil - edir_s;
   dyu += infil - rechg - edir_u - ett_u;
   dyg += rechg - edir_g - ett_g;
-  dyu /= e.porosity[p];
-  ydot[ix.ne + lp] = dyu;
-  bool nan = isnan(dyu);
-  if (!bank) {
+  dyu /= porosity;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      dys -= ovl[j] / area;
-      dyg -= sub[j] / area;
-    }
-    dyg /= e.porosity[p];
-    ydot[lp] = dys;
-    ydot[2 * ix.ne + lp] = dyg;
-    nan = nan || isnan(dys) || isnan(dyg);
-  } else {
-    BankRec *b = &e.bank[e.bank_id[p]];
-    b->dys0 = dys;
-    b->dyg0 = dyg;
-#pragma unroll
-    for (int j = 0; j < 3; j++) { b->ovl[j] = ovl[j]; b->sub[j] = sub[j]; }
+  for (int j = 0; j < 3; j++) {
+    dys -= ovl[j] / area;
+    dyg -= sub[j] / area;
   }
-  if (nan) flag_nan<B>(a, ix);
+  dyg /= porosity;
+  ydot[lp] = dys;
+  ydot[ix.ne + lp] = dyu;
+  ydot[2 * ix.ne + lp] = dyg;
+  if (isnan(dys) || isnan(dyu) || isnan(dyg)) flag_nan<B>(a, ix);
   if (!B && a.closure) {
     // volume rates crossing the element's outer boundary
     double out = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; j++)
       if (bnd[j]) out += ovl[j] + sub[j];
-    a.cl_e[p] = e.pcpdrp[p] * area;
+    a.cl_e[p] = pcpdrp * area;
     a.cl_e[ne + p] = ((((edir_s + edir_u) + edir_g) + ett_u) + ett_g) * area;
     a.cl_e[2L * ne + p] = out;
   }
 }
 
+// What a river segment reads of one bank element: the fields k_prep stores in
+// the element's record, evaluated from the same operands with the same
+// expressions (the river fluxes run in k_prep, beside the element records).
+struct BankView {
+  double surfh, gw, gwz, effkh, zmax, zmin;
+};
+
+template <bool B>
+__device__ __forceinline__ BankView bank_view(const RhsArgs &a, const Ix &ix, int q) {
+  const ElemDev &e = a.e;
+  const int lq = q - ix.eoff;
+  const double s = clamp0(ix.y[lq]);
+  const double g = clamp0(ix.y[2 * ix.ne + lq]);
+  BankView v;
+  v.surfh = surf_h(s);
+  v.gw = g;
+  v.zmin = e.zmin[q];
+  v.gwz = g + v.zmin;
+  v.effkh = eff_kh(e.depth[q], e.dmac[q], e.kmach[q], e.areafv[q], e.ksath[q], g);
+  v.zmax = e.zmax[q];
+  return v;
+}
+
+// SubFlowRiverToRiver (_ARITH_): src/river_flow.c:302-331, aquifer flow from
+// segment u into its downstream segment d, from the bank elements' records
+// (yg: river groundwater of the solve, by river index).  Evaluated in k_flux
+// by d for each upstream u and by u for itself, so both get the same bits.
+__device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, int u, int d) {
+  const RiverDev &R = a.r;
+  const double effk = 0.5 * (a.e.rec[R.left[u]].effkh + a.e.rec[R.right[u]].effkh);
+  const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
+  const double rgw = clamp0(yg[u]);
+  const double dgw = clamp0(yg[d]);
+  const double total_h = rgw + R.zmin[u];
+  const double total_h_down = dgw + R.zmin[d];
+  const double avg_wid = (R.width[u] + R.width[d]) / 2.0;
+  const double diff_h = total_h - total_h_down;
+  const double avg_h = avg_h_sub(diff_h, rgw, dgw);
+  const double distance = 0.5 * (R.length[u] + R.length[d]);
+  const double grad_h = diff_h / distance;
+  const double avg_ksat = 0.5 * (effk + effk_nabr);
+  return avg_ksat * grad_h * avg_h * avg_wid;
+}
+
+// River segment fluxes (src/river_flow.c:10-87, 113-183, 257-300, 333-518),
+// run by the river blocks of k_prep: rivflow 1-8, the upstream-node boundary
+// flux, and the bank-edge replacement values the bank elements take in
+// k_flux.  The aquifer flow to the downstream segment (rivflow 9) needs the
+// downstream banks' records and is left to k_flux.
 template <bool B>
 __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   const int nr = a.nr;
@@ -352,7 +477,7 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   if (!ix.active) return;
   if (B && r - ix.roff >= ix.nr) return;            // unused capacity of a slot
   const double *ys = ix.y + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
-  const double *yg = ys + ix.nr;                           // yg[r']: river groundwater
+  const double *yg = ys + ix.nr;                    // yg[r']: river groundwater
   const double stage = clamp0(ys[r]);
   const double rgw = clamp0(yg[r]);
   const int ord = R.ord[r];
@@ -360,11 +485,10 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   const double rzmin = R.zmin[r];
   const double bcv = R.bc[r];
   const int dcode = R.down[r];
-  const int L = R.left[r], Rr = R.right[r];
-  const NbrRec recL = a.e.rec[L];
-  const NbrRec recR = a.e.rec[Rr];
+  const BankView bl = bank_view<B>(a, ix, R.left[r]);
+  const BankView br = bank_view<B>(a, ix, R.right[r]);
 
-  double upinit = 0.0, rf1, rf9;
+  double upinit = 0.0, rf1;
   if (dcode >= 0) {
     const int d = dcode;
     if (R.bctype[r] != 0) {
@@ -402,21 +526,6 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
       const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
       rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
     }
-    // SubFlowRiverToRiver (_ARITH_): src/river_flow.c:302-331
-    {
-      const double effk = 0.5 * (recL.effkh + recR.effkh);
-      const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
-      const double dgw = clamp0(yg[d]);
-      const double total_h = rgw + rzmin;
-      const double total_h_down = dgw + R.zmin[d];
-      const double avg_wid = (R.width[r] + R.width[d]) / 2.0;
-      const double diff_h = total_h - total_h_down;
-      const double avg_h = avg_h_sub(diff_h, rgw, dgw);
-      const double distance = 0.5 * (length + R.length[d]);
-      const double grad_h = diff_h / distance;
-      const double avg_ksat = 0.5 * (effk + effk_nabr);
-      rf9 = avg_ksat * grad_h * avg_h * avg_wid;
-    }
   } else {
     // OutletFlux: src/river_flow.c:333-390
     double q = 0.0;
@@ -447,45 +556,21 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
       }
     }
     rf1 = q;
-    rf9 = 0.0;
   }
 
   // RiverToElem: src/river_flow.c:113-183
   const double rzmax = R.zmax[r];
   const double cwr = R.cwr[r];
-  const double rf2 = ovl_elem_to_river(rzmax, recL.zmax, recL.surfh, zbed, stage, cwr, length);
-  const double rf3 = ovl_elem_to_river(rzmax, recR.zmax, recR.surfh, zbed, stage, cwr, length);
-  const double effk_left = recL.effkh, effk_right = recR.effkh;
-  const double zminL = a.e.zmin[L], zminR = a.e.zmin[Rr];
-  // ChanFlowElemToRiver: src/river_flow.c:429-460
-  auto chan_e2r = [&](const NbrRec &ec, double ezmin, double effk, double distance) {
-    const double diff_h = (stage + zbed) - ec.gwz;
-    double avg_h;
-    if (ezmin > zbed) avg_h = ec.gw;
-    else if (ec.gwz > zbed) avg_h = ec.gwz - zbed;
-    else avg_h = 0.0;
-    avg_h = avg_h_sub(diff_h, stage, avg_h);
-    const double grad_h = diff_h / distance;
-    const double avg_ksat = 0.5 * (effk + R.ksath[r]);
-    return length * avg_ksat * grad_h * avg_h;
-  };
-  // SubFlowElemToRiver (_ARITH_): src/river_flow.c:462-498
-  auto sub_e2r = [&](const NbrRec &ec, double ezmin, double effk, double effk_riv, double distance) {
-    const double diff_h = (rgw + rzmin) - ec.gwz;
-    double avg_h;
-    if (ezmin > zbed) avg_h = 0.0;
-    else if (ec.gwz > zbed) avg_h = zbed - ezmin;
-    else avg_h = ec.gw;
-    avg_h = avg_h_sub(diff_h, rgw, avg_h);
-    const double avg_ksat = 0.5 * (effk + effk_riv);
-    const double grad_h = diff_h / distance;
-    return length * avg_ksat * grad_h * avg_h;
-  };
-  const double rf4 = chan_e2r(recL, zminL, effk_left, R.dist_l[r]);
-  const double rf5 = chan_e2r(recR, zminR, effk_right, R.dist_r[r]);
-  const double effk_riv = 0.5 * (effk_left + effk_right);
-  const double rf7 = sub_e2r(recL, zminL, effk_left, effk_riv, R.dist_l[r]);
-  const double rf8 = sub_e2r(recR, zminR, effk_right, effk_riv, R.dist_r[r]);
+  const double rksath = R.ksath[r];
+  const double rf2 = ovl_elem_to_river(rzmax, bl.zmax, bl.surfh, zbed, stage, cwr, length);
+  const double rf3 = ovl_elem_to_river(rzmax, br.zmax, br.surfh, zbed, stage, cwr, length);
+  const double rf4 = chan_elem_to_river(stage, zbed, length, rksath, bl.gw, bl.gwz, bl.zmin, bl.effkh, R.dist_l[r]);
+  const double rf5 = chan_elem_to_river(stage, zbed, length, rksath, br.gw, br.gwz, br.zmin, br.effkh, R.dist_r[r]);
+  const double effk_riv = 0.5 * (bl.effkh + br.effkh);
+  const double rf7 = sub_elem_to_river(rgw, rzmin, zbed, length, bl.gw, bl.gwz, bl.zmin, bl.effkh, effk_riv,
+                                       R.dist_l[r]);
+  const double rf8 = sub_elem_to_river(rgw, rzmin, zbed, length, br.gw, br.gwz, br.zmin, br.effkh, effk_riv,
+                                       R.dist_r[r]);
 
   // ChanLeak: src/river_flow.c:500-518
   double rf6;
@@ -507,7 +592,6 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   rf[6 * nr + r] = rf6;
   rf[7 * nr + r] = rf7;
   rf[8 * nr + r] = rf8;
-  rf[9 * nr + r] = rf9;
   R.upinit[r] = upinit;
   // bank-edge replacement values (src/river_flow.c:159-182)
   R.hid_ovl[2 * nr + 2 * r] = -rf2;
@@ -516,11 +600,51 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   R.bank_sub[2 * r + 1] = -(rf5 + rf8);
 }
 
-// Grid: nbr river blocks first (their dependent gathers and channel geometry
-// are the longest per thread, so they start with the first wave of
-// workgroups instead of trailing the element blocks), then nbe element blocks.
+// River ODEs (src/ode.c:238-251), run by the river blocks of k_flux: the
+// aquifer flow to the downstream segment (rivflow 9), and the upstream inflow
+// in ascending caller index, the order of the reference's serial loop
+// (src/river_flow.c:96-110): each upstream segment's channel outflow as
+// k_prep stored it, its aquifer outflow evaluated here.
 template <bool B>
-__global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
+__device__ __forceinline__ void river_ode(const RhsArgs &a, int r) {
+  const int nr = a.nr;
+  const RiverDev &R = a.r;
+  const Ix ix = ix_river<B>(a, r);
+  if (!ix.active) return;
+  if (B && r - ix.roff >= ix.nr) return;
+  const double *yg = ix.y + 3L * ix.ne - ix.roff + ix.nr;   // yg[r']: river groundwater
+  const int dcode = R.down[r];
+  const double rf9 = (dcode >= 0) ? sub_r2r(a, yg, r, dcode) : 0.0;
+  double up = R.upinit[r], upa = 0.0;
+  for (int k = R.up_beg[r]; k < R.up_end[r]; k++) {
+    const int u = R.up_idx[k];
+    up -= R.rf[1 * nr + u];
+    upa -= sub_r2r(a, yg, u, r);
+  }
+  const double area = R.area[r];
+  const double rf6 = R.rf[6 * nr + r];
+  double ds = 0.0;
+  ds -= up / area;
+#pragma unroll
+  for (int j = 1; j <= 6; j++) ds -= R.rf[j * nr + r] / area;
+  double dg = 0.0;
+  dg += -R.rf[7 * nr + r] - R.rf[8 * nr + r] - rf9 - upa + rf6;
+  dg /= R.porosity[r] * area;
+  double *ydr = ix.ydot + 3L * ix.ne - ix.roff;
+  ydr[r] = ds;
+  ydr[ix.nr + r] = dg;
+  if (a.record) {
+    R.rf[0 * nr + r] = up;
+    R.rf[9 * nr + r] = rf9;
+    R.rf[10 * nr + r] = upa;
+  }
+  if (isnan(ds) || isnan(dg)) flag_nan<B>(a, ix);
+}
+
+// Grid: nbr river blocks (river ODEs, short), then nbe element blocks.  At
+// most 168 VGPRs: three wavefronts per SIMD.
+template <bool B>
+__global__ __launch_bounds__(kBlock, 3) void k_flux(RhsArgs a, int nbe) {
   const int nbr = (int)gridDim.x - nbe;
   if ((int)blockIdx.x >= nbr) {
     const int tile = xcd_tile((int)blockIdx.x - nbr, nbe);
@@ -529,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(RhsArgs a, int nbe) {
   } else {
     const int rb = blockIdx.x;
     const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
-    if (r >= 0 && r < a.nr) river_flux<B>(a, r);
+    if (r >= 0 && r < a.nr) river_ode<B>(a, r);
   }
 }
 
@@ -578,88 +702,6 @@ __global__ __launch_bounds__(kBlock) void k_closure_final(RhsArgs a, int nb, dou
   }
 }
 
-template <bool B>
-__global__ __launch_bounds__(kBlock) void k_assemble(RhsArgs a, int nbr) {
-  int t = blockIdx.x * kBlock + threadIdx.x;
-  const int ne = a.ne, nr = a.nr;
-  const RiverDev &R = a.r;
-  if (B && a.bm) {
-    // compact launch: river blocks [0, nbr), then bank-record blocks
-    if ((int)blockIdx.x < nbr) {
-      t = compact_index(a, 1, blockIdx.x);
-      if (t < 0) return;
-    } else {
-      t = compact_index(a, 2, blockIdx.x - nbr);
-      if (t < 0) return;
-      t += nr;
-    }
-  }
-  if (t < nr) {
-    const int r = t;
-    const Ix ix = ix_river<B>(a, r);
-    if (!ix.active) return;
-    if (B && r - ix.roff >= ix.nr) return;
-    // src/river_flow.c:96-110 in ascending upstream index
-    double up = R.upinit[r], upa = 0.0;
-    for (int k = R.up_beg[r]; k < R.up_end[r]; k++) {
-      const int u = R.up_idx[k];
-      up -= R.rf[1 * nr + u];
-      upa -= R.rf[9 * nr + u];
-    }
-    const double area = R.area[r];
-    const double rf6 = R.rf[6 * nr + r];
-    // src/ode.c:238-251
-    double ds = 0.0;
-    ds -= up / area;
-#pragma unroll
-    for (int j = 1; j <= 6; j++) ds -= R.rf[j * nr + r] / area;
-    double dg = 0.0;
-    dg += -R.rf[7 * nr + r] - R.rf[8 * nr + r] - R.rf[9 * nr + r] - upa + rf6;
-    dg /= R.porosity[r] * area;
-    double *ydr = ix.ydot + 3L * ix.ne - ix.roff;
-    ydr[r] = ds;
-    ydr[ix.nr + r] = dg;
-    if (a.record) {
-      R.rf[0 * nr + r] = up;
-      R.rf[10 * nr + r] = upa;
-    }
-    if (isnan(ds) || isnan(dg)) flag_nan<B>(a, ix);
-  } else if (t < nr + a.nbank) {
-    const int k = t - nr;
-    const int p = a.e.bank_elem[k];
-    if (p < 0) return;                              // unused bank record of a slot
-    const Ix ix = ix_elem<B>(a, p);
-    if (!ix.active) return;
-    const BankRec b = a.e.bank[k];
-    const double area = a.e.area[p];
-    double dys = b.dys0, dyg = b.dyg0;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const int c = a.e.nb[j * ne + p];
-      double o, s;
-      if (c >= 0 || c == NB_BOUND) {
-        o = b.ovl[j];
-        s = b.sub[j];
-      } else {
-        const int slot = -c - 1;
-        o = R.hid_ovl[2 * nr + slot];
-        s = R.bank_sub[slot];
-        R.hid_ovl[slot] = o;  // carried to the next call's Infil
-        if (a.record) {
-          a.rec.ovl[j * ne + p] = o;
-          a.rec.sub[j * ne + p] = s;
-        }
-      }
-      dys -= o / area;
-      dyg -= s / area;
-    }
-    dyg /= a.e.porosity[p];
-    double *yde = ix.ydot + (p - ix.eoff);
-    yde[0] = dys;
-    yde[2 * ix.ne] = dyg;
-    if (isnan(dys) || isnan(dyg)) flag_nan<B>(a, ix);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -748,8 +790,6 @@ int validate(const pihm_mesh_desc *m) {
 
 namespace {
 
-inline int bank_cap(const SlotCap &s) { return std::min(s.ne, 2 * s.nr); }
-
 // Allocate a context with one slot per entry of caps.  Element, river, bank,
 // upstream-list and state ranges of the slots are concatenated at their
 // capacities; slots start empty (no elements, no rivers) until a mesh is
@@ -760,19 +800,18 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   if (nmem < 1) return PIHM_ERR_ARG;
   if (surf_mode != KINEMATIC && surf_mode != DIFF_WAVE) return PIHM_ERR_ARG;
   if (riv_mode != KINEMATIC && riv_mode != DIFF_WAVE) return PIHM_ERR_ARG;
-  long ne = 0, nr = 0, neq = 0, nbank = 0, nup = 0;
+  long ne = 0, nr = 0, neq = 0, nup = 0;
   for (const auto &s : caps) {
     if (s.ne < 1 || s.nr < 0) return PIHM_ERR_ARG;
-    ne += s.ne; nr += s.nr; neq += 3L * s.ne + 2L * s.nr; nbank += bank_cap(s); nup += s.nr;
+    ne += s.ne; nr += s.nr; neq += 3L * s.ne + 2L * s.nr; nup += s.nr;
   }
   if (ne > INT32_MAX / 4 || nr > INT32_MAX / 16 || neq > INT32_MAX) return PIHM_ERR_TOO_LARGE;
   pihm_ctx *c = new pihm_ctx();
   c->nmem = nmem;
   c->caps = caps;
   c->members.resize(nmem);
-  c->boff.resize(nmem);
   c->uoff.resize(nmem);
-  long e0 = 0, r0 = 0, y0 = 0, b0 = 0, u0 = 0;
+  long e0 = 0, r0 = 0, y0 = 0, u0 = 0;
   for (int k = 0; k < nmem; k++) {
     auto &M = c->members[k];
     M.eoff = (int)e0; M.ne = 0;
@@ -780,13 +819,11 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
     M.yoff = y0; M.neq = 0;
     M.ceoff = 0; M.croff = 0; M.cyoff = 0;
     M.dt = 60.0;
-    M.nbank = 0;
-    c->boff[k] = (int)b0;
     c->uoff[k] = (int)u0;
     e0 += caps[k].ne; r0 += caps[k].nr; y0 += 3L * caps[k].ne + 2L * caps[k].nr;
-    b0 += bank_cap(caps[k]); u0 += caps[k].nr;
+    u0 += caps[k].nr;
   }
-  c->ne = (int)ne; c->nr = (int)nr; c->neq = (int)neq; c->nbank = (int)nbank;
+  c->ne = (int)ne; c->nr = (int)nr; c->neq = (int)neq;
   c->surf_mode = surf_mode; c->riv_mode = riv_mode; c->dt = 60.0;
   c->stream = nullptr; c->nan_check = 1; c->record = 0; c->ncalls = 0;
   c->arena = nullptr;
@@ -811,11 +848,7 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   for (auto p : ed3) ar.add(p, 3L * ne);
   ar.add(&E.nb, 3L * ne);
   ar.add(&E.bc_type, 3L * ne);
-  ar.add(&E.bank_id, ne);
   ar.add(&E.rec, ne);
-  const long nb1 = std::max<long>(nbank, 1);
-  ar.add(&E.bank, nb1);
-  ar.add(&E.bank_elem, nb1);
   const long nr1 = std::max<long>(nr, 1);
   int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord, &R.up_beg, &R.up_end};
   for (auto p : ri) ar.add(p, nr1);
@@ -848,15 +881,11 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   if (rc) { pihm_ctx_destroy(c); return rc; }
   ar.carve((char *)c->arena);
   *c->nanflag_host = 0;
-  // empty slots: no banks, empty upstream lists, boundary codes
-  std::vector<int32_t> neg(nb1, -1);
+  // empty slots: empty upstream lists, boundary codes
   std::vector<int32_t> rm(nr1, 0), ub(nr1, 0);
   for (int k = 0; k < nmem; k++)
     for (int r = 0; r < caps[k].nr; r++) { rm[c->members[k].roff + r] = k; ub[c->members[k].roff + r] = c->uoff[k]; }
-  std::vector<int32_t> bid(ne, -1);
-  if (hipMemcpy(E.bank_elem, neg.data(), nb1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(E.bank_id, bid.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(E.nb, c->nb_host.data(), 3L * ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(E.nb, c->nb_host.data(), 3L * ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(emem_dev, c->emem_host.data(), ne * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(rmem_dev, rm.data(), nr1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(R.up_beg, ub.data(), nr1 * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -867,15 +896,15 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   c->mem_host.resize(nmem);
   for (int k = 0; k < nmem; k++) {
     const auto &M = c->members[k];
-    c->mem_host[k] = MemberDev{M.yoff, M.eoff, 0, M.roff, 0, M.dt, 0, c->boff[k], 0, 0};
+    c->mem_host[k] = MemberDev{M.yoff, M.eoff, 0, M.roff, 0, M.dt, 0, 0};
   }
   if (hipMemcpy(c->mem_dev, c->mem_host.data(), nmem * sizeof(MemberDev), hipMemcpyHostToDevice) != hipSuccess) {
     pihm_ctx_destroy(c);
     return PIHM_ERR_HIP;
   }
-  A.ne = (int)ne; A.nr = (int)nr; A.nbank = c->nbank;
+  A.ne = (int)ne; A.nr = (int)nr;
   A.surf_mode = surf_mode; A.riv_mode = riv_mode; A.dt = c->dt;
-  A.cur = 0; A.record = 0;
+  A.record = 0;
   A.nanflag = c->nanflag_dev;
   A.nmem = nmem;
   A.mem = c->mem_dev;
@@ -918,7 +947,7 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
   if (n > cap.ne || nrk > cap.nr) return PIHM_ERR_TOO_LARGE;
   auto &M = c->members[k];
   const int off = M.eoff, roff = M.roff;
-  const long ne = c->ne, nr = c->nr;
+  const long ne = c->ne;
 
   // Hilbert order of the centroids
   double xmin = m->x[0], xmax = m->x[0], ymin = m->y[0], ymax = m->y[0];
@@ -943,13 +972,10 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
   }
   for (int lp = 0; lp < n; lp++) c->iperm[off + order[lp]] = off + lp;
 
-  // neighbour codes and banks
-  const int nbc = bank_cap(cap);
-  std::vector<int32_t> nbk(3L * cap.ne, NB_BOUND), bank_id(cap.ne, -1), bank_elem(std::max(nbc, 1), -1);
-  int nb_used = 0;
+  // neighbour codes
+  std::vector<int32_t> nbk(3L * cap.ne, NB_BOUND);
   for (int lp = 0; lp < n; lp++) {
     const int i = order[lp];
-    bool bank = false;
     for (int j = 0; j < 3; j++) {
       const int v = m->nabr[3 * i + j];
       int32_t code;
@@ -959,15 +985,8 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
         const int r = -v - 1;
         const int side = (m->leftele[r] == i + 1) ? 0 : 1;
         code = -(2 * (roff + r) + side + 1);
-        bank = true;
       }
       nbk[(long)j * cap.ne + lp] = code;
-    }
-    if (bank) {
-      if (nb_used >= nbc) return PIHM_ERR_TOO_LARGE;
-      bank_id[lp] = c->boff[k] + nb_used;
-      bank_elem[nb_used] = off + lp;
-      nb_used++;
     }
   }
   for (int j = 0; j < 3; j++)
@@ -1028,9 +1047,6 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
                                 hipMemcpyHostToDevice, st));
     PIHM_HIP_TRY(hipStreamSynchronize(st));
   }
-  PIHM_HIP_TRY(hipMemcpyAsync(E.bank_id + off, bank_id.data(), cap.ne * 4, hipMemcpyHostToDevice, st));
-  if (nbc > 0)
-    PIHM_HIP_TRY(hipMemcpyAsync(E.bank_elem + c->boff[k], bank_elem.data(), nbc * 4, hipMemcpyHostToDevice, st));
   {
     std::vector<NbrRec> rec(cap.ne);
     std::memset(rec.data(), 0, cap.ne * sizeof(NbrRec));
@@ -1072,15 +1088,12 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
     std::vector<double> zr(2L * cap.nr, 0.0);
     PIHM_HIP_TRY(hipMemcpyAsync(R.bc + roff, zr.data(), cap.nr * sizeof(double), hipMemcpyHostToDevice, st));
     PIHM_HIP_TRY(hipMemcpyAsync(R.hid_ovl + 2L * roff, zr.data(), 2L * cap.nr * sizeof(double), hipMemcpyHostToDevice, st));
-    PIHM_HIP_TRY(hipMemcpyAsync(R.hid_ovl + 2L * nr + 2L * roff, zr.data(), 2L * cap.nr * sizeof(double),
-                                hipMemcpyHostToDevice, st));
   }
   M.ne = n;
   M.nr = nrk;
   M.neq = 3 * n + 2 * nrk;
   M.dt = m->stepsize;
-  M.nbank = nb_used;
-  c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, c->boff[k], nb_used, 0};
+  c->mem_host[k] = MemberDev{M.yoff, M.eoff, M.ne, M.roff, M.nr, M.dt, 1, 0};
   PIHM_HIP_TRY(hipMemcpyAsync(c->mem_dev + k, &c->mem_host[k], sizeof(MemberDev), hipMemcpyHostToDevice, st));
   PIHM_HIP_TRY(hipStreamSynchronize(st));
   if (k == 0) { c->dt = m->stepsize; c->args.dt = m->stepsize; }
@@ -1310,7 +1323,7 @@ extern "C" int pihm_ctx_state_from_device(pihm_ctx *c, N_Vector dev, double *hos
   return PIHM_OK;
 }
 
-// Launch the three kernels of one RHS call (no synchronisation).
+// Launch the two kernels of one RHS call (no synchronisation).
 extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
   RhsArgs a = c->args;
   a.y = y;
@@ -1318,15 +1331,12 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
   a.record = c->record;
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
-  const int nas = div_up((long)c->nr + c->nbank, kBlock);
   if (c->nmem > 1) {
-    hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
   } else {
-    hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    if (nas > 0) hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
     if (a.closure) {
       const int nb = std::max(1, std::min(kClosureBlocks, div_up(std::max(c->ne, c->nr), kBlock)));
       hipLaunchKernelGGL(k_closure_part, dim3(nb), dim3(kBlock), 0, c->stream, a);
@@ -1348,17 +1358,15 @@ int pihm_rhs_launch_io(pihm_ctx *c, const MemberIO *mio, int *mnan) {
   a.record = 0;
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
-  const int nas = div_up((long)c->nr + c->nbank, kBlock);
-  hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-  if (nas > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
   return PIHM_OK;
 }
 
 int pihm_rhs_launch_compact(pihm_ctx *c, const MemberIO *mio, int *mnan, const int *bm, const int *bm_mem,
-                            int nact, int nbe, int nbr, int nbb) {
+                            int nact, int nbe, int nbr) {
   if (!c || !mio || !bm || !bm_mem || nact < 1) return PIHM_ERR_ARG;
   RhsArgs a = c->args;
   a.y = nullptr;
@@ -1369,9 +1377,8 @@ int pihm_rhs_launch_compact(pihm_ctx *c, const MemberIO *mio, int *mnan, const i
   a.bm = bm;
   a.bm_mem = bm_mem;
   a.nact = nact;
-  if (nbe > 0) hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+  if (nbe + nbr > 0) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   if (nbe + nbr > 0) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-  if (nbr + nbb > 0) hipLaunchKernelGGL(k_assemble<true>, dim3(nbr + nbb), dim3(kBlock), 0, c->stream, a, nbr);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
   return PIHM_OK;
@@ -1486,36 +1493,31 @@ extern "C" int pihm_ctx_sync(pihm_ctx *c) {
   return PIHM_OK;
 }
 
-// Average duration of each of the three RHS kernels over `iters` back-to-back
-// evaluations, measured with HIP events on the context's stream.
+// Average duration of each of the two RHS kernels over `iters` evaluations,
+// each kernel between its own pair of HIP events on the context's stream.
 extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot, int iters,
-                                     double *ms_prep, double *ms_flux, double *ms_asm) {
+                                     double *ms_prep, double *ms_flux) {
   if (!c || !y || !ydot || iters <= 0) return PIHM_ERR_ARG;
   RhsArgs a = c->args;
   a.y = y;
   a.ydot = ydot;
   a.record = 0;
+  a.closure = 0;
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
-  const int nas = div_up((long)c->nr + c->nbank, kBlock);
-  hipEvent_t ev[4];
+  hipEvent_t ev[3];
   for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
-  float t[3] = {0, 0, 0};
+  float t[2] = {0, 0};
   for (int it = 0; it < iters; it++) {
     PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
-    if (c->nmem > 1) hipLaunchKernelGGL(k_prep<true>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_prep<false>, dim3(nbe), dim3(kBlock), 0, c->stream, a);
+    if (c->nmem > 1) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    else hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
     if (c->nmem > 1) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     else hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[2], c->stream));
-    if (nas > 0) {
-      if (c->nmem > 1) hipLaunchKernelGGL(k_assemble<true>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
-      else hipLaunchKernelGGL(k_assemble<false>, dim3(nas), dim3(kBlock), 0, c->stream, a, 0);
-    }
-    PIHM_HIP_TRY(hipEventRecord(ev[3], c->stream));
-    PIHM_HIP_TRY(hipEventSynchronize(ev[3]));
-    for (int k = 0; k < 3; k++) {
+    PIHM_HIP_TRY(hipEventSynchronize(ev[2]));
+    for (int k = 0; k < 2; k++) {
       float ms = 0;
       PIHM_HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       t[k] += ms;
@@ -1524,7 +1526,6 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
   for (auto &e : ev) (void)hipEventDestroy(e);
   if (ms_prep) *ms_prep = t[0] / iters;
   if (ms_flux) *ms_flux = t[1] / iters;
-  if (ms_asm) *ms_asm = t[2] / iters;
   return PIHM_OK;
 }
 
@@ -1558,4 +1559,38 @@ extern "C" int pihm_math_pow(int n, const double *x, const double *y, double *ou
   if (e == hipSuccess) e = hipMemcpy(out, d + 2L * n, bytes, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? PIHM_OK : PIHM_ERR_HIP;
+}
+
+// Average device time of one whole RHS evaluation (both kernels back to
+// back, as the integrator issues them) over `iters` evaluations: one HIP event
+// before the first launch and one after the last, on the context's stream.
+extern "C" int pihm_ctx_time_rhs(pihm_ctx *c, const double *y, double *ydot, int iters, double *ms_per_eval) {
+  if (!c || !y || !ydot || iters <= 0 || !ms_per_eval) return PIHM_ERR_ARG;
+  RhsArgs a = c->args;
+  a.y = y;
+  a.ydot = ydot;
+  a.record = 0;
+  a.closure = 0;
+  const int nbe = div_up(c->ne, kBlock);
+  const int nbr = div_up(c->nr, kBlock);
+  hipEvent_t ev[2];
+  for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
+  PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
+  for (int it = 0; it < iters; it++) {
+    if (c->nmem > 1) {
+      hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+      hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    } else {
+      hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+      hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    }
+  }
+  PIHM_HIP_TRY(hipGetLastError());
+  PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
+  PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+  float ms = 0;
+  PIHM_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms_per_eval = ms / iters;
+  return PIHM_OK;
 }

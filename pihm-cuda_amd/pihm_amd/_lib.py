@@ -122,7 +122,8 @@ _PROTOS = [
     ("pihm_ctx_set_closure", C.c_int, [C.c_void_p, C.c_int]),
     ("pihm_ctx_sync", C.c_int, [C.c_void_p]),
     ("pihm_ctx_time_kernels", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p,
-                                        c_double_p, c_double_p]),
+                                        c_double_p]),
+    ("pihm_ctx_time_rhs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p]),
     ("pihm_ctx_get_river_edge_ovlflow", C.c_int, [C.c_void_p, c_double_p]),
     ("pihm_ctx_set_river_edge_ovlflow", C.c_int, [C.c_void_p, c_double_p]),
     ("pihm_ctx_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p,

@@ -104,8 +104,15 @@ def test_xpow_outside_the_fast_path_is_libm():
 
 @pytest.mark.gpu
 def test_xpow_device_matches_host_bitwise():
+    """Same bits on host and device on the fast path (finite x > 0, |y ln x| < 708);
+    outside it each side calls its own library pow, which agree to an ulp or two."""
     parts = [np.stack([x, np.broadcast_to(y, x.shape)]) for x, y, _ in _cases(50000, seed=7).values()]
     xy = np.concatenate(parts, axis=1)
     host = xpow(xy[0], xy[1])
     dev = xpow(xy[0], xy[1], device=True)
-    assert np.array_equal(host.view(np.int64), dev.view(np.int64))
+    with np.errstate(divide="ignore"):
+        fast = (xy[0] > 0) & np.isfinite(xy[0]) & (np.abs(xy[1] * np.log(xy[0])) < 700.0)
+    assert fast.sum() > 0.9 * len(fast)
+    assert np.array_equal(host[fast].view(np.int64), dev[fast].view(np.int64))
+    slow = ~fast & np.isfinite(host)
+    assert np.all(_ulps(host[slow], dev[slow]) <= 2)
