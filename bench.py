@@ -25,6 +25,7 @@ from pihm_amd.forcing import StormForcing, run_config  # noqa: E402
 from pihm_amd.meshgen import config_mesh  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+GRAPHS = 1               # replay Krylov iterations as HIP graphs (off with --no-graphs)
 BYTES_PER_ELEM = 344     # algorithmic bytes of one RHS per element
 BYTES_PER_SEG = 200      # ... per river segment
 # HBM bytes one RHS evaluation at the 200,000-triangle workload moves, from the rocprofv3
@@ -35,7 +36,11 @@ BYTES_PER_SEG = 200      # ... per river segment
 RHS_TRAFFIC_BYTES = (43061.7 * 2.0 + 18547.7) * 1024
 
 
-def gpu_run(mesh, forcing, hours_total, mark_hours, graphs=1):
+def _cfg(t_end):
+    return run_config(t_end, graphs=GRAPHS)
+
+
+def gpu_run(mesh, forcing, hours_total, mark_hours):
     """Run the device path for hours_total simulated hours; return wall-clock marks at the
     requested hour boundaries (device synchronised at each), stats and kernel timings."""
     L = _lib.lib()
@@ -72,7 +77,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours, graphs=1):
 
     cb = _lib.StepEndFn(step_end)
     be.model_step_end = C.cast(cb, C.c_void_p)
-    cfg = run_config(hours_total * 3600.0, graphs=graphs)
+    cfg = _cfg(hours_total * 3600.0)
     st = _lib.RunStats()
     L.pihm_ctx_sync(ctx)
     marks[0.0] = time.perf_counter()
@@ -107,7 +112,7 @@ def ensemble_run(nmem, hours, sample, seed):
     specs = member_spec(nmem, seed=seed)
     meshes = build_meshes(specs)
     forc = [member_forcing(sp, m, hours + 3) for sp, m in zip(specs, meshes)]
-    cfg = run_config(hours * 3600.0)
+    cfg = _cfg(hours * 3600.0)
     e = Ensemble(meshes, forc, cfg)
     for k, m in enumerate(meshes):
         e.set_state(k, m.initial_state())
@@ -181,7 +186,7 @@ def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
     pool = ProcessPoolExecutor(max_workers=workers)
     futs = [pool.submit(_serve_payload, rq, rq["t_end"] / 3600.0) for rq in stream]
     reqs = [Request(i, rq["arrival"], rq["t_end"], futs[i]) for i, rq in enumerate(stream)]
-    eng = SlotEngine(caps, run_config(3600.0))
+    eng = SlotEngine(caps, _cfg(3600.0))
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0
     sched = Scheduler(eng, clock, ticks_per_round=8)
@@ -199,7 +204,7 @@ def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
     for i in pick:
         m, f, y0 = futs[i].result()
         cap = [c for c in caps if c[0] >= m.nelem][0]
-        e1 = SlotEngine([cap], run_config(3600.0))
+        e1 = SlotEngine([cap], _cfg(3600.0))
         r1 = Request(0, 0.0, reqs[i].t_end, (m, f, y0))
         Scheduler(e1, time.perf_counter, ticks_per_round=64).run([r1])
         e1.close()
@@ -226,7 +231,7 @@ def closure_run(days):
     f = StormForcing(int(days * 24) + 4, seed=11, nelem=m.nelem)
     y0 = m.initial_state()
     t0 = time.perf_counter()
-    rc, y1, st = run_gpu(m, f, run_config(days * 86400.0), y0, closure=True)
+    rc, y1, st = run_gpu(m, f, _cfg(days * 86400.0), y0, closure=True)
     wall = time.perf_counter() - t0
     if rc:
         raise RuntimeError(f"closure run failed: {rc}")
@@ -241,7 +246,7 @@ def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     run = O.OracleRun(mesh, forcing, nthreads=nthreads)
-    cfg = run_config(hours * 3600.0)
+    cfg = _cfg(hours * 3600.0)
     t0 = time.perf_counter()
     rc, y, st = run.run(mesh.initial_state(), cfg)
     return dict(rc=rc, wall=time.perf_counter() - t0, stats=st.as_dict(), yend=y)
@@ -306,7 +311,8 @@ def main():
                     help="seconds of simulated time per horizon unit (the request spec's unit is a day)")
     ap.add_argument("--serve-check", type=int, default=3)
     ap.add_argument("--no-graphs", action="store_true",
-                    help="launch each Krylov iteration's kernels one by one instead of replaying graphs")
+                    help="every device run launches each Krylov iteration's kernels one by one instead of "
+                         "replaying graphs")
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only run N RHS evaluations and N copy-kernel launches (counter passes)")
     args = ap.parse_args()
@@ -322,7 +328,9 @@ def main():
     W, K = args.warmup, args.steps
     forcing = StormForcing(W + K + 4, seed=11, nelem=mesh.nelem)
     S = args.cpu_hours
-    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S], graphs=0 if args.no_graphs else 1)
+    global GRAPHS
+    GRAPHS = 0 if args.no_graphs else 1
+    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S])
     if g["rc"] != 0 or g["nan"] != 0:
         raise RuntimeError(f"device run failed: rc={g['rc']} nan={g['nan']} {g['stats']}")
     elapsed = g["marks"][float(W + K)] - g["marks"][float(W)]

@@ -4,7 +4,7 @@
 // arrays in the caller's order):
 //   element statics   18 x double[ne]      area .. rzd
 //   edge statics       4 x double[3*ne]    edge, nabrdist, nabr_x, nabr_y  (index j*ne+p)
-//   edge codes         2 x int32[3*ne]     nb (neighbour code), bc_type
+//   edge codes         int32[3*ne]         nb (neighbour code, boundary type folded in)
 //   per-step inputs    4 x double[ne] + double[3*ne]
 //   neighbour record   NbrRec[ne]          values a neighbour reads, one 64-byte load
 //   river statics      17 x double[nr] + 6 x int32[nr], upstream CSR
@@ -18,9 +18,15 @@ namespace pihm {
 
 // Neighbour code nb[j*ne+p]:
 //   >= 0                 device index of the neighbouring element
-//   NB_BOUND             domain boundary (condition in bc_type)
+//   NB_BOUND + t         domain boundary; t = 0 no flow, 1 Dirichlet (bc_type
+//                        > 0), 2 Neumann (bc_type < 0): only the sign of the
+//                        reference's bc_type is used (src/lat_flow.c:340-382)
 //   other negative       river edge: -(2*r + side + 1), side 0 = left bank, 1 = right
 constexpr int32_t NB_BOUND = INT32_MIN;
+__host__ __device__ inline bool nb_is_bound(int32_t c) { return c <= NB_BOUND + 2; }
+__host__ __device__ inline bool nb_is_river(int32_t c) { return c < 0 && c > NB_BOUND + 2; }
+// the reference's bc_type sign of a boundary code: 0, 1 or -1
+__host__ __device__ inline int nb_bc_sign(int32_t c) { return c == NB_BOUND ? 0 : (c == NB_BOUND + 1 ? 1 : -1); }
 
 // What a neighbouring element needs from element p in one RHS call.
 struct alignas(64) NbrRec {
@@ -39,7 +45,7 @@ struct ElemDev {
   double *area, *zmin, *zmax, *depth, *ksath, *ksatv, *kinfv, *dinf, *alpha,
       *beta, *porosity, *dmac, *kmach, *kmacv, *areafv, *areafh, *rough, *rzd;
   double *edge, *nabrdist, *nabr_x, *nabr_y;  // [3*ne]
-  int32_t *nb, *bc_type;                      // [3*ne]
+  int32_t *nb;                                // [3*ne]
   // per-step inputs
   double *pcpdrp, *edir, *ett, *ws0s, *ebc;   // ebc [3*ne]
   NbrRec *rec;

@@ -104,19 +104,21 @@ __device__ __forceinline__ int compact_index(const RhsArgs &a, int kind, int b) 
 // then what the codes point at), so a wavefront waits for memory twice
 // instead of once per neighbour; the arithmetic follows them unchanged.
 template <bool B>
-__device__ __forceinline__ void river_flux(const RhsArgs &a, int r);
+__device__ __forceinline__ void river_flux(const RhsArgs &a, int r, int part);
 
-// Grid: nbr river blocks first (all river segment fluxes, the longest
-// threads of the launch), then nbe element blocks (the neighbour records).
+// Grid: 3*nbr river blocks first (the three parts of every segment's fluxes,
+// part-major, the longest threads of the launch), then nbe element blocks
+// (the neighbour records).
 template <bool B>
 __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
-  const int nbr = (int)gridDim.x - nbe;
-  if ((int)blockIdx.x < nbr) {
-    const int r = (B && a.bm) ? compact_index(a, 1, blockIdx.x) : (int)blockIdx.x * kBlock + threadIdx.x;
-    if (r >= 0 && r < a.nr) river_flux<B>(a, r);
+  const int nbr3 = (int)gridDim.x - nbe, nbr = nbr3 / 3;
+  if ((int)blockIdx.x < nbr3) {
+    const int part = (int)blockIdx.x / nbr, rb = (int)blockIdx.x - part * nbr;
+    const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
+    if (r >= 0 && r < a.nr) river_flux<B>(a, r, part);
     return;
   }
-  const int tile = xcd_tile((int)blockIdx.x - nbr, nbe);
+  const int tile = xcd_tile((int)blockIdx.x - nbr3, nbe);
   const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
   const int ne = a.ne;
   if (p < 0 || p >= ne) return;
@@ -132,13 +134,12 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
   const double ys = y[lp], yg = y[2 * ix.ne + lp];
   const double zmax = e.zmax[p], zmin = e.zmin[p], depth = e.depth[p], dmac = e.dmac[p];
   const double kmach = e.kmach[p], areafv = e.areafv[p], ksath = e.ksath[p];
-  int c[3] = {0, 0, 0}, bt[3] = {0, 0, 0};
+  int c[3] = {0, 0, 0};
   double nx[3] = {0.0, 0.0, 0.0}, ny[3] = {0.0, 0.0, 0.0};
   if (diff) {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       c[j] = e.nb[j * ne + p];
-      bt[j] = e.bc_type[j * ne + p];
       nx[j] = e.nabr_x[j * ne + p];
       ny[j] = e.nabr_y[j * ne + p];
     }
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
       if (c[j] >= 0) {
         ez[j] = e.zmax[c[j]];
         es[j] = y[c[j] - ix.eoff];
-      } else if (c[j] == NB_BOUND) {
+      } else if (nb_is_bound(c[j])) {
         bv[j] = e.ebc[j * ne + p];
       } else {
         const int r = (-c[j] - 1) >> 1;
@@ -182,8 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
     for (int j = 0; j < 3; j++) {
       if (c[j] >= 0) {
         h[j] = ez[j] + surf_h(clamp0(es[j]));
-      } else if (c[j] == NB_BOUND) {
-        h[j] = (bt[j] == 0) ? zmax + surfh : bv[j];
+      } else if (nb_is_bound(c[j])) {
+        h[j] = (c[j] == NB_BOUND) ? zmax + surfh : bv[j];
       } else {
         const double st = clamp0(rs[j]);
         h[j] = (st > rd[j]) ? rb[j] + st : rz[j];
@@ -221,12 +222,11 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const double edir = e.edir[p], ett = e.ett[p], pcpdrp = e.pcpdrp[p], ws0 = e.ws0s[p];
   const double alpha = e.alpha[p], beta = e.beta[p], kinfv = e.kinfv[p], kmacv = e.kmacv[p];
   const double areafh = e.areafh[p], dmac = e.dmac[p], ksatv = e.ksatv[p], porosity = e.porosity[p];
-  int c[3], bt[3];
+  int c[3];
   double dist[3], edge[3];
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     c[j] = e.nb[j * ne + p];
-    bt[j] = e.bc_type[j * ne + p];
     dist[j] = e.nabrdist[j * ne + p];
     edge[j] = e.edge[j * ne + p];
   }
@@ -236,7 +236,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   for (int j = 0; j < 3; j++) {
     if (c[j] >= 0) {
       nrec[j] = e.rec[c[j]];
-    } else if (c[j] == NB_BOUND) {
+    } else if (nb_is_bound(c[j])) {
       bcv[j] = e.ebc[j * ne + p];
     } else {
       hov[j] = a.r.hid_ovl[-c[j] - 1];     // the carried overland flux
@@ -261,7 +261,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   bool bnd[3] = {false, false, false};
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    bnd[j] = (c[j] == NB_BOUND);
+    bnd[j] = nb_is_bound(c[j]);
     if (c[j] >= 0) {
       const NbrRec &nr = nrec[j];
       // SubFlowElemToElem: src/lat_flow.c:285-309
@@ -284,12 +284,13 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
         const double crossa = avg_h * edge[j];
         ovl[j] = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
       }
-    } else if (c[j] == NB_BOUND) {
+    } else if (nb_is_bound(c[j])) {
       // BoundFluxElem: src/lat_flow.c:340-382
       ovl[j] = 0.0;
-      if (bt[j] == 0) {
+      const int bt = nb_bc_sign(c[j]);
+      if (bt == 0) {
         sub[j] = 0.0;
-      } else if (bt[j] > 0) {
+      } else if (bt > 0) {
         const double diff_h = me.gwz - bcv[j];
         const double avg_h = avg_h_sub(diff_h, g, bcv[j] - zmin);
         const double grad_h = diff_h / dist[j];
@@ -373,7 +374,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   if (bank) {
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      if (c[j] < 0 && c[j] != NB_BOUND) {
+      if (nb_is_river(c[j])) {
         const int slot = -c[j] - 1;
         ovl[j] = a.r.hid_ovl[2 * a.nr + slot];
         sub[j] = a.r.bank_sub[slot];
@@ -464,13 +465,26 @@ __device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, in
   return avg_ksat * grad_h * avg_h * avg_wid;
 }
 
+template <bool B>
+__device__ __forceinline__ double effkh_of(const RhsArgs &a, const Ix &ix, int q) {
+  const ElemDev &e = a.e;
+  const double g = clamp0(ix.y[2 * ix.ne + (q - ix.eoff)]);
+  return eff_kh(e.depth[q], e.dmac[q], e.kmach[q], e.areafv[q], e.ksath[q], g);
+}
+
 // River segment fluxes (src/river_flow.c:10-87, 113-183, 257-300, 333-518),
-// run by the river blocks of k_prep: rivflow 1-8, the upstream-node boundary
-// flux, and the bank-edge replacement values the bank elements take in
-// k_flux.  The aquifer flow to the downstream segment (rivflow 9) needs the
+// run by the river blocks of k_prep in three parts per segment, each its own
+// thread so that no thread carries the whole chain of gathers:
+//   part 0: the upstream-node boundary flux, the channel outflow (rivflow 1:
+//           to the downstream segment or through the outlet) and the channel
+//           leakage (rivflow 6);
+//   part 1: the left bank's fluxes (rivflow 2, 4, 7) and its bank-edge
+//           replacement values;
+//   part 2: the same for the right bank (rivflow 3, 5, 8).
+// The aquifer flow to the downstream segment (rivflow 9) needs the
 // downstream banks' records and is left to k_flux.
 template <bool B>
-__device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
+__device__ __forceinline__ void river_flux(const RhsArgs &a, int r, int part) {
   const int nr = a.nr;
   const RiverDev &R = a.r;
   const Ix ix = ix_river<B>(a, r);
@@ -480,35 +494,34 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
   const double *yg = ys + ix.nr;                    // yg[r']: river groundwater
   const double stage = clamp0(ys[r]);
   const double rgw = clamp0(yg[r]);
-  const int ord = R.ord[r];
-  const double coeff = R.coeff[r], zbed = R.zbed[r], length = R.length[r];
+  const double zbed = R.zbed[r], length = R.length[r];
   const double rzmin = R.zmin[r];
-  const double bcv = R.bc[r];
-  const int dcode = R.down[r];
-  const BankView bl = bank_view<B>(a, ix, R.left[r]);
-  const BankView br = bank_view<B>(a, ix, R.right[r]);
 
-  double upinit = 0.0, rf1;
-  if (dcode >= 0) {
-    const int d = dcode;
-    if (R.bctype[r] != 0) {
-      // BoundFluxRiver: src/river_flow.c:392-427
-      double flux = 0.0;
-      if (R.bctype[r] > 0) {
-        const double total_h = stage + zbed;
-        const double distance = 0.5 * length;
-        const double grad_h = (total_h - bcv) / distance;
-        const double perim = riv_perim(ord, stage, coeff);
-        const double crossa = riv_cs_area(ord, stage, coeff);
-        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
-        flux = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
-      } else {
-        flux = -bcv;
+  if (part == 0) {
+    const int ord = R.ord[r];
+    const double coeff = R.coeff[r];
+    const double bcv = R.bc[r];
+    const int dcode = R.down[r];
+    double upinit = 0.0, rf1;
+    if (dcode >= 0) {
+      const int d = dcode;
+      if (R.bctype[r] != 0) {
+        // BoundFluxRiver: src/river_flow.c:392-427
+        double flux = 0.0;
+        if (R.bctype[r] > 0) {
+          const double total_h = stage + zbed;
+          const double distance = 0.5 * length;
+          const double grad_h = (total_h - bcv) / distance;
+          const double perim = riv_perim(ord, stage, coeff);
+          const double crossa = riv_cs_area(ord, stage, coeff);
+          const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+          flux = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+        } else {
+          flux = -bcv;
+        }
+        upinit += flux;
       }
-      upinit += flux;
-    }
-    // ChanFlowRiverToRiver: src/river_flow.c:257-300
-    {
+      // ChanFlowRiverToRiver: src/river_flow.c:257-300
       const double dstage = clamp0(ys[d]);
       const double total_h = stage + zbed;
       const double perim = riv_perim(ord, stage, coeff);
@@ -525,79 +538,70 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r) {
       const double avg_crossa = 0.5 * (crossa + crossa_down);
       const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
       rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
+    } else {
+      // OutletFlux: src/river_flow.c:333-390
+      double q = 0.0;
+      switch (dcode) {
+        case DIRICHLET: {
+          const double total_h = stage + zbed;
+          const double distance = 0.5 * length;
+          const double grad_h = (total_h - bcv) / distance;
+          const double perim = riv_perim(ord, stage, coeff);
+          const double crossa = riv_cs_area(ord, stage, coeff);
+          const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+          q = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+          break;
+        }
+        case NEUMANN: q = -bcv; break;
+        case ZERO_DPTH_GRAD: {
+          const double distance = 0.5 * length;
+          const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
+          const double perim = riv_perim(ord, stage, coeff);
+          const double crossa = riv_cs_area(ord, stage, coeff);
+          q = sqrt(grad_h) * crossa * ((perim > 0.0) ? xpow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
+          break;
+        }
+        default: {  // CRIT_DPTH (other codes rejected at creation)
+          const double crossa = riv_cs_area(ord, stage, coeff);
+          q = crossa * sqrt(GRAV * stage);
+          break;
+        }
+      }
+      rf1 = q;
     }
-  } else {
-    // OutletFlux: src/river_flow.c:333-390
-    double q = 0.0;
-    switch (dcode) {
-      case DIRICHLET: {
-        const double total_h = stage + zbed;
-        const double distance = 0.5 * length;
-        const double grad_h = (total_h - bcv) / distance;
-        const double perim = riv_perim(ord, stage, coeff);
-        const double crossa = riv_cs_area(ord, stage, coeff);
-        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
-        q = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
-        break;
-      }
-      case NEUMANN: q = -bcv; break;
-      case ZERO_DPTH_GRAD: {
-        const double distance = 0.5 * length;
-        const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
-        const double perim = riv_perim(ord, stage, coeff);
-        const double crossa = riv_cs_area(ord, stage, coeff);
-        q = sqrt(grad_h) * crossa * ((perim > 0.0) ? xpow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
-        break;
-      }
-      default: {  // CRIT_DPTH (other codes rejected at creation)
-        const double crossa = riv_cs_area(ord, stage, coeff);
-        q = crossa * sqrt(GRAV * stage);
-        break;
-      }
+    // ChanLeak: src/river_flow.c:500-518
+    double rf6;
+    {
+      double diff_h;
+      if (zbed - (rgw + rzmin) > 0.0) diff_h = stage;
+      else diff_h = stage + zbed - (rgw + rzmin);
+      const double grad_h = diff_h / R.bedthick[r];
+      rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
     }
-    rf1 = q;
+    if (!B && a.closure) a.cl_r[r] = upinit + ((dcode < 0) ? rf1 : 0.0);   // leaves through river boundaries
+    R.rf[1 * nr + r] = rf1;
+    R.rf[6 * nr + r] = rf6;
+    R.upinit[r] = upinit;
+    return;
   }
 
-  // RiverToElem: src/river_flow.c:113-183
-  const double rzmax = R.zmax[r];
-  const double cwr = R.cwr[r];
-  const double rksath = R.ksath[r];
-  const double rf2 = ovl_elem_to_river(rzmax, bl.zmax, bl.surfh, zbed, stage, cwr, length);
-  const double rf3 = ovl_elem_to_river(rzmax, br.zmax, br.surfh, zbed, stage, cwr, length);
-  const double rf4 = chan_elem_to_river(stage, zbed, length, rksath, bl.gw, bl.gwz, bl.zmin, bl.effkh, R.dist_l[r]);
-  const double rf5 = chan_elem_to_river(stage, zbed, length, rksath, br.gw, br.gwz, br.zmin, br.effkh, R.dist_r[r]);
-  const double effk_riv = 0.5 * (bl.effkh + br.effkh);
-  const double rf7 = sub_elem_to_river(rgw, rzmin, zbed, length, bl.gw, bl.gwz, bl.zmin, bl.effkh, effk_riv,
-                                       R.dist_l[r]);
-  const double rf8 = sub_elem_to_river(rgw, rzmin, zbed, length, br.gw, br.gwz, br.zmin, br.effkh, effk_riv,
-                                       R.dist_r[r]);
-
-  // ChanLeak: src/river_flow.c:500-518
-  double rf6;
-  {
-    double diff_h;
-    if (zbed - (rgw + rzmin) > 0.0) diff_h = stage;
-    else diff_h = stage + zbed - (rgw + rzmin);
-    const double grad_h = diff_h / R.bedthick[r];
-    rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
-  }
-
-  if (!B && a.closure) a.cl_r[r] = upinit + ((dcode < 0) ? rf1 : 0.0);   // leaves through river boundaries
-  double *rf = R.rf;
-  rf[1 * nr + r] = rf1;
-  rf[2 * nr + r] = rf2;
-  rf[3 * nr + r] = rf3;
-  rf[4 * nr + r] = rf4;
-  rf[5 * nr + r] = rf5;
-  rf[6 * nr + r] = rf6;
-  rf[7 * nr + r] = rf7;
-  rf[8 * nr + r] = rf8;
-  R.upinit[r] = upinit;
+  // RiverToElem for one bank: src/river_flow.c:113-183
+  const int side = part - 1;
+  const int q = side ? R.right[r] : R.left[r];
+  const BankView bv = bank_view<B>(a, ix, q);
+  const double other = effkh_of<B>(a, ix, side ? R.left[r] : R.right[r]);
+  const double dist = side ? R.dist_r[r] : R.dist_l[r];
+  const double rf_ovl = ovl_elem_to_river(R.zmax[r], bv.zmax, bv.surfh, zbed, stage, R.cwr[r], length);
+  const double rf_chan = chan_elem_to_river(stage, zbed, length, R.ksath[r], bv.gw, bv.gwz, bv.zmin, bv.effkh, dist);
+  const double effk_riv = side ? 0.5 * (other + bv.effkh) : 0.5 * (bv.effkh + other);
+  const double rf_sub =
+      sub_elem_to_river(rgw, rzmin, zbed, length, bv.gw, bv.gwz, bv.zmin, bv.effkh, effk_riv, dist);
+  R.rf[(2 + side) * nr + r] = rf_ovl;
+  R.rf[(4 + side) * nr + r] = rf_chan;
+  R.rf[(7 + side) * nr + r] = rf_sub;
   // bank-edge replacement values (src/river_flow.c:159-182)
-  R.hid_ovl[2 * nr + 2 * r] = -rf2;
-  R.bank_sub[2 * r] = -(rf4 + rf7);
-  R.hid_ovl[2 * nr + 2 * r + 1] = -rf3;
-  R.bank_sub[2 * r + 1] = -(rf5 + rf8);
+  R.hid_ovl[2 * nr + 2 * r + side] = -rf_ovl;
+  R.bank_sub[2 * r + side] = -(rf_chan + rf_sub);
 }
 
 // River ODEs (src/ode.c:238-251), run by the river blocks of k_flux: the
@@ -847,7 +851,6 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   double **ed3[] = {&E.edge, &E.nabrdist, &E.nabr_x, &E.nabr_y, &E.ebc};
   for (auto p : ed3) ar.add(p, 3L * ne);
   ar.add(&E.nb, 3L * ne);
-  ar.add(&E.bc_type, 3L * ne);
   ar.add(&E.rec, ne);
   const long nr1 = std::max<long>(nr, 1);
   int32_t **ri[] = {&R.down, &R.left, &R.right, &R.bctype, &R.ord, &R.up_beg, &R.up_end};
@@ -980,7 +983,10 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
       const int v = m->nabr[3 * i + j];
       int32_t code;
       if (v > 0) code = c->iperm[off + v - 1];
-      else if (v == 0) code = NB_BOUND;
+      else if (v == 0) {
+        const int bt = m->bc_type[3 * i + j];
+        code = NB_BOUND + (bt == 0 ? 0 : (bt > 0 ? 1 : 2));
+      }
       else {
         const int r = -v - 1;
         const int side = (m->leftele[r] == i + 1) ? 0 : 1;
@@ -1039,10 +1045,7 @@ int ctx_load(pihm_ctx *c, int k, const pihm_mesh_desc *m) {
       PIHM_HIP_TRY(hipStreamSynchronize(st));
     }
   }
-  std::vector<int32_t> bt(cap.ne, 0);
   for (int j = 0; j < 3; j++) {
-    for (int lp = 0; lp < n; lp++) bt[lp] = m->bc_type[3L * order[lp] + j];
-    PIHM_HIP_TRY(hipMemcpyAsync(E.bc_type + (long)j * ne + off, bt.data(), n * 4, hipMemcpyHostToDevice, st));
     PIHM_HIP_TRY(hipMemcpyAsync(E.nb + (long)j * ne + off, nbk.data() + (long)j * cap.ne, cap.ne * 4,
                                 hipMemcpyHostToDevice, st));
     PIHM_HIP_TRY(hipStreamSynchronize(st));
@@ -1332,10 +1335,10 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
   if (c->nmem > 1) {
-    hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
     hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   } else {
-    hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
     hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     if (a.closure) {
       const int nb = std::max(1, std::min(kClosureBlocks, div_up(std::max(c->ne, c->nr), kBlock)));
@@ -1358,7 +1361,7 @@ int pihm_rhs_launch_io(pihm_ctx *c, const MemberIO *mio, int *mnan) {
   a.record = 0;
   const int nbe = div_up(c->ne, kBlock);
   const int nbr = div_up(c->nr, kBlock);
-  hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
   hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
@@ -1377,7 +1380,7 @@ int pihm_rhs_launch_compact(pihm_ctx *c, const MemberIO *mio, int *mnan, const i
   a.bm = bm;
   a.bm_mem = bm_mem;
   a.nact = nact;
-  if (nbe + nbr > 0) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  if (nbe + nbr > 0) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
   if (nbe + nbr > 0) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   PIHM_HIP_TRY(hipGetLastError());
   c->ncalls++;
@@ -1421,7 +1424,7 @@ extern "C" int pihm_ctx_get_river_edge_ovlflow(pihm_ctx *c, double *ovl3) {
     for (int j = 0; j < 3; j++) {
       const int code = c->nb_host[(long)j * ne + p];
       double v = 0.0;
-      if (code < 0 && code != NB_BOUND) v = hid[-code - 1];
+      if (nb_is_river(code)) v = hid[-code - 1];
       if (caller_elem(c, p) >= 0) ovl3[3L * caller_elem(c, p) + j] = v;
     }
   return PIHM_OK;
@@ -1435,7 +1438,7 @@ extern "C" int pihm_ctx_set_river_edge_ovlflow(pihm_ctx *c, const double *ovl3) 
   for (int p = 0; p < ne; p++)
     for (int j = 0; j < 3; j++) {
       const int code = c->nb_host[(long)j * ne + p];
-      if (code < 0 && code != NB_BOUND && caller_elem(c, p) >= 0) hid[-code - 1] = ovl3[3L * caller_elem(c, p) + j];
+      if (nb_is_river(code) && caller_elem(c, p) >= 0) hid[-code - 1] = ovl3[3L * caller_elem(c, p) + j];
     }
   PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.hid_ovl, hid.data(), 2L * nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1510,8 +1513,8 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
   float t[2] = {0, 0};
   for (int it = 0; it < iters; it++) {
     PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
-    if (c->nmem > 1) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    else hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    if (c->nmem > 1) hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    else hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
     if (c->nmem > 1) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     else hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
@@ -1578,10 +1581,10 @@ extern "C" int pihm_ctx_time_rhs(pihm_ctx *c, const double *y, double *ydot, int
   PIHM_HIP_TRY(hipEventRecord(ev[0], c->stream));
   for (int it = 0; it < iters; it++) {
     if (c->nmem > 1) {
-      hipLaunchKernelGGL(k_prep<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+      hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
       hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     } else {
-      hipLaunchKernelGGL(k_prep<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+      hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
       hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     }
   }
