@@ -30,10 +30,10 @@ BYTES_PER_ELEM = 344     # algorithmic bytes of one RHS per element
 BYTES_PER_SEG = 200      # ... per river segment
 # HBM bytes one RHS evaluation at the 200,000-triangle workload moves, from the rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (summed over
-# the RHS kernels, KiB per dispatch), each scaled by what the counter reads
-# for the calibration copy in the same run: FETCH_SIZE x 2.000 (262,157 KiB read back for
-# 524,288 KiB copied), WRITE_SIZE x 1.000.  (43,061.7 x 2 + 18,547.7) KiB.
-RHS_TRAFFIC_BYTES = (43061.7 * 2.0 + 18547.7) * 1024
+# k_prep and k_flux, KiB per dispatch), each scaled by what the counter reads for the
+# calibration copy in the same run: FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288
+# KiB copied), WRITE_SIZE x 1.000.  (44,752.5 x 2 + 17,737.5) KiB.
+RHS_TRAFFIC_BYTES = (44752.5 * 2.0 + 17737.5) * 1024
 
 
 def _cfg(t_end):
