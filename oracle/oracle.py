@@ -56,6 +56,9 @@ def lib():
         ("oracle_backend_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.POINTER(P.Forcing), C.c_int]),
         ("oracle_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
         ("oracle_backend_free", None, [C.c_void_p]),
+        ("oracle_meteo_interp", C.c_int, [C.POINTER(P.Meteo), C.c_double, dp]),
+        ("oracle_intcp_snow_et", None, [C.c_int, C.POINTER(P.LsmDesc), dp, C.c_int, dp, dp, C.c_double,
+                                        dp, dp, C.c_double, dp, dp, dp, dp, dp, dp, dp]),
     ]
     for name, res, args in protos:
         f = getattr(L, name)
@@ -208,3 +211,42 @@ class OracleRun:
             self.L.oracle_state_free(self.st)
         except Exception:
             pass
+
+
+def meteo_interp(meteo, t):
+    """IntrplForc for every station: (rc, met[5, nstation]); rc 1 when t is outside."""
+    from pihm_amd.lsm import lsm_structs
+    keep = []
+    ft = np.asarray(meteo[0])
+    dummy = {f: np.zeros(1) for f in _lsm_fields()}
+    dummy["station"] = np.zeros(1, np.int32)
+    _, m = lsm_structs(dummy, meteo, keep)
+    out = np.zeros((P.PIHM_MET_NVAR, ft.shape[0]))
+    rc = lib().oracle_meteo_interp(C.byref(m), float(t), _dptr(out))
+    return rc, out
+
+
+def _lsm_fields():
+    from pihm_amd.lsm import LSM_FIELDS
+    return LSM_FIELDS
+
+
+def intcp_snow_et(params, met, lai, z0, meltf, unsat, gw, stepsize, cmc, sneqv):
+    """IntcpSnowEt on the host for every element (caller order).  ``met`` is [5, nstation]
+    as meteo_interp returns it.  Returns (cmc, sneqv, fluxes) with fluxes keyed pcpdrp,
+    edir, ett, ec, drip; the inputs are not modified."""
+    from pihm_amd.lsm import lsm_structs
+    keep = []
+    met = np.ascontiguousarray(met, dtype=np.float64)
+    ns = met.shape[1]
+    fake_meteo = (np.zeros((ns, 2)), np.zeros((ns, 2, P.PIHM_MET_NVAR)))
+    d, _ = lsm_structs(params, fake_meteo, keep)
+    ne = len(params["station"])
+    f64 = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (ne,))).copy()
+    lai, z0, unsat, gw = f64(lai), f64(z0), f64(unsat), f64(gw)
+    cmc, sneqv = f64(cmc), f64(sneqv)
+    out = {k: np.zeros(ne) for k in ("pcpdrp", "edir", "ett", "ec", "drip")}
+    lib().oracle_intcp_snow_et(ne, C.byref(d), _dptr(met), ns, _dptr(lai), _dptr(z0), float(meltf),
+                               _dptr(unsat), _dptr(gw), float(stepsize), _dptr(cmc), _dptr(sneqv),
+                               *(_dptr(out[k]) for k in ("pcpdrp", "edir", "ett", "ec", "drip")))
+    return cmc, sneqv, out

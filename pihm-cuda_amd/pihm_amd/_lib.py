@@ -80,6 +80,24 @@ class RunStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LsmDesc(C.Structure):
+    """pihm_lsm_desc (include/pihm_lsm.h)."""
+
+    _fields_ = [(n, c_double_p) for n in (
+        "albedomin", "albedomax", "shdfac", "cmcfactr", "cfactr", "zlvl_wind", "rzd",
+        "rgl", "rsmin", "rsmax", "topt", "depth", "porosity", "smcmin", "smcwlt", "smcref", "zmax")] + [
+        ("station", c_int32_p), ("cal_edir", C.c_double), ("cal_ec", C.c_double), ("cal_ett", C.c_double)]
+
+
+PIHM_MET_NVAR = 5
+
+
+class Meteo(C.Structure):
+    """pihm_meteo (include/pihm_lsm.h)."""
+
+    _fields_ = [("nstation", C.c_int32), ("nrec", C.c_int32), ("ftime", c_double_p), ("data", c_double_p)]
+
+
 RhsFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
 ForcingFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)
 SummaryFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
@@ -214,6 +232,13 @@ _PROTOS = [
                                        C.c_double]),
     ("pihm_ens_advance", C.c_int, [C.c_void_p, C.c_long, C.POINTER(C.c_int)]),
     ("pihm_ens_member_status", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_lsm_create", C.c_int, [C.c_void_p, C.POINTER(LsmDesc), C.POINTER(Meteo), C.POINTER(C.c_void_p)]),
+    ("pihm_lsm_destroy", None, [C.c_void_p]),
+    ("pihm_lsm_set_veg", C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_double]),
+    ("pihm_lsm_set_state", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    ("pihm_lsm_get_state", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    ("pihm_lsm_step", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
+    ("pihm_lsm_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]),
 ]
 
 

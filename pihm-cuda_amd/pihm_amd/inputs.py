@@ -17,7 +17,9 @@ File layouts follow the reference readers (src/read_mesh.c, src/read_att.c,
 src/read_soil.c, src/read_river.c, src/read_calib.c, src/read_lc.c, src/read_para.c).
 Only what the right-hand side uses is read.
 """
+import calendar
 import os
+import time
 
 import numpy as np
 
@@ -112,6 +114,60 @@ def read_para(path):
                 stepsize=_keyword(rows, "MODEL_STEPSIZE", float), reltol=_keyword(rows, "RELTOL"),
                 abstol=_keyword(rows, "ABSTOL"), initstep=_keyword(rows, "INIT_SOLVER_STEP"),
                 lsm_step=_keyword(rows, "LSM_STEP", float))
+
+
+def str_time(s):
+    """"YYYY-MM-DD HH:MM" (UTC) to integral seconds since the epoch (StrTime,
+    src/time_func.c:30-58, which calls timegm)."""
+    return float(calendar.timegm(time.strptime(s, "%Y-%m-%d %H:%M")))
+
+
+def read_ts_blocks(path, key, nvar):
+    """Time-series blocks of a forcing file (ReadForc, src/read_forc.c:3-80; ReadLai,
+    src/read_lai.c): each block opens with a ``<key> <index> ...`` line and two header
+    lines, then one record per line, ``YYYY-MM-DD HH:MM v1 ... vn``.  Returns a list of
+    (header fields, ftime[nrec], data[nrec, nvar])."""
+    blocks = []
+    with open(path) as f:
+        lines = [ln.rstrip("\n") for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    i = 0
+    while i < len(lines):
+        head = lines[i].split()
+        if head[0] != key:
+            raise ValueError(f"{path}: expected {key} at '{lines[i]}'")
+        if int(head[1]) != len(blocks) + 1:
+            raise ValueError(f"{path}: {key} blocks out of order")
+        i += 3
+        ft, rows = [], []
+        while i < len(lines) and not lines[i].startswith(key):
+            tok = lines[i].split()
+            ft.append(str_time(tok[0] + " " + tok[1]))
+            rows.append([float(v) for v in tok[2:2 + nvar]])
+            i += 1
+        blocks.append((head, np.array(ft), np.array(rows)))
+    return blocks
+
+
+def read_meteo(path):
+    """Station records of a ``.meteo`` file: (ftime[nstation, nrec], data[nstation, nrec, 5],
+    zlvl_wind[nstation]) with PRCP, SFCTMP, RH, SFCSPD, SOLAR, the variables the simple
+    land-surface step reads (src/forcing.c:340-349).  Every station must hold the same
+    number of records (the device layout is one [nstation, nrec] table)."""
+    blocks = read_ts_blocks(path, "METEO_TS", 5)
+    if len({len(b[1]) for b in blocks}) != 1:
+        raise ValueError("stations with different record counts are not supported")
+    ftime = np.stack([b[1] for b in blocks])
+    data = np.stack([b[2] for b in blocks])
+    zlvl = np.array([float(b[0][3]) for b in blocks])
+    return ftime, data, zlvl
+
+
+def read_lai(path):
+    """LAI forcing of a ``.lai`` file: (ftime[nts, nrec], lai[nts, nrec])."""
+    blocks = read_ts_blocks(path, "LAI_TS", 1)
+    if len({len(b[1]) for b in blocks}) != 1:
+        raise ValueError("LAI series with different record counts are not supported")
+    return np.stack([b[1] for b in blocks]), np.stack([b[2][:, 0] for b in blocks])
 
 
 def read_project(directory, name, lc_table):

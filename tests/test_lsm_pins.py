@@ -1,0 +1,533 @@
+"""Pins of the oracle's land-surface step (oracle/lsm_oracle.c) and the meteorological
+record readers.
+
+The reference ships no tests or output files for its land-surface step, and its sources
+include pihm.h, whose CVODE headers need a generated configuration header, so nothing of
+it is built here.  The oracle's restatement of IntrplForc (src/forcing.c:482-530) and
+IntcpSnowEt (src/is_sm_et.c:5-226) is pinned instead by what the mathematics and the
+model's own bookkeeping fix:
+
+* interpolation: numpy's piecewise-linear interpolation on the reference's own example
+  station records, exact hits at record times, the out-of-range exit;
+* the snow bucket's and the canopy bucket's water balances (precipitation = throughfall +
+  canopy evaporation + storage change), which every branch but one closes exactly;
+* closed forms of the special cases: bare canopy (LAI 0), no available energy and
+  saturated air (zero potential evaporation), the transpiration shut-off, the
+  soil-evaporation and canopy-evaporation ratio, and the reference's rescaling when the
+  canopy store would go negative (src/is_sm_et.c:204-216, ec rescaled first and then used
+  in drip's denominator);
+* calibration multipliers scale their flux and nothing else; station indexing.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+from pihm_amd import inputs
+from pihm_amd.lsm import synthetic_meteo, synthetic_params
+
+DT = 900.0
+
+
+def _met(prcp=2e-4, sfctmp=283.15, rh=60.0, spd=3.0, solar=400.0, ns=1):
+    """Interpolated station values [5, ns] as meteo_interp returns them."""
+    m = np.array([prcp, sfctmp, rh, spd, solar], dtype=np.float64)
+    return np.repeat(m[:, None], ns, axis=1)
+
+
+def _params(ne, seed=0, **over):
+    p = synthetic_params(ne, 1, seed=seed)
+    p["station"] = np.zeros(ne, np.int32)
+    for k, v in over.items():
+        p[k] = np.broadcast_to(np.asarray(v, np.float64), (ne,)).copy() if k not in (
+            "cal_edir", "cal_ec", "cal_ett") else v
+    return p
+
+
+def _run(p, met, lai=2.0, z0=0.1, meltf=2e-8, unsat=0.5, gw=None, dt=DT, cmc=0.0, sneqv=0.0):
+    if gw is None:
+        gw = 0.5 * p["depth"]
+    return O.intcp_snow_et(p, met, lai, z0, meltf, unsat, gw, dt, cmc, sneqv)
+
+
+# ------------------------------------------------------------------ interpolation
+
+def test_read_meteo_fixture():
+    ft, data, zlvl = inputs.read_meteo(os.path.join(GOLDEN, "example.meteo"))
+    assert ft.shape == (1, 361) and data.shape == (1, 361, 5)
+    assert zlvl.tolist() == [10.0]
+    assert ft[0, 0] == inputs.str_time("2008-12-31 00:00") == 1230681600.0
+    assert np.all(np.diff(ft[0]) == 3600.0)
+    # 2009-01-01 00:00 record (line 8788 of the reference file)
+    i = int((inputs.str_time("2009-01-01 00:00") - ft[0, 0]) // 3600)
+    assert data[0, i].tolist() == [0.0, 265.93, 54.45, 5.52, 0.0]
+
+
+def test_read_lai_fixture():
+    ft, lai = inputs.read_lai(os.path.join(GOLDEN, "example.lai"))
+    assert ft.shape == lai.shape and ft.shape[0] == 1
+    assert ft[0, 1] == inputs.str_time("2008-01-01 00:00") and lai[0, 1] == 0.5
+
+
+def test_interp_matches_numpy_on_example_records():
+    ft, data, _ = inputs.read_meteo(os.path.join(GOLDEN, "example.meteo"))
+    rng = np.random.default_rng(0)
+    times = np.concatenate([ft[0], ft[0, :-1] + 1800.0, rng.integers(ft[0, 0], ft[0, -1], 200).astype(float)])
+    for t in times:
+        rc, met = O.meteo_interp((ft, data), t)
+        assert rc == 0
+        j = min(max(np.searchsorted(ft[0], t), 1), ft.shape[1] - 1)
+        for v in range(5):
+            want = np.interp(t, ft[0], data[0, :, v])
+            # both round each product once: the bound is relative to the interval's ends
+            ends = max(abs(data[0, j - 1, v]), abs(data[0, j, v]))
+            assert abs(met[v, 0] - want) <= 4e-16 * ends, (t, v, met[v, 0], want)
+
+
+def test_interp_record_times_and_stations():
+    ft, data = synthetic_meteo(4, 30, t0=1000.0, dt=600.0, seed=3)
+    # irregular spacing, per station
+    off = np.cumsum(np.random.default_rng(1).integers(0, 300, ft.shape), axis=1)
+    ft = ft + off - off[:, :1]
+    ft[:, -1] = ft[:, -1].max()           # one common first and last record time
+    for j in (0, 1, 7, 29):
+        rc, met = O.meteo_interp((ft, data), ft[2, j])
+        assert rc == 0
+        assert np.allclose(met[:, 2], data[2, j], rtol=2e-16, atol=0)
+    t = 0.5 * (ft[:, 10].max() + ft[:, 11].min())
+    rc, met = O.meteo_interp((ft, data), t)
+    for k in range(4):
+        for v in range(5):
+            assert abs(met[v, k] - np.interp(t, ft[k], data[k, :, v])) <= 1e-15 * abs(data[k, :, v]).max()
+
+
+def test_interp_out_of_range_exits():
+    ft, data = synthetic_meteo(2, 10, t0=0.0, dt=3600.0)
+    assert O.meteo_interp((ft, data), -1.0)[0] == 1
+    assert O.meteo_interp((ft, data), ft[0, -1] + 1.0)[0] == 1
+    # the end records themselves, to the rounding of (dt * v) / dt
+    rc, met = O.meteo_interp((ft, data), ft[0, -1])
+    assert rc == 0 and np.allclose(met[:, 1], data[1, -1], rtol=2.3e-16, atol=0)
+    rc, met = O.meteo_interp((ft, data), 0.0)
+    assert rc == 0 and np.allclose(met[:, 0], data[0, 0], rtol=2.3e-16, atol=0)
+
+
+# ------------------------------------------------------------------ snow bucket
+
+@pytest.mark.parametrize("tc", [-10.0, -3.0, -1.0, 0.5, 1.0, 4.0])
+def test_snow_partition_and_melt(tc):
+    ne = 64
+    rng = np.random.default_rng(5)
+    p = _params(ne)
+    sneqv0 = rng.uniform(0.0, 2e-3, ne)
+    sneqv0[:8] = 0.0
+    prcp_mm = 0.3  # kg m-2 s-1
+    meltf = 5e-7
+    cmc, sneqv, f = _run(p, _met(prcp=prcp_mm, sfctmp=273.15 + tc), meltf=meltf, sneqv=sneqv0)
+    prcp = prcp_mm / 1000.0
+    sfctmp = (273.15 + tc) - 273.15
+    fs = 1.0 if sfctmp < -3.0 else (0.0 if sfctmp > 1.0 else (1.0 - sfctmp) / 4.0)
+    pot = sneqv0 + fs * prcp * DT
+    melt = np.minimum(max(sfctmp, 0.0) * meltf, pot / DT)
+    np.testing.assert_allclose(sneqv, np.maximum(pot - melt * DT, 0.0), rtol=1e-14, atol=1e-18)
+    # snow bucket balance: d(sneqv)/dt = snowfall - melt
+    np.testing.assert_allclose((sneqv - sneqv0) / DT, fs * prcp - melt, rtol=1e-12, atol=1e-18)
+
+
+# ------------------------------------------------------------------ canopy bucket and energy
+
+def _random_case(ne, seed):
+    rng = np.random.default_rng(seed)
+    p = synthetic_params(ne, 3, seed=seed)
+    ft, data = synthetic_meteo(3, 48, seed=seed)
+    rc, met = O.meteo_interp((ft, data), 3600.0 * rng.uniform(0, 47))
+    assert rc == 0
+    intcp = p["cmcfactr"] * 2.0 * p["shdfac"]
+    st = dict(lai=rng.uniform(0.0, 4.0, ne), z0=rng.uniform(0.01, 0.5, ne),
+              unsat=rng.uniform(-0.05, 1.5, ne), gw=p["depth"] * rng.uniform(0.1, 1.0, ne),
+              cmc=intcp * rng.uniform(0.0, 1.2, ne), sneqv=rng.uniform(0.0, 0.01, ne))
+    st["lai"][:10] = 0.0
+    return p, met, st
+
+
+def test_water_balance_closes():
+    ne = 4000
+    for seed in range(4):
+        p, met, st = _random_case(ne, seed)
+        # daytime radiation and rain everywhere: every branch of the canopy store
+        met[4] = np.maximum(met[4], 300.0)
+        met[0] = np.where(np.arange(3) == 0, 0.0, 5e-4)
+        cmc, sneqv, f = _run(p, met, meltf=3e-8, dt=DT, **st)
+        prcp = met[0, p["station"]] / 1000.0
+        lhs = prcp
+        rhs = f["pcpdrp"] + f["ec"] + (cmc - st["cmc"]) / DT + (sneqv - st["sneqv"]) / DT
+        closed = cmc > 0.0          # every branch except the negative-store rescale
+        assert closed.sum() > ne // 2
+        scale = prcp + f["pcpdrp"] + f["ec"] + (st["cmc"] + cmc + st["sneqv"] + sneqv) / DT
+        assert np.all(np.abs(lhs - rhs)[closed] <= 1e-12 * scale[closed])
+        assert np.all(cmc <= p["cmcfactr"] * st["lai"] * p["shdfac"] * (1 + 1e-15))
+        for k in ("pcpdrp", "edir", "ett", "ec", "drip"):
+            assert np.all(f[k] >= 0.0), k
+
+
+def test_bare_canopy_closed_form():
+    ne = 200
+    rng = np.random.default_rng(2)
+    p = _params(ne, seed=2)
+    cmc0 = rng.uniform(0.0, 1e-4, ne)
+    for tc in (-5.0, 0.0, 5.0):
+        cmc, sneqv, f = _run(p, _met(prcp=0.4, sfctmp=273.15 + tc), lai=0.0, cmc=cmc0, meltf=1e-8)
+        prcp = 0.4 / 1000.0
+        fs = 1.0 if tc < -3 else (0.0 if tc > 1 else (1.0 - tc) / 4.0)
+        melt = max(tc, 0.0) * 1e-8 * np.ones(ne)
+        melt = np.minimum(melt, fs * prcp)   # no snow on the ground at the start
+        assert np.all(f["ec"] == 0.0) and np.all(f["ett"] == 0.0) and np.all(cmc == 0.0)
+        np.testing.assert_allclose(f["drip"], cmc0 / DT + (1 - fs) * prcp * p["shdfac"], rtol=1e-14)
+        np.testing.assert_allclose(f["pcpdrp"], (1 - fs) * prcp + cmc0 / DT + melt, rtol=1e-13)
+
+
+def test_no_energy_no_evaporation():
+    ne = 300
+    p, met, st = _random_case(ne, 9)
+    met[4] = 0.0     # no radiation
+    met[2] = 100.0   # saturated air
+    cmc, sneqv, f = _run(p, met, **st)
+    for k in ("edir", "ec", "ett"):
+        assert np.all(f[k] == 0.0), k
+    # negative radiation records are clamped to zero (src/forcing.c:345)
+    met2 = met.copy()
+    met2[4] = -25.0
+    _, _, g = _run(p, met2, **st)
+    for k in f:
+        assert np.array_equal(f[k], g[k]), k
+    # with energy and a vapour deficit every flux is positive where the branches allow
+    met[4], met[2] = 500.0, 50.0
+    cmc_h, _, h = _run(p, met, **st)
+    assert np.all(h["edir"] > 0.0)
+    # canopy evaporation from a store that is not emptied (and rescaled) in the step
+    veg = (st["lai"] > 0.0) & (st["cmc"] > 0) & (cmc_h > 0)
+    assert veg.sum() > ne // 2 and np.all(h["ec"][veg] > 0.0)
+
+
+def test_transpiration_shutoff():
+    ne = 100
+    p = _params(ne, seed=4, rzd=0.5, depth=4.0)
+    gw = np.where(np.arange(ne) % 2 == 0, 1.0, 3.8)   # below / inside the root zone
+    unsat = np.where(np.arange(ne) % 4 < 2, 0.0, 0.3)
+    _, _, f = _run(p, _met(solar=600.0, rh=40.0), gw=gw, unsat=unsat, cmc=1e-5)
+    off = (gw < 4.0 - 0.5) & (unsat <= 0.0)
+    assert np.all(f["ett"][off] == 0.0)
+    assert np.all(f["ett"][~off] > 0.0)
+
+
+def test_soil_and_canopy_evaporation_ratio():
+    ne = 400
+    rng = np.random.default_rng(6)
+    p = _params(ne, seed=6, cal_edir=0.7, cal_ec=1.3)
+    depth, rzd = p["depth"], p["rzd"]
+    wet_root = rng.random(ne) < 0.5
+    gw = np.where(wet_root, depth - 0.5 * rzd, 0.2 * depth)
+    unsat = rng.uniform(0.0, 1.0, ne) * (depth - gw)
+    lai = 2.5
+    intcp = p["cmcfactr"] * lai * p["shdfac"]
+    cmc0 = intcp * rng.uniform(0.05, 1.0, ne)
+    _, _, f = _run(p, _met(solar=700.0, rh=35.0), lai=lai, gw=gw, unsat=unsat, cmc=cmc0)
+    x = unsat / (depth - gw)
+    # the reference's saturation curve uses 3.14 for pi (src/is_sm_et.c:119)
+    satn = np.where(wet_root, 1.0, np.clip(0.5 * (1.0 - np.cos(3.14 * np.clip(x, 0.0, 1.0))), 0.0, 1.0))
+    betas = np.clip((satn * p["porosity"] + p["smcmin"] - p["smcwlt"]) / (p["smcref"] - p["smcwlt"]), 1e-4, 1.0)
+    wet = np.minimum(cmc0, intcp) / intcp
+    want = (1 - p["shdfac"]) * betas ** 2 * 0.7 / (p["shdfac"] * wet ** p["cfactr"] * 1.3)
+    # ec is only rescaled when the canopy store would go negative: not with these stores
+    np.testing.assert_allclose(f["edir"] / f["ec"], want, rtol=2e-14)
+
+
+def test_calibration_scales_only_its_flux():
+    ne = 500
+    p, met, st = _random_case(ne, 12)
+    _, _, a = _run(p, met, **st)
+    p2 = dict(p, cal_edir=2.0, cal_ett=0.5)
+    _, _, b = _run(p2, met, **st)
+    assert np.array_equal(b["edir"], 2.0 * a["edir"])
+    assert np.array_equal(b["ett"], 0.5 * a["ett"])
+    for k in ("pcpdrp", "ec", "drip"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_negative_store_rescale():
+    """isval < 0 (src/is_sm_et.c:204-216): with cmc set to 0 first,
+    ec' = ec / (ec + drip) * W and drip' = drip / (ec' + drip) * W, W the rain caught in
+    the step.  ec and drip before the rescale do not depend on the rain rate: a run with a
+    full store gives ec, and a run whose rain keeps the store part-full gives drip."""
+    ne = 50
+    rng = np.random.default_rng(8)
+    p = _params(ne, seed=8, shdfac=0.9, cfactr=0.5)
+    p["station"] = np.arange(ne, dtype=np.int32)       # one station per element
+    lai = 3.0
+    intcp = p["cmcfactr"] * lai * p["shdfac"]
+    cmc0 = intcp * rng.uniform(0.02, 0.2, ne)
+    dt = 7200.0
+    kw = dict(lai=lai, cmc=cmc0, dt=dt, gw=0.5 * p["depth"], unsat=1.0)
+    met = _met(prcp=1.0, sfctmp=300.0, rh=10.0, solar=900.0, spd=8.0, ns=ne)   # rain only
+    cmc_f, _, ff = _run(p, met, **kw)
+    assert np.all(cmc_f == intcp)                       # full store: ec as computed
+    ec0 = ff["ec"]
+    met[0] = 1000.0 * (ec0 * dt + 0.5 * intcp) / (p["shdfac"] * dt)
+    cmc_m, _, fm = _run(p, met, **kw)
+    assert np.all(cmc_m > 0.0) and np.all(cmc_m < intcp)   # part-full store: drip as computed
+    assert np.array_equal(fm["ec"], ec0)
+    drip0 = fm["drip"]
+    met[0] = 1e-6
+    cmc_d, _, fd = _run(p, met, **kw)
+    w = 1e-6 / 1000.0 * p["shdfac"] * dt
+    neg = cmc0 + w - ec0 * dt - drip0 * dt < 0.0
+    assert neg.sum() > ne // 2
+    ec1 = ec0 / (ec0 + drip0) * w
+    drip1 = drip0 / (ec1 + drip0) * w
+    assert np.all(cmc_d[neg] == 0.0)
+    np.testing.assert_allclose(fd["ec"][neg], ec1[neg], rtol=1e-14)
+    np.testing.assert_allclose(fd["drip"][neg], drip1[neg], rtol=1e-14)
+    np.testing.assert_allclose(fd["ec"][~neg], ec0[~neg], rtol=1e-15)
+
+
+def test_station_indexing():
+    ne = 90
+    p = synthetic_params(ne, 3, seed=21)
+    ft, data = synthetic_meteo(3, 24, seed=22)
+    rc, met = O.meteo_interp((ft, data), 11.5 * 3600.0)
+    st = dict(lai=1.5, z0=0.1, unsat=0.4, gw=0.5 * p["depth"], cmc=1e-4, sneqv=1e-3)
+    cmc, sneqv, f = _run(p, met, **st)
+    for k in range(3):
+        q = dict(p, station=np.zeros(ne, np.int32))
+        cmc_k, sneqv_k, g = _run(q, met[:, k:k + 1].copy(), **st)
+        on = p["station"] == k
+        assert on.any()
+        assert np.array_equal(cmc[on], cmc_k[on]) and np.array_equal(sneqv[on], sneqv_k[on])
+        for name in f:
+            assert np.array_equal(f[name][on], g[name][on]), name
+
+
+# ------------------------------------------------------------------ potential evaporation
+
+def _etp_from_ec(p, f, wet, cal_ec=1.0):
+    """Potential evaporation recovered from canopy evaporation, ec = shdfac wet^cfactr etp."""
+    return f["ec"] / (p["shdfac"] * wet ** p["cfactr"] * cal_ec)
+
+
+def test_radiation_enters_through_albedo():
+    """Net radiation is soldn (1 - albedo) and enters etp linearly: the soil-evaporation
+    increment between two radiation levels scales with 1 - albedo."""
+    ne = 40
+    rng = np.random.default_rng(31)
+    p = _params(ne, seed=31)
+    a = rng.uniform(0.05, 0.6, ne)
+    p["albedomin"], p["albedomax"] = a - 0.02, a + 0.02
+    for k in ("shdfac", "depth", "rzd", "zmax", "porosity", "smcmin", "smcwlt", "smcref"):
+        p[k] = np.full(ne, p[k][0])
+    kw = dict(lai=0.0, gw=0.5 * p["depth"], unsat=0.6)
+    e = [_run(p, _met(solar=s, rh=70.0), **kw)[2]["edir"] for s in (0.0, 300.0, 600.0)]
+    d1, d2 = e[1] - e[0], e[2] - e[0]
+    np.testing.assert_allclose(d2, 2.0 * d1, rtol=1e-9)            # linear in radiation
+    np.testing.assert_allclose(d1 / d1[0], (1.0 - a) / (1.0 - a[0]), rtol=1e-9)
+
+
+def test_vapour_deficit_and_wind_laws():
+    """Without radiation etp is the aerodynamic term, proportional to the humidity deficit
+    qvsat - qv = qvsat (1 - rh); at large wind (small aerodynamic resistance ra ~ 1/wind)
+    it grows like 1/ra."""
+    ne = 8
+    p = _params(ne, seed=33)
+    for k in p:
+        if k not in ("cal_edir", "cal_ec", "cal_ett", "station"):
+            p[k] = np.full(ne, p[k][0])
+    kw = dict(lai=0.0, gw=0.5 * p["depth"], unsat=0.6)
+    rh = np.linspace(10.0, 95.0, ne)
+    p["station"] = np.arange(ne, dtype=np.int32)
+    met = _met(solar=0.0, ns=ne)
+    met[2] = rh
+    e = _run(p, met, **kw)[2]["edir"]
+    assert np.all(e > 0.0)
+    np.testing.assert_allclose(e / e[0], (1.0 - rh / 100.0) / (1.0 - rh[0] / 100.0), rtol=1e-12)
+    met = _met(solar=0.0, rh=50.0, ns=2)
+    met[3] = [1e7, 2e7]
+    p2 = _params(2, seed=33)
+    p2["station"] = np.arange(2, dtype=np.int32)
+    for k in p2:
+        if k not in ("cal_edir", "cal_ec", "cal_ett", "station"):
+            p2[k] = np.full(2, p2[k][0])
+    e2 = _run(p2, met, lai=0.0, gw=0.5 * p2["depth"], unsat=0.6)[2]["edir"]
+    np.testing.assert_allclose(e2[1] / e2[0], 2.0, rtol=1e-4)
+
+
+def test_full_canopy_store_no_transpiration():
+    """A full canopy store (wet fraction 1) evaporates at shdfac * etp and shuts
+    transpiration off: ett = shdfac pc (1 - wet^cfactr) etp = 0."""
+    ne = 60
+    p = _params(ne, seed=35)
+    lai = 2.0
+    intcp = p["cmcfactr"] * lai * p["shdfac"]
+    _, _, f = _run(p, _met(solar=600.0, rh=40.0), lai=lai, cmc=intcp * 1.5, unsat=0.8)
+    assert np.all(f["ett"] == 0.0) and np.all(f["ec"] > 0.0)
+    # and the same etp as the soil-evaporation route: edir = (1 - shdfac) betas^2 etp
+    etp = f["ec"] / p["shdfac"]
+    b2 = f["edir"] / ((1 - p["shdfac"]) * etp)
+    assert np.all((b2 > 0.0) & (b2 <= 1.0 + 1e-12))
+
+
+def test_stomatal_resistance_inverse_in_soil_stress():
+    """Canopy resistance rs = rsmin alphar / (betas lai etas gammas) (a Jarvis form):
+    between elements that differ only in soil water, 1/pc - 1 = (rs/ra) / (1 + delta/gamma)
+    scales as 1/betas.  pc comes from ett, betas from edir and etp from ec."""
+    ne = 30
+    p = _params(ne, seed=37, rsmin=60.0, rsmax=1e9)     # rs never at its cap
+    for k in p:
+        if k not in ("cal_edir", "cal_ec", "cal_ett", "station"):
+            p[k] = np.full(ne, p[k][0])
+    depth, rzd = p["depth"], p["rzd"]
+    gw = np.full(ne, 0.1 * depth[0])
+    x = np.linspace(0.2, 0.8, ne)
+    unsat = x * (depth - gw)
+    lai = 2.0
+    intcp = p["cmcfactr"] * lai * p["shdfac"]
+    cmc0 = 0.4 * intcp
+    _, _, f = _run(p, _met(solar=500.0, rh=45.0), lai=lai, cmc=cmc0, gw=gw, unsat=unsat)
+    wet = 0.4
+    etp = _etp_from_ec(p, f, wet)
+    betas = np.sqrt(f["edir"] / ((1 - p["shdfac"]) * etp))
+    free = (betas > 1.01e-4) & (betas < 0.999)       # stress factor not at a clamp
+    assert free.sum() >= 8
+    pc = f["ett"] / (p["shdfac"] * (1.0 - wet ** p["cfactr"]) * etp)
+    inv = ((1.0 / pc - 1.0) * betas)[free]
+    np.testing.assert_allclose(inv / inv[0], 1.0, rtol=1e-10)
+
+
+# Physical constants of the reference (src/include/pihm_const.h:9-15)
+LV, CP, RD, RV, SIGMA = 2.501e6, 1004.0, 287.04, 461.5, 5.67e-8
+
+
+def _bolton_es(tc):
+    """Saturation vapour pressure [Pa] over water, Bolton (1980): 611.2 exp(17.67 T / (T + 243.5))."""
+    return 611.2 * np.exp(17.67 * tc / (tc + 243.5))
+
+
+def _uniform_params(n, seed, **over):
+    p = _params(n, seed=seed, **over)
+    for k in p:
+        if k not in ("cal_edir", "cal_ec", "cal_ett", "station"):
+            p[k] = np.full(n, p[k][0])
+    p["station"] = np.arange(n, dtype=np.int32)
+    return p
+
+
+def test_penman_energy_terms():
+    """etp = (Rn delta + gamma * 1.2 Lv (qsat - q) / ra) / (1000 Lv (delta + gamma)), read off
+    the outputs term by term:
+    * the radiation slope gives x = delta / (delta + gamma);
+    * doubling the wind halves ra (ra = ln(z/z0) ln(10 z/z0) / (0.16 u), the neutral log
+      law with k = 0.4 and a heat roughness of z0/10), so two winds separate
+      gamma = 1 + K ra into K ra;
+    * K ra is the radiative correction 4 eps sigma T^3 ra / (rho cp), eps = 0.7,
+      rho = p / (Rd T), p from the standard-atmosphere barometric formula at zmax;
+    * delta = 0.622 Lv^2 qsat / (Rv cp T^2), qsat = 0.622 es / p with Bolton's es (the
+      reference's slope carries 0.622 once more than the Clausius-Clapeyron dqsat/dT)."""
+    temps = (268.0, 283.0, 298.0)
+    winds = (2.0, 4.0)
+    sols = (200.0, 400.0)
+    cases = [(T, w, s) for T in temps for w in winds for s in sols]
+    n = len(cases)
+    p = _uniform_params(n, seed=41)
+    met = _met(rh=50.0, ns=n)
+    for i, (T, w, s) in enumerate(cases):
+        met[1, i], met[3, i], met[4, i] = T, w, s
+    lai, wet = 2.0, 0.5
+    intcp = p["cmcfactr"][0] * lai * p["shdfac"][0]
+    z0 = 0.1
+    _, _, f = _run(p, met, lai=lai, z0=z0, cmc=wet * intcp, gw=0.5 * p["depth"], unsat=0.6, dt=60.0)
+    etp = _etp_from_ec(p, f, wet)
+    albedo = 0.5 * (p["albedomin"][0] + p["albedomax"][0])
+    zmax = p["zmax"][0]
+    pres = 101.325e3 * ((293.0 - 0.0065 * zmax) / 293.0) ** 5.26
+    ra2 = np.log(10.0 / z0) * np.log(100.0 / z0) / (0.16 * 2.0)
+    idx = {c: i for i, c in enumerate(cases)}
+    for T in temps:
+        r = {}
+        for w in winds:
+            e1, e2 = etp[idx[(T, w, 200.0)]], etp[idx[(T, w, 400.0)]]
+            x = 1000.0 * LV * (e2 - e1) / (200.0 * (1.0 - albedo))
+            r[w] = x / (1.0 - x)                      # delta / gamma
+        q = r[2.0] / r[4.0]                           # gamma(4) / gamma(2)
+        kra = (1.0 - q) / (q - 0.5)                   # K ra at u = 2
+        rho = pres / (RD * T)
+        np.testing.assert_allclose(kra, 4 * 0.7 * SIGMA * T ** 3 * ra2 / (rho * CP), rtol=1e-6)
+        delta = r[2.0] * (kra + 1.0)
+        qsat = 0.622 * _bolton_es(T - 273.15) / pres
+        np.testing.assert_allclose(delta, 0.622 * LV ** 2 * qsat / (RV * CP * T ** 2), rtol=1e-6)
+
+
+def test_canopy_resistance_jarvis_factors():
+    """ett = shdfac pc (1 - wet^cfactr) etp with pc = (1 + d/g) / (1 + rs/ra + d/g): the
+    canopy resistance read off the outputs is rs = rsmin alphar / (betas lai etas gammas),
+    with the Noah/Jarvis factors 1/alphar = (fr + rsmin/rsmax) / (1 + fr),
+    fr = 1.1 Rn / (rgl lai), etas = 1 - 0.0016 (topt - T)^2 and the reference's
+    vapour-deficit factor gammas = 1 / (1 + 0.00025 (es - e)) [Pa].  rs does not depend on
+    the wind, so rs/ra doubles with it."""
+    temps = (278.0, 290.0, 301.0)
+    cases = [(T, w, s) for T in temps for w in (2.0, 4.0) for s in (200.0, 400.0)]
+    n = len(cases)
+    p = _uniform_params(n, seed=43, rsmin=80.0, rsmax=1e9)
+    rh = 0.45
+    met = _met(rh=100.0 * rh, ns=n)
+    for i, (T, w, s) in enumerate(cases):
+        met[1, i], met[3, i], met[4, i] = T, w, s
+    lai, wet, z0 = 2.0, 0.3, 0.1
+    intcp = p["cmcfactr"][0] * lai * p["shdfac"][0]
+    gw = 0.5 * p["depth"]
+    # soil water for a stress factor betas = 1/2 (saturation curve 0.5 (1 - cos(3.14 x)))
+    satn = (0.5 * (p["smcref"] - p["smcwlt"]) - p["smcmin"] + p["smcwlt"]) / p["porosity"]
+    unsat = np.arccos(1.0 - 2.0 * satn) / 3.14 * (p["depth"] - gw)
+    _, _, f = _run(p, met, lai=lai, z0=z0, cmc=wet * intcp, gw=gw, unsat=unsat, dt=60.0)
+    sh, cf = p["shdfac"][0], p["cfactr"][0]
+    etp = _etp_from_ec(p, f, wet)
+    albedo = 0.5 * (p["albedomin"][0] + p["albedomax"][0])
+    betas = np.sqrt(f["edir"] / ((1.0 - sh) * etp))
+    assert np.all((betas > 1.01e-4) & (betas < 0.999))
+    pc = f["ett"] / (sh * (1.0 - wet ** cf) * etp)
+    idx = {c: i for i, c in enumerate(cases)}
+    for T in temps:
+        rs_ra = {}
+        for w in (2.0, 4.0):
+            e1, e2 = etp[idx[(T, w, 200.0)]], etp[idx[(T, w, 400.0)]]
+            x = 1000.0 * LV * (e2 - e1) / (200.0 * (1.0 - albedo))
+            dg = x / (1.0 - x)
+            i = idx[(T, w, 400.0)]
+            rs_ra[w] = (1.0 / pc[i] - 1.0) * (1.0 + dg)
+        np.testing.assert_allclose(rs_ra[4.0] / rs_ra[2.0], 2.0, rtol=1e-6)
+        ra = np.log(10.0 / z0) * np.log(100.0 / z0) / (0.16 * 2.0)
+        rs = rs_ra[2.0] * ra
+        tc = T - 273.15
+        rn = 400.0 * (1.0 - albedo)
+        fr = 1.1 * rn / (p["rgl"][0] * lai)
+        alphar = (1.0 + fr) / (fr + p["rsmin"][0] / p["rsmax"][0])
+        etas = 1.0 - 0.0016 * (p["topt"][0] - 273.15 - tc) ** 2
+        es = _bolton_es(tc)
+        gammas = 1.0 / (1.0 + 0.00025 * (es - es * rh))
+        want = p["rsmin"][0] * alphar / (betas[idx[(T, 2.0, 400.0)]] * lai * etas * gammas)
+        np.testing.assert_allclose(rs, want, rtol=1e-6)
+
+
+def test_drip_is_rutter_exponential():
+    """Canopy drainage (Rutter and Morton 1977): D = D0 Cmax exp(3.89 C / Cmax) with
+    D0 = 3.91e-5 m/min = 6.52e-7 m/s, read where the store stays part-full."""
+    ne = 20
+    p = _uniform_params(ne, seed=45)
+    lai = 3.0
+    intcp = p["cmcfactr"][0] * lai * p["shdfac"][0]
+    c = intcp * np.linspace(0.05, 0.9, ne)
+    met = _met(prcp=0.0, solar=0.0, rh=100.0, ns=ne)     # no rain, no evaporation
+    dt = 1.0
+    cmc, _, f = _run(p, met, lai=lai, cmc=c, dt=dt)
+    assert np.all(f["ec"] == 0.0)
+    np.testing.assert_allclose(f["drip"], 6.52e-7 * intcp * np.exp(3.89 * c / intcp), rtol=1e-14)
+    np.testing.assert_allclose(cmc, c - f["drip"] * dt, rtol=1e-14)
