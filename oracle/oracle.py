@@ -57,6 +57,12 @@ def lib():
         ("oracle_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
         ("oracle_backend_free", None, [C.c_void_p]),
         ("oracle_meteo_interp", C.c_int, [C.POINTER(P.Meteo), C.c_double, dp]),
+        ("oracle_lsm_backend_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.POINTER(P.LsmDesc),
+                                                C.POINTER(P.Meteo), dp, dp, C.c_double, dp, dp, C.c_double]),
+        ("oracle_lsm_backend_start", C.c_int, [C.c_void_p, C.c_void_p]),
+        ("oracle_lsm_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
+        ("oracle_lsm_backend_state", None, [C.c_void_p, dp, dp]),
+        ("oracle_lsm_backend_free", None, [C.c_void_p]),
         ("oracle_intcp_snow_et", None, [C.c_int, C.POINTER(P.LsmDesc), dp, C.c_int, dp, dp, C.c_double,
                                         dp, dp, C.c_double, dp, dp, dp, dp, dp, dp, dp]),
     ]
@@ -250,3 +256,48 @@ def intcp_snow_et(params, met, lai, z0, meltf, unsat, gw, stepsize, cmc, sneqv):
                                _dptr(unsat), _dptr(gw), float(stepsize), _dptr(cmc), _dptr(sneqv),
                                *(_dptr(out[k]) for k in ("pcpdrp", "edir", "ett", "ec", "drip")))
     return cmc, sneqv, out
+
+
+class OracleLsmRun:
+    """The model-step driver over the oracle RHS with the oracle land-surface step as the
+    forcing, on host vectors (the host leg of pihm_run_gpu_lsm's parity test)."""
+
+    def __init__(self, mesh, params, meteo, lai, z0, meltf, cmc0, sneqv0, etstep):
+        from pihm_amd.lsm import lsm_structs
+        self.L = lib()
+        self.PL = P.lib()
+        self.mesh = mesh
+        self.desc = mesh.desc()
+        self.keep = []
+        self.d, self.m = lsm_structs(params, meteo, self.keep)
+        arr = lambda a: self.keep.append(np.ascontiguousarray(a, dtype=np.float64)) or self.keep[-1]
+        lai, z0, cmc0, sneqv0 = arr(lai), arr(z0), arr(cmc0), arr(sneqv0)
+        self.st = self.L.oracle_state_new(mesh.nelem, mesh.nriver)
+        self.b = self.L.oracle_lsm_backend_new(C.byref(self.desc), self.st, C.byref(self.d), C.byref(self.m),
+                                               _dptr(lai), _dptr(z0), float(meltf), _dptr(cmc0),
+                                               _dptr(sneqv0), float(etstep))
+
+    def run(self, y0, cfg):
+        n = self.mesh.neq
+        v = self.L.oracle_nv_new(n, 1)
+        try:
+            data = np.ctypeslib.as_array(C.cast(self.L.oracle_nv_data(v), dp), shape=(n,))
+            data[:] = y0
+            self.L.oracle_lsm_backend_start(self.b, v)
+            st = P.RunStats()
+            rc = self.PL.pihm_drive(self.L.oracle_lsm_backend_get(self.b), v, C.byref(cfg), C.byref(st))
+            return rc, data.copy(), st
+        finally:
+            self.L.oracle_nv_destroy(v)
+
+    def storages(self):
+        cmc, sneqv = np.zeros(self.mesh.nelem), np.zeros(self.mesh.nelem)
+        self.L.oracle_lsm_backend_state(self.b, _dptr(cmc), _dptr(sneqv))
+        return cmc, sneqv
+
+    def __del__(self):
+        try:
+            self.L.oracle_lsm_backend_free(self.b)
+            self.L.oracle_state_free(self.st)
+        except Exception:
+            pass

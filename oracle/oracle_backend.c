@@ -102,3 +102,97 @@ void oracle_backend_free(oracle_backend *b)
 
 /* y as a host vector for the driver; data[] in caller order */
 N_Vector oracle_backend_vector(long n, int nthreads) { return oracle_nv_new(n, nthreads); }
+
+/* ------------------------------------------------------------------ land-surface backend
+ * apply_forcing runs the land-surface step (oracle_meteo_interp, oracle_intcp_snow_et) on
+ * the state the last summary saw, over etstep, as the reference's host path calls
+ * IntcpSnowEt (src/pihm.c:30-63).  The descriptor, records, lai and z0 arrays are the
+ * caller's and must outlive the backend. */
+#include "../include/pihm_lsm.h"
+
+int oracle_meteo_interp(const pihm_meteo *m, double t, double *met);
+void oracle_intcp_snow_et(int ne, const pihm_lsm_desc *d, const double *met, int nstation,
+                          const double *lai_e, const double *z0_e, double meltf,
+                          const double *unsat_e, const double *gw_e, double stepsize,
+                          double *cmc_e, double *sneqv_e, double *pcpdrp_e, double *edir_e,
+                          double *ett_e, double *ec_e, double *drip_e);
+
+typedef struct oracle_lsm_backend {
+    const pihm_mesh_desc *m;
+    oracle_state *s;
+    const pihm_lsm_desc *d;
+    const pihm_meteo *met;
+    const double *lai, *z0;
+    double meltf, etstep;
+    double *cmc, *sneqv, *ec, *drip, *metv;
+    const double *y;
+    pihm_backend be;
+} oracle_lsm_backend;
+
+static int oracle_cvrhs_lsm(double t, N_Vector y, N_Vector ydot, void *ud)
+{
+    oracle_lsm_backend *b = (oracle_lsm_backend *)ud;
+    return oracle_ode(b->m, b->s, t, y->ops->nvgetarraypointer(y), ydot->ops->nvgetarraypointer(ydot), NULL);
+}
+
+static int oracle_lsm_forcing(void *user, double t)
+{
+    oracle_lsm_backend *b = (oracle_lsm_backend *)user;
+    const int ne = b->m->nelem;
+    if (!b->y || oracle_meteo_interp(b->met, t, b->metv)) return PIHM_ERR_ARG;
+    oracle_intcp_snow_et(ne, b->d, b->metv, b->met->nstation, b->lai, b->z0, b->meltf, b->y + ne,
+                         b->y + 2 * ne, b->etstep, b->cmc, b->sneqv, b->s->pcpdrp, b->s->edir,
+                         b->s->ett, b->ec, b->drip);
+    return 0;
+}
+
+static int oracle_lsm_summary(void *user, N_Vector y)
+{
+    oracle_lsm_backend *b = (oracle_lsm_backend *)user;
+    b->y = y->ops->nvgetarraypointer(y);
+    memcpy(b->s->ws0_surf, b->y, b->m->nelem * sizeof(double));
+    return 0;
+}
+
+oracle_lsm_backend *oracle_lsm_backend_new(const pihm_mesh_desc *m, oracle_state *s,
+                                           const pihm_lsm_desc *d, const pihm_meteo *met,
+                                           const double *lai, const double *z0, double meltf,
+                                           const double *cmc0, const double *sneqv0, double etstep)
+{
+    oracle_lsm_backend *b = (oracle_lsm_backend *)calloc(1, sizeof(*b));
+    const size_t ne = (size_t)m->nelem;
+    b->m = m; b->s = s; b->d = d; b->met = met;
+    b->lai = lai; b->z0 = z0; b->meltf = meltf; b->etstep = etstep;
+    b->cmc = (double *)malloc(ne * sizeof(double));
+    b->sneqv = (double *)malloc(ne * sizeof(double));
+    b->ec = (double *)calloc(ne, sizeof(double));
+    b->drip = (double *)calloc(ne, sizeof(double));
+    b->metv = (double *)calloc((size_t)met->nstation * PIHM_MET_NVAR, sizeof(double));
+    memcpy(b->cmc, cmc0, ne * sizeof(double));
+    memcpy(b->sneqv, sneqv0, ne * sizeof(double));
+    b->be.rhs = oracle_cvrhs_lsm;
+    b->be.rhs_data = b;
+    b->be.apply_forcing = oracle_lsm_forcing;
+    b->be.summary = oracle_lsm_summary;
+    b->be.model_step_end = NULL;
+    b->be.user = b;
+    return b;
+}
+
+/* the initial Summary (ws0 = y0) before the driver runs */
+int oracle_lsm_backend_start(oracle_lsm_backend *b, N_Vector y) { return oracle_lsm_summary(b, y); }
+
+const pihm_backend *oracle_lsm_backend_get(oracle_lsm_backend *b) { return &b->be; }
+
+void oracle_lsm_backend_state(const oracle_lsm_backend *b, double *cmc, double *sneqv)
+{
+    memcpy(cmc, b->cmc, b->m->nelem * sizeof(double));
+    memcpy(sneqv, b->sneqv, b->m->nelem * sizeof(double));
+}
+
+void oracle_lsm_backend_free(oracle_lsm_backend *b)
+{
+    if (!b) return;
+    free(b->cmc); free(b->sneqv); free(b->ec); free(b->drip); free(b->metv);
+    free(b);
+}

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/pihm_driver.h"
+#include "../../include/pihm_lsm.h"
 #include "common.hpp"
 #include "ctx.hpp"
 #include "nvector_impl.hpp"
@@ -141,13 +142,15 @@ static int gpu_forcing(void *user, double t) {
   return hipGetLastError() == hipSuccess ? 0 : PIHM_ERR_HIP;
 }
 
-static int gpu_summary(void *user, N_Vector y) {
-  pihm_gpu_backend *b = (pihm_gpu_backend *)user;
-  pihm_ctx *c = b->ctx;
-  // ws0.surf <- y[SURF] (device order on both sides)
+// ws0.surf <- y[SURF] (device order on both sides)
+static int copy_ws0(pihm_ctx *c, N_Vector y) {
   hipLaunchKernelGGL(k_copy, dim3(div_up(c->ne, kBlock)), dim3(kBlock), 0, c->stream, (long)c->ne,
                      N_VGetDeviceArrayPointer_Hip(y), c->args.e.ws0s);
   return hipGetLastError() == hipSuccess ? 0 : PIHM_ERR_HIP;
+}
+
+static int gpu_summary(void *user, N_Vector y) {
+  return copy_ws0(((pihm_gpu_backend *)user)->ctx, y);
 }
 
 extern "C" int pihm_gpu_backend_create(pihm_ctx *c, const pihm_forcing *f, pihm_gpu_backend **out) {
@@ -193,22 +196,20 @@ extern "C" int pihm_gpu_backend_get(pihm_gpu_backend *b, pihm_backend *out) {
   return PIHM_OK;
 }
 
-extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_config *cfg,
-                            double *y_inout, pihm_run_stats *st) {
-  if (!c || !f || !cfg || !y_inout) return PIHM_ERR_ARG;
-  pihm_gpu_backend *b = nullptr;
-  int rc = pihm_gpu_backend_create(c, f, &b);
-  if (rc) return rc;
+// Device vector, stream and NaN polling around pihm_drive for a backend whose
+// rhs_data is the context.  The backend's summary sees the device vector first
+// (initial ws0 = initial state, src/initialize.c InitVar).
+static int run_device(pihm_ctx *c, const pihm_run_config *cfg, double *y_inout, pihm_run_stats *st,
+                      const pihm_backend *be) {
   // graphs need a stream of their own (capture is not possible on the null stream)
   hipStream_t own = nullptr;
   void *saved_stream = c->stream;
   if (cfg->graphs && !c->stream) {
-    if (hipStreamCreate(&own) != hipSuccess) { pihm_gpu_backend_destroy(b); return PIHM_ERR_HIP; }
+    if (hipStreamCreate(&own) != hipSuccess) return PIHM_ERR_HIP;
     c->stream = own;
   }
   N_Vector y = N_VNew_Hip(c->neq);
   if (!y) {
-    pihm_gpu_backend_destroy(b);
     c->stream = (hipStream_t)saved_stream;
     if (own) (void)hipStreamDestroy(own);
     return PIHM_ERR_HIP;
@@ -216,14 +217,9 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   N_VSetStream_Hip(y, c->stream);
   const int saved_check = c->nan_check;
   c->nan_check = 0;   // NaN shows up in the norms; polled once at the end
-  rc = pihm_ctx_state_to_device(c, y_inout, y);
-  if (!rc) {
-    // initial ws0 = initial state (src/initialize.c InitVar)
-    rc = gpu_summary(b, y);
-  }
-  pihm_backend be;
-  pihm_gpu_backend_get(b, &be);
-  if (!rc) rc = pihm_drive(&be, y, cfg, st);
+  int rc = pihm_ctx_state_to_device(c, y_inout, y);
+  if (!rc) rc = be->summary(be->user, y);
+  if (!rc) rc = pihm_drive(be, y, cfg, st);
   if (!rc) rc = pihm_ctx_poll_nan(c);
   if (!rc || rc == PIHM_ERR_NAN) {
     int rc2 = pihm_ctx_state_from_device(c, y, y_inout);
@@ -231,11 +227,67 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   }
   c->nan_check = saved_check;
   N_VDestroy_Hip(y);
-  pihm_gpu_backend_destroy(b);
   if (own) {
     (void)hipStreamSynchronize(own);
     c->stream = (hipStream_t)saved_stream;
     (void)hipStreamDestroy(own);
   }
   return rc;
+}
+
+extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_config *cfg,
+                            double *y_inout, pihm_run_stats *st) {
+  if (!c || !f || !cfg || !y_inout) return PIHM_ERR_ARG;
+  pihm_gpu_backend *b = nullptr;
+  int rc = pihm_gpu_backend_create(c, f, &b);
+  if (rc) return rc;
+  pihm_backend be;
+  pihm_gpu_backend_get(b, &be);
+  rc = run_device(c, cfg, y_inout, st, &be);
+  pihm_gpu_backend_destroy(b);
+  return rc;
+}
+
+// ---------------------------------------------------------------- land-surface backend
+namespace pihm {
+pihm_ctx *lsm_context(const pihm_lsm *l);   // lsm.hip
+}
+
+namespace {
+
+// Every LSM step the land-surface step runs on the state the last Summary saw
+// (IntcpSnowEt reads elem.ws, src/pihm.c:30-63); the driver integrates y in
+// place, so the vector Summary is handed is the one the next step reads.
+struct LsmRun {
+  pihm_ctx *ctx;
+  pihm_lsm *lsm;
+  N_Vector y;
+  double etstep;
+};
+
+int lsm_forcing(void *user, double t) {
+  LsmRun *r = (LsmRun *)user;
+  return pihm_lsm_step(r->lsm, t, r->etstep, r->y);
+}
+
+int lsm_summary(void *user, N_Vector y) {
+  LsmRun *r = (LsmRun *)user;
+  r->y = y;
+  return copy_ws0(r->ctx, y);
+}
+
+}  // namespace
+
+extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, const pihm_run_config *cfg, double *y_inout,
+                                pihm_run_stats *st) {
+  if (!c || !lsm || !cfg || !y_inout || pihm::lsm_context(lsm) != c) return PIHM_ERR_ARG;
+  LsmRun r{c, lsm, nullptr, cfg->etstep};
+  pihm_backend be;
+  std::memset(&be, 0, sizeof(be));
+  be.rhs = gpu_rhs;
+  be.rhs_data = c;
+  be.apply_forcing = lsm_forcing;
+  be.summary = lsm_summary;
+  be.user = &r;
+  return run_device(c, cfg, y_inout, st, &be);
 }

@@ -237,6 +237,10 @@ int download_elem(pihm_lsm *l, const double *src, double *dst) {
 
 }  // namespace
 
+namespace pihm {
+pihm_ctx *lsm_context(const pihm_lsm *l) { return l ? l->ctx : nullptr; }
+}
+
 extern "C" {
 
 int pihm_lsm_create(pihm_ctx *ctx, const pihm_lsm_desc *desc, const pihm_meteo *met, pihm_lsm **out) {

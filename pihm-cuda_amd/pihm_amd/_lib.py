@@ -239,6 +239,7 @@ _PROTOS = [
     ("pihm_lsm_get_state", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("pihm_lsm_step", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     ("pihm_lsm_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("pihm_run_gpu_lsm", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p, C.POINTER(RunStats)]),
 ]
 
 

@@ -125,6 +125,26 @@ class HipLSM:
         return out
 
 
+def run_gpu_lsm(mesh, params, meteo, veg, cfg, y0, cmc0=None, sneqv0=None):
+    """Integrate on the device with the land-surface step as the forcing
+    (pihm_run_gpu_lsm).  ``veg`` is (lai, z0, meltf).  Returns (rc, y, stats, cmc, sneqv)."""
+    from .rhs import HipRHS
+    h = HipRHS(mesh, nan_check=False)
+    lsm = HipLSM(h, params, meteo)
+    try:
+        lsm.set_veg(*veg)
+        ne = mesh.nelem
+        lsm.set_state(np.zeros(ne) if cmc0 is None else cmc0, np.zeros(ne) if sneqv0 is None else sneqv0)
+        y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+        st = _lib.RunStats()
+        rc = h.L.pihm_run_gpu_lsm(h.ctx, lsm.h, C.byref(cfg), _dp(y), C.byref(st))
+        cmc, sneqv = lsm.get_state()
+        return rc, y, st, cmc, sneqv
+    finally:
+        lsm.close()
+        h.close()
+
+
 def synthetic_params(ne, nstation, seed=0):
     """Seeded per-element land-surface parameters spanning the reference's land-cover and
     soil tables (vegprmt.tbl, .soil): shading 0.05-0.95, root depth 0.05-1 m, minimum

@@ -189,3 +189,49 @@ def test_lsm_argument_errors(c0):
     assert L.pihm_lsm_create(bctx, C.byref(d), C.byref(met), C.byref(out)) == _lib.PIHM_ERR_ARG
     L.pihm_ctx_destroy(bctx)
     h.close()
+
+
+def _wrms(a, b, rtol=1e-3, atol=1e-4):
+    w = 1.0 / (rtol * np.abs(b) + atol)
+    return float(np.sqrt(np.mean(((a - b) * w) ** 2)))
+
+
+def test_lsm_run_matches_host_run(c0):
+    """pihm_run_gpu_lsm (device LSM every 900 s, then the device solve) against the same
+    model-step driver over the oracle RHS and the oracle LSM on host vectors: the states
+    agree within the solver tolerance, and the canopy and snow storages, which the step
+    computes from the forcing alone, agree to rounding.  Two device runs agree bit for bit."""
+    from pihm_amd.forcing import run_config
+    from pihm_amd.lsm import run_gpu_lsm
+    m = c0
+    p, meteo, lai, z0, meltf, cmc0, sneqv0 = _setup(m, 7)
+    y0 = m.initial_state()
+    cfg = run_config(T0 + 6 * 3600.0, t_start=T0)
+    print("[lsm run] c0 6 h device", flush=True)
+    rc_g, y_g, st_g, cmc_g, sn_g = run_gpu_lsm(m, p, meteo, (lai, z0, meltf), cfg, y0, cmc0, sneqv0)
+    assert rc_g == 0, st_g.as_dict()
+    assert st_g.model_steps == 360
+    print("[lsm run] c0 6 h host", flush=True)
+    o = O.OracleLsmRun(m, p, meteo, lai, z0, meltf, cmc0, sneqv0, cfg.etstep)
+    rc_c, y_c, st_c = o.run(y0, cfg)
+    assert rc_c == 0
+    cmc_c, sn_c = o.storages()
+    print("gpu", st_g.as_dict(), "\ncpu", st_c.as_dict(), flush=True)
+    assert _wrms(y_g, y_c) < 1.0
+    np.testing.assert_allclose(cmc_g, cmc_c, rtol=1e-10, atol=1e-18)
+    np.testing.assert_allclose(sn_g, sn_c, rtol=1e-10, atol=1e-18)
+    r2 = run_gpu_lsm(m, p, meteo, (lai, z0, meltf), cfg, y0, cmc0, sneqv0)
+    assert np.array_equal(r2[1], y_g) and np.array_equal(r2[3], cmc_g) and np.array_equal(r2[4], sn_g)
+
+
+def test_lsm_run_stops_outside_records(c0):
+    from pihm_amd.forcing import run_config
+    from pihm_amd.lsm import run_gpu_lsm
+    m = c0
+    p, meteo, lai, z0, meltf, cmc0, sneqv0 = _setup(m, 8)
+    t_last = meteo[0][0, -1]
+    cfg = run_config(t_last + 1800.0, t_start=t_last - 1800.0)
+    rc, y, st, _, _ = run_gpu_lsm(m, p, meteo, (lai, z0, meltf), cfg, m.initial_state())
+    # LSM steps at t_last - 1800, - 900, t_last itself, then t_last + 900: outside
+    assert rc == _lib.PIHM_ERR_ARG
+    assert st.model_steps == 45

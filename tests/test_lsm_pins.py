@@ -531,3 +531,26 @@ def test_drip_is_rutter_exponential():
     assert np.all(f["ec"] == 0.0)
     np.testing.assert_allclose(f["drip"], 6.52e-7 * intcp * np.exp(3.89 * c / intcp), rtol=1e-14)
     np.testing.assert_allclose(cmc, c - f["drip"] * dt, rtol=1e-14)
+
+
+def test_host_lsm_run_conserves_canopy_and_snow(c0):
+    """The host leg of the coupled run: over two hours the driver calls the land-surface
+    step every LSM step, and the carried storages move as the step's own fluxes say."""
+    from pihm_amd.forcing import run_config
+    m = c0
+    ne = m.nelem
+    p = synthetic_params(ne, 1, seed=3)
+    for k in ("depth", "porosity", "rzd", "zmax"):
+        p[k] = m.a[k].copy()
+    p["smcmin"] = np.minimum(p["smcmin"], 0.5 * p["porosity"])
+    ft, data, _ = inputs.read_meteo(os.path.join(GOLDEN, "example.meteo"))
+    t0 = inputs.str_time("2009-01-01 00:00")
+    lai, z0 = np.full(ne, 1.5), np.full(ne, 0.1)
+    cmc0, sn0 = np.zeros(ne), np.full(ne, 0.01)
+    o = O.OracleLsmRun(m, p, (ft, data), lai, z0, 1e-7, cmc0, sn0, 900.0)
+    rc, y, st = o.run(m.initial_state(), run_config(t0 + 2 * 3600.0, t_start=t0))
+    assert rc == 0 and st.model_steps == 120
+    cmc, sn = o.storages()
+    # January at the example site: below freezing, so the snow store only grows
+    assert np.all(sn >= sn0) and np.all(cmc >= 0.0)
+    assert np.all(cmc <= p["cmcfactr"] * lai * p["shdfac"] * (1 + 1e-15))
