@@ -61,6 +61,8 @@ typedef struct pihm_backend {
   int (*summary)(void *user, N_Vector y);       /* ws0 <- state after a model step */
   int (*model_step_end)(void *user, double t, N_Vector y);  /* optional, may be NULL */
   void *user;
+  int (*apply_bc)(void *user, double t);        /* every model step before the forcing
+                                                   (ApplyBc, src/pihm.c:17-24); may be NULL */
 } pihm_backend;
 
 int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_config *cfg,

@@ -22,6 +22,7 @@
 #define PIHM_LSM_H
 
 #include <stdint.h>
+#include "pihm_bc.h"
 #include "pihm_driver.h"
 #include "pihm_nvector.h"
 #include "pihm_rhs.h"
@@ -82,15 +83,17 @@ int  pihm_lsm_step(pihm_lsm *lsm, double t, double stepsize, N_Vector y);
  * caller order; any pointer may be NULL.  Synchronises the stream. */
 int  pihm_lsm_get_fluxes(pihm_lsm *lsm, double *pcpdrp, double *edir, double *ett, double *ec, double *drip);
 
-/* A whole run with the land-surface step as the forcing: the model-step loop of
- * pihm_run_gpu (src/main.c:217-254, src/pihm.c:10-187) where every LSM step
- * (cfg->etstep) starts with pihm_lsm_step at the step's time over etstep, as
- * the reference's host path calls IntcpSnowEt (src/pihm.c:60-63), on the
- * state the last Summary saw.  cfg->t_start is on the records' clock.  The
- * carried canopy and snow storages stay in the LSM (pihm_lsm_get_state).
- * Returns PIHM_ERR_ARG when a step's time lies outside the records. */
-int  pihm_run_gpu_lsm(pihm_ctx *ctx, pihm_lsm *lsm, const pihm_run_config *cfg, double *y_inout,
-                      pihm_run_stats *stats);
+/* A whole run with the device producers as the forcing: the model-step loop
+ * of pihm_run_gpu (src/main.c:217-254, src/pihm.c:10-187) where every model
+ * step starts with pihm_bc_apply at the step's time when bc is not NULL
+ * (ApplyBc, src/pihm.c:17-24), and every LSM step (cfg->etstep) with
+ * pihm_lsm_step at the step's time over etstep, as the reference's host path
+ * calls IntcpSnowEt (src/pihm.c:60-63), on the state the last Summary saw.
+ * cfg->t_start is on the records' clock.  The carried canopy and snow
+ * storages stay in the LSM (pihm_lsm_get_state).  Returns PIHM_ERR_ARG when a
+ * step's time lies outside the records or series. */
+int  pihm_run_gpu_lsm(pihm_ctx *ctx, pihm_lsm *lsm, pihm_bc *bc, const pihm_run_config *cfg,
+                      double *y_inout, pihm_run_stats *stats);
 
 #ifdef __cplusplus
 }

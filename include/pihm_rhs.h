@@ -132,6 +132,11 @@ int  pihm_ctx_set_stream(pihm_ctx *ctx, void *hip_stream);
 int  pihm_stream_create(void **hip_stream);
 int  pihm_stream_destroy(void *hip_stream);
 int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
+/* The step inputs the RHS will read, in caller order (layouts as in
+ * pihm_step_inputs); any pointer may be NULL.  Synchronises the stream.  Shows
+ * what the device producers (land-surface step, boundary series) wrote. */
+int  pihm_ctx_get_step_inputs(pihm_ctx *ctx, double *pcpdrp, double *edir, double *ett,
+                              double *ws0_surf, double *elem_bc, double *river_bc);
 int  pihm_ctx_neq(const pihm_ctx *ctx);
 
 /* Exact permutations between caller order and device order. `perm_out`

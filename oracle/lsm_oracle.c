@@ -1,11 +1,13 @@
 /*
- * lsm_oracle.c - CPU restatement of the reference's simple land-surface step.
+ * lsm_oracle.c - CPU restatement of the reference's per-step producers: the simple
+ * land-surface step and the boundary-condition series.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/ may load this code, as the checker
  * of the device land-surface step.  The product library never links it.
  *
  *   oracle_meteo_interp   IntrplForc (src/forcing.c:482-530) for every
  *                         station's records
+ *   oracle_bc_apply       ApplyElemBc / ApplyRiverBc (src/forcing.c:127-244)
  *   oracle_intcp_snow_et  IntcpSnowEt (src/is_sm_et.c:5-226) with
  *                         ApplyMeteoForc's per-element values
  *                         (src/forcing.c:340-349)
@@ -14,6 +16,7 @@
  */
 #include <math.h>
 
+#include "../include/pihm_bc.h"
 #include "../include/pihm_lsm.h"
 
 /* src/include/pihm_const.h:9-15 */
@@ -23,31 +26,73 @@
 #define O_RD 287.04
 #define O_RV 461.5
 
+/* IntrplForc (src/forcing.c:482-530) on one series: ft[nrec], d[nrec*nvar]
+ * record-major, into out[v * ostride]; 1 when t is outside (the reference
+ * exits). */
+static int interp_series(const double *ft, const double *d, int nrec, int nvar, double t, double *out,
+                         long ostride)
+{
+    int first = 1, last = nrec - 1, middle, j;
+    if (t < ft[0] || t > ft[nrec - 1]) return 1;
+    while (first <= last) {
+        middle = (first + last) / 2;
+        if (t >= ft[middle - 1] && t <= ft[middle]) {
+            for (j = 0; j < nvar; j++)
+                out[j * ostride] =
+                    ((ft[middle] - t) * d[(long)(middle - 1) * nvar + j] +
+                     (t - ft[middle - 1]) * d[(long)middle * nvar + j]) /
+                    (ft[middle] - ft[middle - 1]);
+            break;
+        } else if (ft[middle] > t) {
+            last = middle - 1;
+        } else {
+            first = middle + 1;
+        }
+    }
+    return 0;
+}
+
 /* Each station's records at time t into met[v * nstation + k]; 1 when t is
- * outside a station's records (the reference exits). */
+ * outside a station's records. */
 int oracle_meteo_interp(const pihm_meteo *m, double t, double *met)
 {
     int k;
-    for (k = 0; k < m->nstation; k++) {
-        const double *ft = m->ftime + (long)k * m->nrec;
-        const double *d = m->data + (long)k * m->nrec * PIHM_MET_NVAR;
-        int first = 1, last = m->nrec - 1, middle, j;
-        if (t < ft[0] || t > ft[m->nrec - 1]) return 1;
-        while (first <= last) {
-            middle = (first + last) / 2;
-            if (t >= ft[middle - 1] && t <= ft[middle]) {
-                for (j = 0; j < PIHM_MET_NVAR; j++)
-                    met[j * m->nstation + k] =
-                        ((ft[middle] - t) * d[(long)(middle - 1) * PIHM_MET_NVAR + j] +
-                         (t - ft[middle - 1]) * d[(long)middle * PIHM_MET_NVAR + j]) /
-                        (ft[middle] - ft[middle - 1]);
-                break;
-            } else if (ft[middle] > t) {
-                last = middle - 1;
-            } else {
-                first = middle + 1;
-            }
+    for (k = 0; k < m->nstation; k++)
+        if (interp_series(m->ftime + (long)k * m->nrec, m->data + (long)k * m->nrec * PIHM_MET_NVAR,
+                          m->nrec, PIHM_MET_NVAR, t, met + k, m->nstation))
+            return 1;
+    return 0;
+}
+
+/* ApplyElemBc and ApplyRiverBc (src/forcing.c:127-177, 214-244): every series
+ * interpolated at t, then each element edge (elem_bc[3*i+j], caller order)
+ * and river segment with a nonzero code takes series |code|-1's value.
+ * Either set may be absent (NULL).  1 when t is outside a series. */
+int oracle_bc_apply(const pihm_ts *ets, const int32_t *etype, int ne, double *elem_bc,
+                    const pihm_ts *rts, const int32_t *rtype, int nr, double *river_bc, double t)
+{
+    double ev[4096], rv[4096];
+    int k, i, j;
+    const int nts_e = ets ? ets->nts : 0, nts_r = rts ? rts->nts : 0;
+    if (nts_e > 4096 || nts_r > 4096) return 2;
+    for (k = 0; k < nts_e; k++)
+        if (interp_series(ets->ftime + (long)k * ets->nrec, ets->value + (long)k * ets->nrec, ets->nrec, 1,
+                          t, ev + k, 1))
+            return 1;
+    for (k = 0; k < nts_r; k++)
+        if (interp_series(rts->ftime + (long)k * rts->nrec, rts->value + (long)k * rts->nrec, rts->nrec, 1,
+                          t, rv + k, 1))
+            return 1;
+    for (i = 0; etype && i < ne; i++)
+        for (j = 0; j < 3; j++) {
+            const int c = etype[3 * i + j];
+            if (c > 0) elem_bc[3 * i + j] = ev[c - 1];
+            else if (c < 0) elem_bc[3 * i + j] = ev[-c - 1];
         }
+    for (i = 0; rtype && i < nr; i++) {
+        const int c = rtype[i];
+        if (c > 0) river_bc[i] = rv[c - 1];
+        else if (c < 0) river_bc[i] = rv[-c - 1];
     }
     return 0;
 }

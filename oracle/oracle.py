@@ -63,6 +63,9 @@ def lib():
         ("oracle_lsm_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
         ("oracle_lsm_backend_state", None, [C.c_void_p, dp, dp]),
         ("oracle_lsm_backend_free", None, [C.c_void_p]),
+        ("oracle_lsm_backend_set_bc", None, [C.c_void_p, C.POINTER(P.Ts), ip, C.POINTER(P.Ts), ip]),
+        ("oracle_bc_apply", C.c_int, [C.POINTER(P.Ts), ip, C.c_int, dp, C.POINTER(P.Ts), ip, C.c_int, dp,
+                                      C.c_double]),
         ("oracle_intcp_snow_et", None, [C.c_int, C.POINTER(P.LsmDesc), dp, C.c_int, dp, dp, C.c_double,
                                         dp, dp, C.c_double, dp, dp, dp, dp, dp, dp, dp]),
     ]
@@ -258,11 +261,27 @@ def intcp_snow_et(params, met, lai, z0, meltf, unsat, gw, stepsize, cmc, sneqv):
     return cmc, sneqv, out
 
 
+def bc_apply(ne, nr, t, elem_type=None, elem_series=None, river_type=None, river_series=None,
+             elem_bc=None, river_bc=None):
+    """ApplyElemBc/ApplyRiverBc on the host: (rc, elem_bc[ne, 3], river_bc[nr]) starting from
+    the given values (zeros by default)."""
+    from pihm_amd.bc import code_array, ts_struct
+    keep = []
+    es, rs = ts_struct(elem_series, keep), ts_struct(river_series, keep)
+    eb = np.zeros(3 * ne) if elem_bc is None else np.ascontiguousarray(elem_bc, np.float64).reshape(3 * ne).copy()
+    rb = np.zeros(max(nr, 1)) if river_bc is None else np.ascontiguousarray(river_bc, np.float64).copy()
+    rc = lib().oracle_bc_apply(C.byref(es) if es is not None else None, code_array(elem_type, 3 * ne, keep), ne,
+                               _dptr(eb), C.byref(rs) if rs is not None else None,
+                               code_array(river_type, nr, keep), nr, _dptr(rb), float(t))
+    return rc, eb.reshape(ne, 3), rb[:nr]
+
+
 class OracleLsmRun:
     """The model-step driver over the oracle RHS with the oracle land-surface step as the
-    forcing, on host vectors (the host leg of pihm_run_gpu_lsm's parity test)."""
+    forcing, on host vectors (the host leg of pihm_run_gpu_lsm's parity test).  ``bc`` is
+    (elem_type, elem_series, river_type, river_series) or None."""
 
-    def __init__(self, mesh, params, meteo, lai, z0, meltf, cmc0, sneqv0, etstep):
+    def __init__(self, mesh, params, meteo, lai, z0, meltf, cmc0, sneqv0, etstep, bc=None):
         from pihm_amd.lsm import lsm_structs
         self.L = lib()
         self.PL = P.lib()
@@ -276,6 +295,14 @@ class OracleLsmRun:
         self.b = self.L.oracle_lsm_backend_new(C.byref(self.desc), self.st, C.byref(self.d), C.byref(self.m),
                                                _dptr(lai), _dptr(z0), float(meltf), _dptr(cmc0),
                                                _dptr(sneqv0), float(etstep))
+        if bc is not None:
+            from pihm_amd.bc import code_array, ts_struct
+            et, es, rt, rs = bc
+            self.es, self.rs = ts_struct(es, self.keep), ts_struct(rs, self.keep)
+            self.L.oracle_lsm_backend_set_bc(
+                self.b, C.byref(self.es) if self.es is not None else None,
+                code_array(et, 3 * mesh.nelem, self.keep), C.byref(self.rs) if self.rs is not None else None,
+                code_array(rt, mesh.nriver, self.keep))
 
     def run(self, y0, cfg):
         n = self.mesh.neq

@@ -126,8 +126,20 @@ typedef struct oracle_lsm_backend {
     double meltf, etstep;
     double *cmc, *sneqv, *ec, *drip, *metv;
     const double *y;
+    const pihm_ts *ets, *rts;              /* boundary series, optional */
+    const int32_t *etype, *rtype;
     pihm_backend be;
 } oracle_lsm_backend;
+
+int oracle_bc_apply(const pihm_ts *ets, const int32_t *etype, int ne, double *elem_bc,
+                    const pihm_ts *rts, const int32_t *rtype, int nr, double *river_bc, double t);
+
+static int oracle_lsm_bc(void *user, double t)
+{
+    oracle_lsm_backend *b = (oracle_lsm_backend *)user;
+    return oracle_bc_apply(b->ets, b->etype, b->m->nelem, b->s->elem_bc, b->rts, b->rtype, b->m->nriver,
+                           b->s->river_bc, t) ? PIHM_ERR_ARG : 0;
+}
 
 static int oracle_cvrhs_lsm(double t, N_Vector y, N_Vector ydot, void *ud)
 {
@@ -177,6 +189,14 @@ oracle_lsm_backend *oracle_lsm_backend_new(const pihm_mesh_desc *m, oracle_state
     b->be.model_step_end = NULL;
     b->be.user = b;
     return b;
+}
+
+/* Boundary series applied every model step (ApplyBc, src/pihm.c:17-24). */
+void oracle_lsm_backend_set_bc(oracle_lsm_backend *b, const pihm_ts *ets, const int32_t *etype,
+                               const pihm_ts *rts, const int32_t *rtype)
+{
+    b->ets = ets; b->etype = etype; b->rts = rts; b->rtype = rtype;
+    b->be.apply_bc = oracle_lsm_bc;
 }
 
 /* the initial Summary (ws0 = y0) before the driver runs */

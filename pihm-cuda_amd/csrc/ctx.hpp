@@ -178,3 +178,17 @@ struct pihm_ctx {
   int *nanflag_host;             // pinned
   double *stage_host;            // scratch for host-side permutations
 };
+
+// Caller-order index of device element p / device river r (the loaded slots
+// concatenated at their actual sizes), or -1 for unused slot capacity.
+static inline long caller_elem(const pihm_ctx *c, int p) {
+  if (c->perm[p] < 0) return -1;
+  return (long)c->members[c->emem_host[p]].ceoff + c->perm[p];
+}
+static inline long caller_river(const pihm_ctx *c, int r) {
+  for (int k = 0; k < c->nmem; k++) {
+    const auto &M = c->members[k];
+    if (r >= M.roff && r < M.roff + c->caps[k].nr) return (r - M.roff < M.nr) ? M.croff + (r - M.roff) : -1;
+  }
+  return -1;
+}

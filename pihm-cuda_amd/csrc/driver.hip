@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/pihm_driver.h"
+#include "../../include/pihm_bc.h"
 #include "../../include/pihm_lsm.h"
 #include "common.hpp"
 #include "ctx.hpp"
@@ -47,6 +48,8 @@ extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_con
   const long et_every = std::max(1L, (long)std::llround(cfg->etstep / cfg->stepsize));
   for (long k = 0; k < nsteps; k++) {
     const double t = tstart + k * cfg->stepsize;
+    // ApplyBc every model step (src/pihm.c:17-24)
+    if (be->apply_bc && (flag = be->apply_bc(be->user, t))) break;
     // ApplyForc every LSM step (src/pihm.c:30-45)
     if (k % et_every == 0 && be->apply_forcing) {
       if ((flag = be->apply_forcing(be->user, t))) break;
@@ -251,6 +254,7 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
 // ---------------------------------------------------------------- land-surface backend
 namespace pihm {
 pihm_ctx *lsm_context(const pihm_lsm *l);   // lsm.hip
+pihm_ctx *bc_context(const pihm_bc *b);     // bc.hip
 }
 
 namespace {
@@ -261,9 +265,15 @@ namespace {
 struct LsmRun {
   pihm_ctx *ctx;
   pihm_lsm *lsm;
+  pihm_bc *bc;
   N_Vector y;
   double etstep;
 };
+
+int lsm_bc(void *user, double t) {
+  LsmRun *r = (LsmRun *)user;
+  return pihm_bc_apply(r->bc, t);
+}
 
 int lsm_forcing(void *user, double t) {
   LsmRun *r = (LsmRun *)user;
@@ -278,16 +288,18 @@ int lsm_summary(void *user, N_Vector y) {
 
 }  // namespace
 
-extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, const pihm_run_config *cfg, double *y_inout,
-                                pihm_run_stats *st) {
+extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, pihm_bc *bc, const pihm_run_config *cfg,
+                                double *y_inout, pihm_run_stats *st) {
   if (!c || !lsm || !cfg || !y_inout || pihm::lsm_context(lsm) != c) return PIHM_ERR_ARG;
-  LsmRun r{c, lsm, nullptr, cfg->etstep};
+  if (bc && pihm::bc_context(bc) != c) return PIHM_ERR_ARG;
+  LsmRun r{c, lsm, bc, nullptr, cfg->etstep};
   pihm_backend be;
   std::memset(&be, 0, sizeof(be));
   be.rhs = gpu_rhs;
   be.rhs_data = c;
   be.apply_forcing = lsm_forcing;
   be.summary = lsm_summary;
+  be.apply_bc = bc ? lsm_bc : nullptr;
   be.user = &r;
   return run_device(c, cfg, y_inout, st, &be);
 }

@@ -16,6 +16,7 @@
 #include "../../include/pihm_lsm.h"
 #include "common.hpp"
 #include "ctx.hpp"
+#include "tsinterp.hpp"
 
 using namespace pihm;
 
@@ -41,30 +42,12 @@ struct LsmDev {
   double *met;              // [PIHM_MET_NVAR * nstation] interpolated station values
 };
 
-// IntrplForc: src/forcing.c:499-529 (the same bisection, so a time equal to
-// a record time picks the same interval and rounds the same way).
+// IntrplForc (src/forcing.c:499-529) for every station, one thread each.
 __global__ __launch_bounds__(kBlock) void k_meteo(int nstation, int nrec, const double *ftime, const double *data,
                                                   double t, double *met) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= nstation) return;
-  const double *ft = ftime + (long)k * nrec;
-  const double *d = data + (long)k * nrec * PIHM_MET_NVAR;
-  int first = 1, last = nrec - 1;
-  while (first <= last) {
-    const int middle = (first + last) / 2;
-    if (t >= ft[middle - 1] && t <= ft[middle]) {
-#pragma unroll
-      for (int j = 0; j < PIHM_MET_NVAR; j++)
-        met[j * nstation + k] = ((ft[middle] - t) * d[(long)(middle - 1) * PIHM_MET_NVAR + j] +
-                                 (t - ft[middle - 1]) * d[(long)middle * PIHM_MET_NVAR + j]) /
-                                (ft[middle] - ft[middle - 1]);
-      break;
-    } else if (ft[middle] > t) {
-      last = middle - 1;
-    } else {
-      first = middle + 1;
-    }
-  }
+  ts_interp<PIHM_MET_NVAR>(ftime + (long)k * nrec, data + (long)k * nrec * PIHM_MET_NVAR, nrec, t, met + k, nstation);
 }
 
 struct StepArgs {

@@ -1250,20 +1250,6 @@ extern "C" int pihm_ctx_get_perm(const pihm_ctx *c, int32_t *out) {
   return PIHM_OK;
 }
 
-// Caller-order index of device element p / device river r (the loaded slots
-// concatenated at their actual sizes), or -1 for unused slot capacity.
-static inline long caller_elem(const pihm_ctx *c, int p) {
-  if (c->perm[p] < 0) return -1;
-  return (long)c->members[c->emem_host[p]].ceoff + c->perm[p];
-}
-static inline long caller_river(const pihm_ctx *c, int r) {
-  for (int k = 0; k < c->nmem; k++) {
-    const auto &M = c->members[k];
-    if (r >= M.roff && r < M.roff + c->caps[k].nr) return (r - M.roff < M.nr) ? M.croff + (r - M.roff) : -1;
-  }
-  return -1;
-}
-
 extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in) {
   if (!c || !in) return PIHM_ERR_ARG;
   const int ne = c->ne, nr = c->nr;
@@ -1300,6 +1286,45 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
     }
     PIHM_HIP_TRY(hipMemcpyAsync(c->args.r.bc, rb.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_step_inputs(pihm_ctx *c, double *pcpdrp, double *edir, double *ett,
+                                        double *ws0_surf, double *elem_bc, double *river_bc) {
+  if (!c) return PIHM_ERR_ARG;
+  const int ne = c->ne, nr = c->nr;
+  std::vector<double> hd(3L * ne);
+  auto down = [&](const double *src, long n) -> int {
+    PIHM_HIP_TRY(hipMemcpyAsync(hd.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PIHM_OK;
+  };
+  const ElemDev &E = c->args.e;
+  const double *src[] = {E.pcpdrp, E.edir, E.ett, E.ws0s};
+  double *dst[] = {pcpdrp, edir, ett, ws0_surf};
+  for (int f = 0; f < 4; f++) {
+    if (!dst[f]) continue;
+    if (int rc = down(src[f], ne)) return rc;
+    for (int p = 0; p < ne; p++) {
+      const long ci = caller_elem(c, p);
+      if (ci >= 0) dst[f][ci] = hd[p];
+    }
+  }
+  if (elem_bc) {
+    if (int rc = down(E.ebc, 3L * ne)) return rc;
+    for (int j = 0; j < 3; j++)
+      for (int p = 0; p < ne; p++) {
+        const long ci = caller_elem(c, p);
+        if (ci >= 0) elem_bc[3L * ci + j] = hd[(long)j * ne + p];
+      }
+  }
+  if (river_bc && nr > 0) {
+    if (int rc = down(c->args.r.bc, nr)) return rc;
+    for (int r = 0; r < nr; r++) {
+      const long cr = caller_river(c, r);
+      if (cr >= 0) river_bc[cr] = hd[r];
+    }
   }
   return PIHM_OK;
 }

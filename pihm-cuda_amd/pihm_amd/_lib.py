@@ -92,6 +92,12 @@ class LsmDesc(C.Structure):
 PIHM_MET_NVAR = 5
 
 
+class Ts(C.Structure):
+    """pihm_ts (include/pihm_bc.h)."""
+
+    _fields_ = [("nts", C.c_int32), ("nrec", C.c_int32), ("ftime", c_double_p), ("value", c_double_p)]
+
+
 class Meteo(C.Structure):
     """pihm_meteo (include/pihm_lsm.h)."""
 
@@ -108,7 +114,8 @@ class Backend(C.Structure):
     """pihm_backend (include/pihm_driver.h)."""
 
     _fields_ = [("rhs", C.c_void_p), ("rhs_data", C.c_void_p), ("apply_forcing", C.c_void_p),
-                ("summary", C.c_void_p), ("model_step_end", C.c_void_p), ("user", C.c_void_p)]
+                ("summary", C.c_void_p), ("model_step_end", C.c_void_p), ("user", C.c_void_p),
+                ("apply_bc", C.c_void_p)]
 
 
 _lib = None
@@ -239,7 +246,14 @@ _PROTOS = [
     ("pihm_lsm_get_state", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("pihm_lsm_step", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     ("pihm_lsm_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]),
-    ("pihm_run_gpu_lsm", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p, C.POINTER(RunStats)]),
+    ("pihm_run_gpu_lsm", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p,
+                                   C.POINTER(RunStats)]),
+    ("pihm_bc_create", C.c_int, [C.c_void_p, c_int32_p, C.POINTER(Ts), c_int32_p, C.POINTER(Ts),
+                                 C.POINTER(C.c_void_p)]),
+    ("pihm_bc_destroy", None, [C.c_void_p]),
+    ("pihm_bc_apply", C.c_int, [C.c_void_p, C.c_double]),
+    ("pihm_ctx_get_step_inputs", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                           c_double_p]),
 ]
 
 

@@ -102,9 +102,18 @@ def read_riv(path):
     nmat = int(rows[k][1])
     mat = np.array([[float(v) for v in r[1:6]] for r in rows[k + 2:k + 2 + nmat]])
     k += 2 + nmat
-    if int(rows[k][1]) != 0:
-        raise ValueError("river boundary-condition time series are not supported")
-    return dict(fromnode=body[:, 1], tonode=body[:, 2], down=body[:, 3], left=body[:, 4], right=body[:, 5],
+    nbc = int(rows[k][1])
+    series = None
+    if nbc > 0:
+        # RIV_TS blocks up to RES or the end (src/read_river.c:118-175)
+        lines = _data_lines(path)
+        at = next(i for i, ln in enumerate(lines) if ln.split()[0] == "BC")
+        blocks, _ = _ts_blocks(lines, at + 1, "RIV_TS", 1, stop=("RES",))
+        if len(blocks) != nbc:
+            raise ValueError("river boundary series count does not match BC")
+        _equal_length(blocks, "river boundary series")
+        series = (np.stack([b[1] for b in blocks]), np.stack([b[2][:, 0] for b in blocks]))
+    return dict(bc_series=series, fromnode=body[:, 1], tonode=body[:, 2], down=body[:, 3], left=body[:, 4], right=body[:, 5],
                 shape=body[:, 6], matl=body[:, 7], bc=body[:, 8], shp=shp, mat=mat)
 
 
@@ -122,30 +131,51 @@ def str_time(s):
     return float(calendar.timegm(time.strptime(s, "%Y-%m-%d %H:%M")))
 
 
-def read_ts_blocks(path, key, nvar):
-    """Time-series blocks of a forcing file (ReadForc, src/read_forc.c:3-80; ReadLai,
-    src/read_lai.c): each block opens with a ``<key> <index> ...`` line and two header
-    lines, then one record per line, ``YYYY-MM-DD HH:MM v1 ... vn``.  Returns a list of
-    (header fields, ftime[nrec], data[nrec, nvar])."""
+def _ts_blocks(lines, i, key, nvar, stop=()):
+    """Time-series blocks from lines[i:]: each opens with ``<key> <index> ...`` and two
+    header lines, then one record per line, ``YYYY-MM-DD HH:MM v1 ... vn``, up to the next
+    block, a line starting with a ``stop`` keyword, or the end.  Returns (blocks, next i)
+    with blocks a list of (header fields, ftime[nrec], data[nrec, nvar])."""
     blocks = []
-    with open(path) as f:
-        lines = [ln.rstrip("\n") for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
-    i = 0
-    while i < len(lines):
+    while i < len(lines) and not lines[i].split()[0] in stop:
         head = lines[i].split()
         if head[0] != key:
-            raise ValueError(f"{path}: expected {key} at '{lines[i]}'")
+            raise ValueError(f"expected {key} at '{lines[i]}'")
         if int(head[1]) != len(blocks) + 1:
-            raise ValueError(f"{path}: {key} blocks out of order")
+            raise ValueError(f"{key} blocks out of order")
         i += 3
         ft, rows = [], []
-        while i < len(lines) and not lines[i].startswith(key):
+        while i < len(lines) and lines[i].split()[0] not in (key,) + tuple(stop):
             tok = lines[i].split()
             ft.append(str_time(tok[0] + " " + tok[1]))
             rows.append([float(v) for v in tok[2:2 + nvar]])
             i += 1
-        blocks.append((head, np.array(ft), np.array(rows)))
-    return blocks
+        blocks.append((head, np.array(ft), np.array(rows).reshape(len(ft), nvar)))
+    return blocks, i
+
+
+def _data_lines(path):
+    with open(path) as f:
+        return [ln.rstrip("\n") for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+
+
+def read_ts_blocks(path, key, nvar):
+    """Every ``key`` block of a forcing file (ReadForc, src/read_forc.c:3-80; ReadLai,
+    src/read_lai.c; ReadBc, src/read_bc.c)."""
+    return _ts_blocks(_data_lines(path), 0, key, nvar)[0]
+
+
+def _equal_length(blocks, what):
+    if len({len(b[1]) for b in blocks}) > 1:
+        raise ValueError(f"{what} with different record counts are not supported")
+
+
+def read_bc(path):
+    """Boundary-condition series of a ``.bc`` file (BC_TS blocks, src/read_bc.c):
+    (ftime[nts, nrec], value[nts, nrec])."""
+    blocks = read_ts_blocks(path, "BC_TS", 1)
+    _equal_length(blocks, "boundary series")
+    return np.stack([b[1] for b in blocks]), np.stack([b[2][:, 0] for b in blocks])
 
 
 def read_meteo(path):
@@ -154,8 +184,7 @@ def read_meteo(path):
     land-surface step reads (src/forcing.c:340-349).  Every station must hold the same
     number of records (the device layout is one [nstation, nrec] table)."""
     blocks = read_ts_blocks(path, "METEO_TS", 5)
-    if len({len(b[1]) for b in blocks}) != 1:
-        raise ValueError("stations with different record counts are not supported")
+    _equal_length(blocks, "stations")
     ftime = np.stack([b[1] for b in blocks])
     data = np.stack([b[2] for b in blocks])
     zlvl = np.array([float(b[0][3]) for b in blocks])
@@ -165,8 +194,7 @@ def read_meteo(path):
 def read_lai(path):
     """LAI forcing of a ``.lai`` file: (ftime[nts, nrec], lai[nts, nrec])."""
     blocks = read_ts_blocks(path, "LAI_TS", 1)
-    if len({len(b[1]) for b in blocks}) != 1:
-        raise ValueError("LAI series with different record counts are not supported")
+    _equal_length(blocks, "LAI series")
     return np.stack([b[1] for b in blocks]), np.stack([b[2][:, 0] for b in blocks])
 
 

@@ -125,22 +125,28 @@ class HipLSM:
         return out
 
 
-def run_gpu_lsm(mesh, params, meteo, veg, cfg, y0, cmc0=None, sneqv0=None):
+def run_gpu_lsm(mesh, params, meteo, veg, cfg, y0, cmc0=None, sneqv0=None, bc=None):
     """Integrate on the device with the land-surface step as the forcing
-    (pihm_run_gpu_lsm).  ``veg`` is (lai, z0, meltf).  Returns (rc, y, stats, cmc, sneqv)."""
+    (pihm_run_gpu_lsm).  ``veg`` is (lai, z0, meltf); ``bc`` the HipBC arguments
+    (elem_type, elem_series, river_type, river_series) or None.  Returns
+    (rc, y, stats, cmc, sneqv)."""
+    from .bc import HipBC
     from .rhs import HipRHS
     h = HipRHS(mesh, nan_check=False)
     lsm = HipLSM(h, params, meteo)
+    hb = HipBC(h, *bc) if bc is not None else None
     try:
         lsm.set_veg(*veg)
         ne = mesh.nelem
         lsm.set_state(np.zeros(ne) if cmc0 is None else cmc0, np.zeros(ne) if sneqv0 is None else sneqv0)
         y = np.ascontiguousarray(y0, dtype=np.float64).copy()
         st = _lib.RunStats()
-        rc = h.L.pihm_run_gpu_lsm(h.ctx, lsm.h, C.byref(cfg), _dp(y), C.byref(st))
+        rc = h.L.pihm_run_gpu_lsm(h.ctx, lsm.h, hb.h if hb else None, C.byref(cfg), _dp(y), C.byref(st))
         cmc, sneqv = lsm.get_state()
         return rc, y, st, cmc, sneqv
     finally:
+        if hb:
+            hb.close()
         lsm.close()
         h.close()
 
