@@ -34,6 +34,13 @@ BYTES_PER_SEG = 200      # ... per river segment
 # calibration copy in the same run: FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288
 # KiB copied), WRITE_SIZE x 1.000.  (44,752.5 x 2 + 17,737.5) KiB.
 RHS_TRAFFIC_BYTES = (44752.5 * 2.0 + 17737.5) * 1024
+# Land-surface step (k_intcp_snow_et), algorithmic bytes per element: 15 FP64 parameters
+# (albedo, shdfac, cmcfactr, cfactr, rzd, rgl, rsmin, rsmax, topt, depth, porosity, smcmin,
+# smcwlt, smcref, surface pressure: 120) + station index (4) + lai, the aerodynamic
+# resistance's numerator and the carried cmc, sneqv (32) + unsat, gw of the state (16)
+# read; cmc, sneqv, ec, drip, pcpdrp, edir, ett (56) written.
+LSM_BYTES_PER_ELEM = 228
+LSM_STATIONS = 64
 
 
 def _cfg(t_end):
@@ -242,6 +249,64 @@ def closure_run(days):
             "sim_days_per_s": days / wall, "nst": st.nst}
 
 
+def _producers(mesh, seed):
+    """A single-solve context with the land-surface step (64 stations, 48 hourly records)
+    and boundary series on every boundary edge and outlet (16 + 1 series) bound to it."""
+    from pihm_amd.bc import HipBC
+    from pihm_amd.lsm import HipLSM, synthetic_meteo, synthetic_params
+    from pihm_amd.rhs import HipRHS
+    ne = mesh.nelem
+    h = HipRHS(mesh, nan_check=False)
+    p = synthetic_params(ne, LSM_STATIONS, seed=seed)
+    for k in ("depth", "porosity", "rzd", "zmax"):
+        p[k] = mesh.a[k].copy()
+    p["smcmin"] = np.minimum(p["smcmin"], 0.5 * p["porosity"])
+    lsm = HipLSM(h, p, synthetic_meteo(LSM_STATIONS, 48, seed=seed))
+    rng = np.random.default_rng(seed)
+    lsm.set_veg(rng.uniform(0.0, 4.0, ne), rng.uniform(0.01, 0.5, ne), 1e-7)
+    lsm.set_state(np.zeros(ne), rng.uniform(0.0, 0.01, ne))
+    bnd = mesh.a["nabr"].reshape(ne, 3) == 0
+    code = np.where(bnd, 1 + np.arange(3 * ne).reshape(ne, 3) % 16, 0).astype(np.int32)
+    rcode = np.where(mesh.a["down"] < 0, -1, 0).astype(np.int32)
+    ft = np.tile(3600.0 * np.arange(48.0), (16, 1))
+    bc = HipBC(h, code, (ft, rng.uniform(100.0, 200.0, ft.shape)), rcode,
+               (ft[:1], rng.uniform(0.01, 0.1, (1, 48))))
+    y0 = mesh.initial_state()
+    assert lsm.step(3600.0 * 5.5, 900.0, y0) == 0
+    ncoded = int(bnd.sum()) + int((rcode != 0).sum())
+    bc_bytes = 3 * ne * 4 + mesh.nriver * 4 + 8 * ncoded
+    return h, lsm, bc, bc_bytes
+
+
+def producers_run(mesh, seed, iters=200):
+    """The per-step producers on the device at the 200,000-triangle workload, timed with
+    HIP events: one land-surface step (k_meteo + k_intcp_snow_et) and one boundary-series
+    apply (k_bc_interp + k_bc_scatter)."""
+    L = _lib.lib()
+    h, lsm, bc, bc_bytes = _producers(mesh, seed)
+    ms_l, ms_b = C.c_double(), C.c_double()
+    t = 3600.0 * 12.25
+    assert L.pihm_lsm_time(lsm.h, t, 900.0, h.y, 5, C.byref(ms_l)) == 0
+    assert L.pihm_lsm_time(lsm.h, t, 900.0, h.y, iters, C.byref(ms_l)) == 0
+    assert L.pihm_bc_time(bc.h, t, 5, C.byref(ms_b)) == 0
+    assert L.pihm_bc_time(bc.h, t, iters, C.byref(ms_b)) == 0
+    ne = mesh.nelem
+    lsm_bytes = LSM_BYTES_PER_ELEM * ne
+    ach = lsm_bytes / (ms_l.value * 1e-3) / 1e9
+    bc.close()
+    lsm.close()
+    h.close()
+    return {
+        "lsm_step": {"ms": ms_l.value, "elements": ne, "stations": LSM_STATIONS,
+                     "roofline": {"kernel": "k_meteo + k_intcp_snow_et (one LSM step)", "bound": "hbm",
+                                  "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": lsm_bytes}},
+        "bc_apply": {"ms": ms_b.value, "algorithmic_bytes": bc_bytes,
+                     "achieved_GBps": bc_bytes / (ms_b.value * 1e-3) / 1e9,
+                     "note": "two launches over 600k edge codes: launch-latency bound"},
+    }
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -275,6 +340,13 @@ def profile_kernels(iters, seed):
     ms_eval = C.c_double()
     L.pihm_ctx_time_rhs(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), iters,
                         C.byref(ms_eval))
+    h, lsm, bc, _ = _producers(mesh, seed)
+    ms_l, ms_b = C.c_double(), C.c_double()
+    L.pihm_lsm_time(lsm.h, 3600.0 * 12.25, 900.0, h.y, iters, C.byref(ms_l))
+    L.pihm_bc_time(bc.h, 3600.0 * 12.25, iters, C.byref(ms_b))
+    bc.close()
+    lsm.close()
+    h.close()
     n = 64 << 20
     a = L.N_VNew_Hip(n)
     b = L.N_VNew_Hip(n)
@@ -286,7 +358,9 @@ def profile_kernels(iters, seed):
         L.N_VDestroy_Hip(v)
     L.pihm_ctx_destroy(ctx)
     print(json.dumps({"profile_kernels": iters, "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
-                      "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8}))
+                      "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8,
+                      "lsm_ms_per_step_events": ms_l.value, "bc_ms_per_apply_events": ms_b.value,
+                      "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem}))
 
 
 def main():
@@ -377,6 +451,7 @@ def main():
             gs = g["marks"][S] - g["marks"][0.0]
             out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
+    out["producers"] = producers_run(mesh, args.seed)
     if not args.no_closure:
         out["closure"] = closure_run(args.closure_days)
     if not args.no_ensemble:

@@ -48,6 +48,11 @@ void pihm_bc_destroy(pihm_bc *bc);
  * exits, when t lies outside a series (src/forcing.c:487-497). */
 int  pihm_bc_apply(pihm_bc *bc, double t);
 
+/* Measurement: iters back-to-back applies at t on the context's stream, timed
+ * with HIP events; *ms_per_apply = elapsed / iters.  No counterpart in the
+ * reference. */
+int  pihm_bc_time(pihm_bc *bc, double t, int iters, double *ms_per_apply);
+
 #ifdef __cplusplus
 }
 #endif

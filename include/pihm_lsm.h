@@ -79,6 +79,11 @@ int  pihm_lsm_get_state(pihm_lsm *lsm, double *cmc, double *sneqv);
  * records (src/forcing.c:487-497). */
 int  pihm_lsm_step(pihm_lsm *lsm, double t, double stepsize, N_Vector y);
 
+/* Measurement: iters back-to-back steps at t on the context's stream, timed
+ * with HIP events; *ms_per_step = elapsed / iters.  The carried storages
+ * advance by iters steps.  No counterpart in the reference. */
+int  pihm_lsm_time(pihm_lsm *lsm, double t, double stepsize, N_Vector y, int iters, double *ms_per_step);
+
 /* The last step's pcpdrp, edir, ett, ec (canopy evaporation) and drip,
  * caller order; any pointer may be NULL.  Synchronises the stream. */
 int  pihm_lsm_get_fluxes(pihm_lsm *lsm, double *pcpdrp, double *edir, double *ett, double *ec, double *drip);

@@ -146,18 +146,39 @@ void pihm_bc_destroy(pihm_bc *b) {
   delete b;
 }
 
-int pihm_bc_apply(pihm_bc *b, double t) {
-  if (!b) return PIHM_ERR_ARG;
+static void launch_apply(pihm_bc *b, double t) {
   const int nts = b->nts_e + b->nts_r;
-  if (nts == 0) return PIHM_OK;
-  if (t < b->t_lo || t > b->t_hi) return PIHM_ERR_ARG;
   pihm_ctx *c = b->ctx;
   hipLaunchKernelGGL(k_bc_interp, dim3(div_up(nts, kBlock)), dim3(kBlock), 0, c->stream, b->nts_e, b->nrec_e,
                      b->ft_e, b->v_e, b->nts_r, b->nrec_r, b->ft_r, b->v_r, t, b->val);
   const long n = b->n_edge + b->nr;
   hipLaunchKernelGGL(k_bc_scatter, dim3((unsigned)div_up(n, (long)kBlock)), dim3(kBlock), 0, c->stream, b->n_edge,
                      b->nr, b->code_e, b->code_r, b->nts_e, b->val, c->args.e.ebc, c->args.r.bc);
+}
+
+int pihm_bc_apply(pihm_bc *b, double t) {
+  if (!b) return PIHM_ERR_ARG;
+  if (b->nts_e + b->nts_r == 0) return PIHM_OK;
+  if (t < b->t_lo || t > b->t_hi) return PIHM_ERR_ARG;
+  launch_apply(b, t);
   PIHM_HIP_TRY(hipGetLastError());
+  return PIHM_OK;
+}
+
+int pihm_bc_time(pihm_bc *b, double t, int iters, double *ms_per_apply) {
+  if (!b || iters <= 0 || !ms_per_apply || b->nts_e + b->nts_r == 0) return PIHM_ERR_ARG;
+  if (t < b->t_lo || t > b->t_hi) return PIHM_ERR_ARG;
+  hipEvent_t ev[2];
+  for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
+  PIHM_HIP_TRY(hipEventRecord(ev[0], b->ctx->stream));
+  for (int it = 0; it < iters; it++) launch_apply(b, t);
+  PIHM_HIP_TRY(hipGetLastError());
+  PIHM_HIP_TRY(hipEventRecord(ev[1], b->ctx->stream));
+  PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+  float ms = 0;
+  PIHM_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms_per_apply = ms / iters;
   return PIHM_OK;
 }
 

@@ -6,8 +6,12 @@
 // then one thread per element evaluates interception, snow and
 // evapotranspiration (IntcpSnowEt, src/is_sm_et.c:5-226) and writes the
 // RHS's per-step inputs pcpdrp, edir and ett into the context.  Every
-// expression keeps the reference's operation order; the transcendental
-// functions are the device library's (exp, log, pow, cos).
+// expression keeps the reference's operation order, with two changes of form:
+// terms that depend only on static parameters or on the vegetation inputs
+// (albedo, surface pressure, the aerodynamic resistance's numerator) are
+// evaluated once, when those are set, with the same expressions; and integer
+// powers (T^4, T^2, betas^2) are products.  exp, log, pow and cos are the
+// device library's.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,6 +39,8 @@ struct LsmDev {
   double *albedomin, *albedomax, *shdfac, *cmcfactr, *cfactr, *zlvl_wind, *rzd;
   double *rgl, *rsmin, *rsmax, *topt;
   double *depth, *porosity, *smcmin, *smcwlt, *smcref, *zmax;
+  double *albedo, *pres;    // from albedomin/albedomax and zmax, once at creation
+  double *ra_num;           // log(zlvl / z0) * log(10 zlvl / z0), at every vegetation update
   int32_t *station;
   double *lai, *z0;
   double *cmc, *sneqv;      // carried
@@ -48,6 +54,22 @@ __global__ __launch_bounds__(kBlock) void k_meteo(int nstation, int nrec, const 
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= nstation) return;
   ts_interp<PIHM_MET_NVAR>(ftime + (long)k * nrec, data + (long)k * nrec * PIHM_MET_NVAR, nrec, t, met + k, nstation);
+}
+
+// Per-element values that change only with the parameters or the vegetation
+// inputs, with the reference's expressions (src/is_sm_et.c:47,55-56,95-96).
+__global__ __launch_bounds__(kBlock) void k_lsm_static(int ne, LsmDev l) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  l.albedo[p] = 0.5 * (l.albedomin[p] + l.albedomax[p]);
+  l.pres[p] = 101.325 * 1.0e3 * pow((293.0 - 0.0065 * l.zmax[p]) / 293.0, 5.26);
+}
+
+__global__ __launch_bounds__(kBlock) void k_lsm_veg(int ne, LsmDev l) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const double z0 = l.z0[p], zlvl = l.zlvl_wind[p];
+  l.ra_num[p] = log(zlvl / z0) * log(10.0 * zlvl / z0);
 }
 
 struct StepArgs {
@@ -77,13 +99,13 @@ __global__ __launch_bounds__(kBlock) void k_intcp_snow_et(StepArgs a) {
   const double shdfac = l.shdfac[p], depth = l.depth[p], rzd = l.rzd[p];
   double cmc = l.cmc[p], sneqv = l.sneqv[p];
 
-  const double albedo = 0.5 * (l.albedomin[p] + l.albedomax[p]);
+  const double albedo = l.albedo[p];
   const double radnet = soldn * (1.0 - albedo);
   const double sfctmp = sfctmp_k - 273.15;
   const double wind = sfcspd;
   const double rh = rh_pct / 100.0;
   const double vp = 611.2 * exp(17.67 * sfctmp / (sfctmp + 243.5)) * rh;
-  const double pres = 101.325 * 1.0e3 * pow((293.0 - 0.0065 * l.zmax[p]) / 293.0, 5.26);
+  const double pres = l.pres[p];
   const double qv = 0.622 * vp / pres;
   const double qvsat = 0.622 * (vp / rh) / pres;
   const double lai = l.lai[p];
@@ -104,11 +126,11 @@ __global__ __launch_bounds__(kBlock) void k_intcp_snow_et(StepArgs a) {
 
   // throughfall and evaporation from the canopy
   const double intcp_max = l.cmcfactr[p] * lai * shdfac;
-  const double z0 = l.z0[p];
-  const double zlvl = l.zlvl_wind[p];
-  const double ra = log(zlvl / z0) * log(10.0 * zlvl / z0) / (wind * 0.16);
-  const double gamma = 4.0 * 0.7 * SIGMA * RD / CP * pow(sfctmp + 273.15, 4.0) / (pres / ra) + 1.0;
-  const double delta = LVH2O * LVH2O * 0.622 / RV / CP / pow(sfctmp + 273.15, 2.0) * qvsat;
+  const double ra = l.ra_num[p] / (wind * 0.16);
+  // pow(T, 4) and pow(T, 2) as products (within an ulp of the library's)
+  const double tk = sfctmp + 273.15, tk2 = tk * tk;
+  const double gamma = 4.0 * 0.7 * SIGMA * RD / CP * (tk2 * tk2) / (pres / ra) + 1.0;
+  const double delta = LVH2O * LVH2O * 0.622 / RV / CP / tk2 * qvsat;
   const double etp = (radnet * delta + gamma * (1.2 * LVH2O * (qvsat - qv) / ra)) / (1000.0 * LVH2O * (delta + gamma));
 
   double satn;
@@ -121,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_intcp_snow_et(StepArgs a) {
   }
   double betas = (satn * l.porosity[p] + l.smcmin[p] - l.smcwlt[p]) / (l.smcref[p] - l.smcwlt[p]);
   betas = (betas < 0.0001) ? 0.0001 : ((betas > 1.0) ? 1.0 : betas);
-  double edir = (1.0 - shdfac) * pow(betas, 2.0) * etp;
+  double edir = (1.0 - shdfac) * (betas * betas) * etp;
   edir *= a.cal_edir;
   edir = (edir < 0.0) ? 0.0 : edir;
 
@@ -136,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_intcp_snow_et(StepArgs a) {
     const double rsmin = l.rsmin[p], rsmax = l.rsmax[p];
     double alphar = (1.0 + fr) / (fr + (rsmin / rsmax));
     alphar = (alphar > 10000.0) ? 10000.0 : alphar;
-    double etas = 1.0 - 0.0016 * (pow((l.topt[p] - 273.15 - sfctmp), 2.0));
+    const double dt_opt = l.topt[p] - 273.15 - sfctmp;
+    double etas = 1.0 - 0.0016 * (dt_opt * dt_opt);
     etas = (etas < 0.0001) ? 0.0001 : etas;
     double gammas = 1.0 / (1.0 + 0.00025 * (vp / rh - vp));
     gammas = (gammas < 0.01) ? 0.01 : gammas;
@@ -257,7 +280,8 @@ int pihm_lsm_create(pihm_ctx *ctx, const pihm_lsm_desc *desc, const pihm_meteo *
   LsmDev &D = l->d;
   double **fields[] = {&D.albedomin, &D.albedomax, &D.shdfac, &D.cmcfactr, &D.cfactr, &D.zlvl_wind, &D.rzd,
                        &D.rgl, &D.rsmin, &D.rsmax, &D.topt, &D.depth, &D.porosity, &D.smcmin, &D.smcwlt,
-                       &D.smcref, &D.zmax, &D.lai, &D.z0, &D.cmc, &D.sneqv, &D.ec, &D.drip};
+                       &D.smcref, &D.zmax, &D.lai, &D.z0, &D.cmc, &D.sneqv, &D.ec, &D.drip,
+                       &D.albedo, &D.pres, &D.ra_num};
   const int nf = (int)(sizeof(fields) / sizeof(fields[0]));
   const size_t fbytes = ((size_t)ne * sizeof(double) + 255) & ~size_t(255);
   const size_t sbytes = ((size_t)ne * sizeof(int32_t) + 255) & ~size_t(255);
@@ -284,6 +308,10 @@ int pihm_lsm_create(pihm_ctx *ctx, const pihm_lsm_desc *desc, const pihm_meteo *
             hipSuccess)
       rc = PIHM_ERR_HIP;
   }
+  if (!rc) {
+    hipLaunchKernelGGL(k_lsm_static, dim3(div_up(ne, kBlock)), dim3(kBlock), 0, ctx->stream, ne, l->d);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PIHM_ERR_HIP;
+  }
   if (rc) { pihm_lsm_destroy(l); return rc; }
   l->t_first.resize(ns);
   l->t_last.resize(ns);
@@ -306,6 +334,8 @@ int pihm_lsm_set_veg(pihm_lsm *l, const double *lai, const double *z0, double me
   int rc = upload_elem(l, l->d.lai, lai);
   if (!rc) rc = upload_elem(l, l->d.z0, z0);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_lsm_veg, dim3(div_up(l->ne, kBlock)), dim3(kBlock), 0, l->ctx->stream, l->ne, l->d);
+  PIHM_HIP_TRY(hipGetLastError());
   l->meltf = meltf;
   l->veg_set = 1;
   return PIHM_OK;
@@ -324,11 +354,15 @@ int pihm_lsm_get_state(pihm_lsm *l, double *cmc, double *sneqv) {
   return rc;
 }
 
-int pihm_lsm_step(pihm_lsm *l, double t, double stepsize, N_Vector y) {
+static int check_step(pihm_lsm *l, double t, double stepsize, N_Vector y) {
   if (!l || !y || !(stepsize > 0.0) || !l->veg_set) return PIHM_ERR_ARG;
   if (N_VGetLength_Hip(y) != l->ctx->neq) return PIHM_ERR_ARG;
   for (int k = 0; k < l->nstation; k++)
     if (t < l->t_first[k] || t > l->t_last[k]) return PIHM_ERR_ARG;
+  return PIHM_OK;
+}
+
+static void launch_step(pihm_lsm *l, double t, double stepsize, N_Vector y) {
   hipStream_t st = l->ctx->stream;
   hipLaunchKernelGGL(k_meteo, dim3(div_up(l->nstation, kBlock)), dim3(kBlock), 0, st, l->nstation, l->nrec,
                      l->ftime, l->data, t, l->d.met);
@@ -346,7 +380,29 @@ int pihm_lsm_step(pihm_lsm *l, double t, double stepsize, N_Vector y) {
   a.edir = l->ctx->args.e.edir;
   a.ett = l->ctx->args.e.ett;
   hipLaunchKernelGGL(k_intcp_snow_et, dim3(div_up(l->ne, kBlock)), dim3(kBlock), 0, st, a);
+}
+
+int pihm_lsm_step(pihm_lsm *l, double t, double stepsize, N_Vector y) {
+  if (int rc = check_step(l, t, stepsize, y)) return rc;
+  launch_step(l, t, stepsize, y);
   PIHM_HIP_TRY(hipGetLastError());
+  return PIHM_OK;
+}
+
+int pihm_lsm_time(pihm_lsm *l, double t, double stepsize, N_Vector y, int iters, double *ms_per_step) {
+  if (iters <= 0 || !ms_per_step) return PIHM_ERR_ARG;
+  if (int rc = check_step(l, t, stepsize, y)) return rc;
+  hipEvent_t ev[2];
+  for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
+  PIHM_HIP_TRY(hipEventRecord(ev[0], l->ctx->stream));
+  for (int it = 0; it < iters; it++) launch_step(l, t, stepsize, y);
+  PIHM_HIP_TRY(hipGetLastError());
+  PIHM_HIP_TRY(hipEventRecord(ev[1], l->ctx->stream));
+  PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+  float ms = 0;
+  PIHM_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms_per_step = ms / iters;
   return PIHM_OK;
 }
 

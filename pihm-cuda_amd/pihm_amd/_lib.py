@@ -252,6 +252,8 @@ _PROTOS = [
                                  C.POINTER(C.c_void_p)]),
     ("pihm_bc_destroy", None, [C.c_void_p]),
     ("pihm_bc_apply", C.c_int, [C.c_void_p, C.c_double]),
+    ("pihm_bc_time", C.c_int, [C.c_void_p, C.c_double, C.c_int, c_double_p]),
+    ("pihm_lsm_time", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int, c_double_p]),
     ("pihm_ctx_get_step_inputs", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
                                            c_double_p]),
 ]
