@@ -4,7 +4,7 @@
  * A CVODE-shaped C API covering what MM-PIHM uses (src/ode.c:413-661,
  * src/main.c:83-116): create, init/reinit, scalar tolerances, user data,
  * initial/max step, max steps, stop time, SPGMR without preconditioning
- * (maxl default 5), CV_NORMAL / CV_ONE_STEP solves and the counters read
+ * (maxl default 5) or with a right preconditioner, CV_NORMAL / CV_ONE_STEP solves and the counters read
  * by AdjCVodeMaxStep and PrintPerf.  Method: fixed-leading-coefficient BDF
  * of order 1..5 in Nordsieck form with the error test, step/order selection,
  * Newton iteration and difference-quotient Jacobian-vector products of
@@ -36,6 +36,7 @@ extern "C" {
 #define PIHM_CV_TOO_MUCH_ACC  -2
 #define PIHM_CV_ERR_FAILURE   -3
 #define PIHM_CV_CONV_FAILURE  -4
+#define PIHM_CV_LSETUP_FAIL   -6
 #define PIHM_CV_LSOLVE_FAIL   -7
 #define PIHM_CV_RHSFUNC_FAIL  -8
 #define PIHM_CV_FIRST_RHSFUNC_ERR -9
@@ -49,6 +50,17 @@ extern "C" {
 #define PIHM_CV_BAD_T         -25
 
 typedef int (*PihmRhsFn)(realtype t, N_Vector y, N_Vector ydot, void *user_data);
+
+/* Preconditioning of SPGMR: the values of sundials_iterative.h:57 and the
+ * callback shapes of CVSpilsPrecSetupFn / CVSpilsPrecSolveFn
+ * (cvode/include/cvode/cvode_spils.h:162-222).  Only right preconditioning
+ * is provided. */
+#define PIHM_PREC_NONE  0
+#define PIHM_PREC_RIGHT 2
+typedef int (*PihmPrecSetupFn)(realtype t, N_Vector y, N_Vector fy, int jok, int *jcurPtr, realtype gamma,
+                               void *user_data, N_Vector tmp1, N_Vector tmp2, N_Vector tmp3);
+typedef int (*PihmPrecSolveFn)(realtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z, realtype gamma,
+                               realtype delta, int lr, void *user_data, N_Vector tmp);
 
 void *PihmCVodeCreate(int lmm, int iter);
 int   PihmCVodeInit(void *mem, PihmRhsFn f, realtype t0, N_Vector y0);
@@ -70,7 +82,15 @@ int   PihmCVodeSetStabLimDet(void *mem, int onoff);
  * (or if capture fails) it runs the same steps launch by launch.  Results are
  * the same bits either way.  No counterpart in CVODE. */
 int   PihmCVodeSetGraphRhs(void *mem, int onoff);
+/* pretype PIHM_PREC_NONE (MM-PIHM's choice, src/ode.c:523) or PIHM_PREC_RIGHT. */
 int   PihmCVSpgmr(void *mem, int pretype, int maxl);
+/* CVSpilsSetPreconditioner (cvode_spils.h): the callbacks get the user data.
+ * With right preconditioning the setup runs when CVODE would call lsetup
+ * (cvode.c:2651-2727, cvode_spgmr.c:289-330).  With device vectors the
+ * difference-quotient increment of J*P^-1*v is the unpreconditioned one
+ * (sqrt(N) for the unit basis vector), and the preconditioner solve must only
+ * enqueue device work on the vectors' stream when graphs are on. */
+int   PihmCVSpilsSetPreconditioner(void *mem, PihmPrecSetupFn pset, PihmPrecSolveFn psolve);
 int   PihmCVode(void *mem, realtype tout, N_Vector yout, realtype *tret, int itask);
 int   PihmCVodeGetDky(void *mem, realtype t, int k, N_Vector dky);
 void  PihmCVodeFree(void **mem);
@@ -86,6 +106,9 @@ int PihmCVodeGetLastStep(void *mem, realtype *hlast);
 int PihmCVodeGetLastOrder(void *mem, int *qlast);
 int PihmCVodeGetCurrentTime(void *mem, realtype *tcur);
 int PihmCVodeGetCurrentStep(void *mem, realtype *hcur);
+int PihmCVodeGetNumLinSolvSetups(void *mem, long int *nlinsetups);
+int PihmCVSpilsGetNumPrecEvals(void *mem, long int *npevals);
+int PihmCVSpilsGetNumPrecSolves(void *mem, long int *npsolves);
 
 #ifdef __cplusplus
 }

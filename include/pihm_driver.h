@@ -36,6 +36,9 @@ typedef struct pihm_run_config {
   int maxl;                     /* SPGMR dimension (0 = 5) */
   long max_steps_per_call;      /* CVodeSetMaxNumSteps; 0 = stepsize*10 as the reference */
   int graphs;                   /* pihm_run_gpu: replay Krylov iterations from HIP graphs */
+  int precond;                  /* 0: SPGMR without preconditioning, as the reference
+                                   (src/ode.c:523); 1: right block-Jacobi preconditioner
+                                   from the backend's prec_setup / prec_solve */
 } pihm_run_config;
 
 typedef struct pihm_forcing {
@@ -51,6 +54,7 @@ typedef struct pihm_run_stats {
   long model_steps;
   double wall_s;
   int flag;                     /* last integrator flag */
+  long nsetups, npe, nps;       /* preconditioner setups, Jacobian evaluations, solves */
 } pihm_run_stats;
 
 /* Backend: the right-hand side plus the per-step producers. */
@@ -63,6 +67,8 @@ typedef struct pihm_backend {
   void *user;
   int (*apply_bc)(void *user, double t);        /* every model step before the forcing
                                                    (ApplyBc, src/pihm.c:17-24); may be NULL */
+  PihmPrecSetupFn prec_setup;                   /* used when cfg->precond; called with rhs_data */
+  PihmPrecSolveFn prec_solve;
 } pihm_backend;
 
 int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_config *cfg,

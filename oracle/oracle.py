@@ -56,6 +56,14 @@ def lib():
         ("oracle_backend_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.POINTER(P.Forcing), C.c_int]),
         ("oracle_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
         ("oracle_backend_free", None, [C.c_void_p]),
+        ("oracle_backend_enable_precond", None, [C.c_void_p, C.c_long]),
+        ("oracle_backend_bj", C.c_void_p, [C.c_void_p]),
+        ("oracle_bj_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.c_long]),
+        ("oracle_bj_free", None, [C.c_void_p]),
+        ("oracle_bj_ncolor", C.c_int, [C.c_void_p]),
+        ("oracle_bj_colors", ip, [C.c_void_p]),
+        ("oracle_bj_jac", dp, [C.c_void_p]),
+        ("oracle_bj_jacobian", C.c_int, [C.c_void_p, C.c_double, dp]),
         ("oracle_meteo_interp", C.c_int, [C.POINTER(P.Meteo), C.c_double, dp]),
         ("oracle_lsm_backend_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.POINTER(P.LsmDesc),
                                                 C.POINTER(P.Meteo), dp, dp, C.c_double, dp, dp, C.c_double]),
@@ -183,7 +191,7 @@ def init_geometry(mesh):
 class OracleRun:
     """The model-step driver (product integrator) over the oracle RHS on host vectors."""
 
-    def __init__(self, mesh, forcing, nthreads=1):
+    def __init__(self, mesh, forcing, nthreads=1, precond=False):
         self.L = lib()
         self.PL = P.lib()
         self.mesh = mesh
@@ -193,6 +201,8 @@ class OracleRun:
         self.fs = forcing.struct()
         self.be = self.L.oracle_backend_new(C.byref(self.desc), self.st, C.byref(self.fs), nthreads)
         self.nthreads = nthreads
+        if precond:
+            self.L.oracle_backend_enable_precond(self.be, mesh.neq)
 
     def run(self, y0, cfg):
         n = self.mesh.neq

@@ -17,9 +17,18 @@ N_Vector oracle_nv_new(long int length, int nthreads);
 void oracle_nv_destroy(N_Vector v);
 double *oracle_nv_data(N_Vector v);
 
+typedef struct oracle_bj oracle_bj;
+oracle_bj *oracle_bj_new(const pihm_mesh_desc *m, oracle_state *s, long neq);
+void oracle_bj_free(oracle_bj *p);
+int oracle_bj_psetup(realtype t, N_Vector y, N_Vector fy, int jok, int *jcur, realtype gamma, void *ud,
+                     N_Vector t1, N_Vector t2, N_Vector t3);
+int oracle_bj_psolve(realtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z, realtype gamma, realtype delta,
+                     int lr, void *ud, N_Vector tmp);
+
 typedef struct oracle_backend {
     const pihm_mesh_desc *m;
     oracle_state *s;
+    oracle_bj *bj;
     int nstation, nrec;
     double rec_dt;
     double *prcp, *edir, *ett;
@@ -91,11 +100,35 @@ oracle_backend *oracle_backend_new(const pihm_mesh_desc *m, oracle_state *s,
     return b;
 }
 
+/* the RHS's user data is the backend: hand the preconditioner its own data */
+static int backend_psetup(realtype t, N_Vector y, N_Vector fy, int jok, int *jcur, realtype gamma, void *ud,
+                          N_Vector t1, N_Vector t2, N_Vector t3)
+{
+    return oracle_bj_psetup(t, y, fy, jok, jcur, gamma, ((oracle_backend *)ud)->bj, t1, t2, t3);
+}
+
+static int backend_psolve(realtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z, realtype gamma,
+                          realtype delta, int lr, void *ud, N_Vector tmp)
+{
+    return oracle_bj_psolve(t, y, fy, r, z, gamma, delta, lr, ((oracle_backend *)ud)->bj, tmp);
+}
+
+/* Block-Jacobi preconditioner for runs with cfg->precond (neq: the state length) */
+void oracle_backend_enable_precond(oracle_backend *b, long neq)
+{
+    if (!b->bj) b->bj = oracle_bj_new(b->m, b->s, neq);
+    b->be.prec_setup = backend_psetup;
+    b->be.prec_solve = backend_psolve;
+}
+
+oracle_bj *oracle_backend_bj(oracle_backend *b) { return b->bj; }
+
 const pihm_backend *oracle_backend_get(oracle_backend *b) { return &b->be; }
 
 void oracle_backend_free(oracle_backend *b)
 {
     if (!b) return;
+    oracle_bj_free(b->bj);
     free(b->prcp); free(b->edir); free(b->ett); free(b->station);
     free(b);
 }

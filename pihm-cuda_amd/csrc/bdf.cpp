@@ -40,6 +40,13 @@ struct Mem {
   double tstop = 0.0;
   int maxl = 5;
   double eplifac = 0.05;
+  // right preconditioning (PihmCVSpilsSetPreconditioner)
+  int pretype = 0;
+  PihmPrecSetupFn pset = nullptr;
+  PihmPrecSolveFn psolve = nullptr;
+  bool jcur = false;
+  long nstlp = 0, nstlpre = 0, npe = 0, nps = 0, nsetups = 0;
+  double delta_ls = 0.0;
   bool fused = false;
   // Krylov iterations replayed from graphs (device vectors on a non-null
   // stream and an RHS declared capturable, PihmCVodeSetGraphRhs)
@@ -81,6 +88,20 @@ inline N_Vector VClone(N_Vector w) { return w->ops->nvclone(w); }
 inline void VDestroy(N_Vector v) { if (v) v->ops->nvdestroy(v); }
 
 bool is_hip(N_Vector v) { return v->ops->nvgetvectorid == N_VGetVectorID_Hip; }
+
+// cv_setupNonNull (cvode_spgmr.c CVSpgmr / CVSpilsSetPreconditioner)
+bool setup_nonnull(const Mem *m) { return m->pretype != PIHM_PREC_NONE && m->pset != nullptr; }
+bool pre_right(const Mem *m) { return m->pretype == PIHM_PREC_RIGHT && m->psolve != nullptr; }
+
+// CVSpilsPSolve (cvode_spils.c:639-655): z = P^-1 r, right side
+int psolve(Mem *m, N_Vector r, N_Vector z) {
+  const int rv = m->psolve(m->tn, m->ycur, m->fcur, r, z, m->gamma, m->delta_ls, PIHM_PREC_RIGHT, m->ud, m->ytemp);
+  m->nps++;
+  return rv;
+}
+
+// Newton's retry with fresh preconditioner data (cvode.c:2760-2830)
+inline int retry_or(const Mem *m, int flag) { return (!m->jcur && setup_nonnull(m)) ? TRY_AGAIN : flag; }
 
 void free_graphs(Mem *m) {
   for (auto &g : m->gx) {
@@ -278,10 +299,16 @@ int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli) {
   VScale(1.0 / r_norm, V[0], V[0]);
   bool converged = false;
   int krydim = 0;
+  const bool pre = pre_right(m);
   for (int l = 0; l < m->maxl; l++) {
     (*nli)++;
     krydim = l + 1;
     VDiv(V[l], s, m->vtemp);
+    if (pre) {
+      VScale(1.0, m->vtemp, V[l + 1]);
+      const int ps = psolve(m, V[l + 1], m->vtemp);
+      if (ps != 0) return (ps < 0) ? SP_PSOLVE_FAIL_UNREC : SP_PSOLVE_FAIL_REC;
+    }
     const int ier = atimes(m, m->vtemp, V[l + 1]);
     if (ier != 0) return (ier < 0) ? SP_ATIMES_FAIL_UNREC : SP_ATIMES_FAIL_REC;
     VProd(s, V[l + 1], V[l + 1]);
@@ -298,7 +325,13 @@ int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli) {
   for (int k = 0; k < krydim; k++) VLin(m->yg[k], V[k], 1.0, m->xcor, m->xcor);
   if (converged || rho < beta) {
     VDiv(m->xcor, s, m->xcor);
-    VLin(1.0, x, 1.0, m->xcor, x);
+    if (pre) {
+      const int ps = psolve(m, m->xcor, m->vtemp);
+      if (ps != 0) return (ps < 0) ? SP_PSOLVE_FAIL_UNREC : SP_PSOLVE_FAIL_REC;
+      VLin(1.0, x, 1.0, m->vtemp, x);
+    } else {
+      VLin(1.0, x, 1.0, m->xcor, x);
+    }
     return converged ? SP_SUCCESS : SP_RES_REDUCED;
   }
   return SP_CONV_FAIL;
@@ -315,6 +348,7 @@ int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
   m->ycur = ynow;
   m->fcur = fnow;
   const double delta = deltar * m->sqrtN;
+  m->delta_ls = delta;
   VConst(0.0, m->xs);
   int nli = 0;
   const int rv = spgmr(m, m->xs, b, delta, &nli);
@@ -323,9 +357,25 @@ int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
   switch (rv) {
     case SP_SUCCESS: return 0;
     case SP_RES_REDUCED: return (m->mnewt == 0) ? 0 : 1;
-    case SP_CONV_FAIL: case SP_QRFACT_FAIL: case SP_ATIMES_FAIL_REC: return 1;
+    case SP_CONV_FAIL: case SP_QRFACT_FAIL: case SP_ATIMES_FAIL_REC: case SP_PSOLVE_FAIL_REC: return 1;
     default: return -1;
   }
+}
+
+// CVSpgmrSetup (cvode_spgmr.c:289-330)
+int lsetup(Mem *m, int convfail, N_Vector ypred, N_Vector fpred) {
+  const double dgamma = std::fabs((m->gamma / m->gammap) - 1.0);
+  const bool jbad = (m->nst == 0) || (m->nst > m->nstlpre + SPILS_MSBPRE) ||
+                    ((convfail == CV_FAIL_BAD_J) && (dgamma < SPILS_DGMAX)) || (convfail == CV_FAIL_OTHER);
+  int jcur = jbad ? 1 : 0;
+  const int rv = m->pset(m->tn, ypred, fpred, jbad ? 0 : 1, &jcur, m->gamma, m->ud, m->acor, m->y, m->tempv);
+  if (jbad) jcur = 1;
+  if (jcur) {
+    m->npe++;
+    m->nstlpre = m->nst;
+  }
+  m->jcur = jcur != 0;
+  return rv;
 }
 
 // ---------------------------------------------------------------- fused (device vectors)
@@ -346,9 +396,18 @@ bool build_graph(Mem *m, int l) {
   if (!st || !m->kry) { m->graph_off = true; return false; }
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) { m->graph_off = true; return false; }
-  int bad = pihm_gm_prep_dev(l, m->V, m->kry, m->y, m->ewt, m->sqrtN, m->ytemp);
+  const bool pre = pre_right(m);
+  int bad = pihm_gm_prep_dev(l, m->V, m->kry, m->y, m->ewt, m->sqrtN, m->ytemp, pre ? m->xs : nullptr);
+  if (!bad && pre) {
+    // the preconditioner's solve is captured with the iteration: its data is
+    // read from device memory when the graph replays
+    bad = psolve(m, m->xs, m->xcor);
+    m->nps--;
+    if (!bad) VLin(m->sqrtN, m->xcor, 1.0, m->y, m->ytemp);
+  }
   if (!bad) bad = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
-  if (!bad) bad = pihm_gm_arnoldi(l, m->V, m->ewt, m->vtemp, m->ftemp, 1.0 / m->sqrtN, 0.0, nullptr, m->kry);
+  if (!bad) bad = pihm_gm_arnoldi(l, m->V, m->ewt, m->vtemp, m->ftemp, 1.0 / m->sqrtN, 0.0, nullptr, m->kry,
+                                  pre ? m->xcor : nullptr);
   const hipError_t ec = hipStreamEndCapture(st, &g);
   if (bad || ec != hipSuccess || !g) {
     if (g) (void)hipGraphDestroy(g);
@@ -379,6 +438,7 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
   bool converged = false;
   int krydim = 0;
   bool graph = m->graph_rhs && !m->graph_off && m->kry;   // gamma was left in kry by pihm_gm_start
+  const bool pre = pre_right(m);
   for (int l = 0; l < m->maxl; l++) {
     krydim = l + 1;
     double sums[MAXL_CAP + 2];
@@ -388,16 +448,23 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
       hipStream_t st = (hipStream_t)pihm_nv_stream(s);
       if (hipGraphLaunch(m->gx[l], st) != hipSuccess) return SP_ATIMES_FAIL_UNREC;
       m->nfes++;
+      if (pre) m->nps++;
       pihm_redwork w = pihm_nv_redwork(s);
       if (pihm_red_wait(w, st)) return SP_ATIMES_FAIL_UNREC;
       for (int k = 0; k <= l + 1; k++) sums[k] = w.result[k];
     } else {
-      if (pihm_gm_prep(l, V, lazy, cscale, m->ycur, s, sig, m->ytemp)) return SP_ATIMES_FAIL_UNREC;
+      if (pihm_gm_prep(l, V, lazy, cscale, m->ycur, s, sig, m->ytemp, pre ? m->xs : nullptr)) return SP_ATIMES_FAIL_UNREC;
+      if (pre) {
+        const int ps = psolve(m, m->xs, m->xcor);
+        if (ps != 0) return (ps < 0) ? SP_PSOLVE_FAIL_UNREC : SP_PSOLVE_FAIL_REC;
+        VLin(sig, m->xcor, 1.0, m->ycur, m->ytemp);
+      }
       const int rv = m->f(m->tn, m->ytemp, m->vtemp, m->ud);
       m->nfes++;
       if (rv < 0) return SP_ATIMES_FAIL_UNREC;
       if (rv > 0) return SP_ATIMES_FAIL_REC;
-      if (pihm_gm_arnoldi(l, V, s, m->vtemp, m->fcur, siginv, m->gamma, sums, m->kry)) return SP_ATIMES_FAIL_UNREC;
+      if (pihm_gm_arnoldi(l, V, s, m->vtemp, m->fcur, siginv, m->gamma, sums, m->kry, pre ? m->xcor : nullptr))
+        return SP_ATIMES_FAIL_UNREC;
     }
     const double ww = sums[l + 1];
     double hsq = 0.0;
@@ -452,21 +519,32 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
     } else {
       m->ycur = m->y;
       m->fcur = m->ftemp;
+      m->delta_ls = deltar * m->sqrtN;
       const int sp = spgmr_fused(m, sumsq, deltar * m->sqrtN, &krydim);
       m->nli += krydim;
       switch (sp) {
         case SP_ZERO: mode = 2; break;
         case SP_SUCCESS: mode = 0; break;
         case SP_RES_REDUCED: mode = 0; lret = (m->mnewt == 0) ? 0 : 1; break;
-        case SP_CONV_FAIL: case SP_QRFACT_FAIL: case SP_ATIMES_FAIL_REC: lret = 1; break;
+        case SP_CONV_FAIL: case SP_QRFACT_FAIL: case SP_ATIMES_FAIL_REC: case SP_PSOLVE_FAIL_REC: lret = 1; break;
         default: lret = -1; break;
       }
     }
     m->nni++;
     if (lret < 0) return PIHM_CV_LSOLVE_FAIL;
-    if (lret > 0) return CONV_FAIL;
+    if (lret > 0) return retry_or(m, CONV_FAIL);
     double sums[2];
-    if (pihm_gm_finish(mode, krydim, m->yg, m->V, m->ewt, m->tempv, m->zn[0], m->acor, m->y, sums))
+    const double *xsrc = nullptr;
+    if (mode == 0 && pre_right(m)) {
+      // x = P^-1 (sum_j y_j V_j / s), then as the unpreconditioned finish
+      if (pihm_gm_combine(krydim, m->yg, m->V, m->ewt, m->xs)) return PIHM_CV_LSOLVE_FAIL;
+      const int ps = psolve(m, m->xs, m->xcor);
+      if (ps < 0) return PIHM_CV_LSOLVE_FAIL;
+      if (ps > 0) return retry_or(m, CONV_FAIL);
+      mode = 3;
+      xsrc = N_VGetDeviceArrayPointer_Hip(m->xcor);
+    }
+    if (pihm_gm_finish(mode, krydim, m->yg, m->V, m->ewt, m->tempv, m->zn[0], m->acor, m->y, xsrc, sums))
       return PIHM_CV_LSOLVE_FAIL;
     del = std::sqrt(sums[0] / N);
     if (mi > 0) m->crate = std::fmax(CRDOWN * m->crate, del / delp);
@@ -476,12 +554,12 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
       return PIHM_CV_SUCCESS;
     }
     m->mnewt = ++mi;
-    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return CONV_FAIL;
+    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
     delp = del;
     const int fr = m->f(m->tn, m->y, m->ftemp, m->ud);
     m->nfe++;
     if (fr < 0) return PIHM_CV_RHSFUNC_FAIL;
-    if (fr > 0) return RHSFUNC_RECVR;
+    if (fr > 0) return retry_or(m, RHSFUNC_RECVR);
   }
 }
 
@@ -500,7 +578,7 @@ int newton_iteration(Mem *m) {   // cvode.c:2729-2832
     const int rv = lsolve(m, b, m->y, m->ftemp);
     m->nni++;
     if (rv < 0) return PIHM_CV_LSOLVE_FAIL;
-    if (rv > 0) return CONV_FAIL;
+    if (rv > 0) return retry_or(m, CONV_FAIL);
     del = VWrms(b, m->ewt);
     VLin(1.0, m->acor, 1.0, b, m->acor);
     VLin(1.0, m->zn[0], 1.0, m->acor, m->y);
@@ -511,24 +589,46 @@ int newton_iteration(Mem *m) {   // cvode.c:2729-2832
       return PIHM_CV_SUCCESS;
     }
     m->mnewt = ++mi;
-    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return CONV_FAIL;
+    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
     delp = del;
     const int fr = m->f(m->tn, m->y, m->ftemp, m->ud);
     m->nfe++;
     if (fr < 0) return PIHM_CV_RHSFUNC_FAIL;
-    if (fr > 0) return RHSFUNC_RECVR;
+    if (fr > 0) return retry_or(m, RHSFUNC_RECVR);
   }
 }
 
-int nls_newton(Mem *m) {   // cvode.c:2651-2727 (no setup routine: PREC_NONE)
-  m->crate = 1.0;
-  const int rv = m->f(m->tn, m->zn[0], m->ftemp, m->ud);
-  m->nfe++;
-  if (rv < 0) return PIHM_CV_RHSFUNC_FAIL;
-  if (rv > 0) return RHSFUNC_RECVR;
-  VConst(0.0, m->acor);
-  VScale(1.0, m->zn[0], m->y);
-  return m->fused ? newton_iteration_fused(m) : newton_iteration(m);
+int nls_newton(Mem *m, int nflag) {   // cvode.c:2651-2727
+  int convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? CV_NO_FAILURES : CV_FAIL_OTHER;
+  bool call_setup = false;
+  if (setup_nonnull(m)) {
+    call_setup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (m->nst == 0) ||
+                 (m->nst >= m->nstlp + MSBP) || (std::fabs(m->gamrat - 1.0) > DGMAX);
+  } else {
+    m->crate = 1.0;
+  }
+  for (;;) {
+    const int rv = m->f(m->tn, m->zn[0], m->ftemp, m->ud);
+    m->nfe++;
+    if (rv < 0) return PIHM_CV_RHSFUNC_FAIL;
+    if (rv > 0) return RHSFUNC_RECVR;
+    if (call_setup) {
+      const int ier = lsetup(m, convfail, m->zn[0], m->ftemp);
+      m->nsetups++;
+      call_setup = false;
+      m->gamrat = m->crate = 1.0;
+      m->gammap = m->gamma;
+      m->nstlp = m->nst;
+      if (ier < 0) return PIHM_CV_LSETUP_FAIL;
+      if (ier > 0) return CONV_FAIL;
+    }
+    VConst(0.0, m->acor);
+    VScale(1.0, m->zn[0], m->y);
+    const int ier = m->fused ? newton_iteration_fused(m) : newton_iteration(m);
+    if (ier != TRY_AGAIN) return ier;
+    call_setup = true;
+    convfail = CV_FAIL_BAD_J;
+  }
 }
 
 int handle_nflag(Mem *m, int *nflag, double saved_t, int *ncf) {   // cvode.c:2834-2880
@@ -536,6 +636,7 @@ int handle_nflag(Mem *m, int *nflag, double saved_t, int *ncf) {   // cvode.c:28
   if (nf == PIHM_CV_SUCCESS) return DO_ERROR_TEST;
   m->ncfn++;
   restore(m, saved_t);
+  if (nf == PIHM_CV_LSETUP_FAIL) return PIHM_CV_LSETUP_FAIL;
   if (nf == PIHM_CV_LSOLVE_FAIL) return PIHM_CV_LSOLVE_FAIL;
   if (nf == PIHM_CV_RHSFUNC_FAIL) return PIHM_CV_RHSFUNC_FAIL;
   (*ncf)++;
@@ -682,7 +783,7 @@ int step(Mem *m) {   // cvode.c:2005-2066
   for (;;) {
     predict(m);
     set_bdf(m);
-    nflag = nls_newton(m);
+    nflag = nls_newton(m, nflag);
     const int kflag = handle_nflag(m, &nflag, saved_t, &ncf);
     if (kflag == PREDICT_AGAIN) continue;
     if (kflag != DO_ERROR_TEST) return kflag;
@@ -746,6 +847,8 @@ int PihmCVodeInit(void *mem, PihmRhsFn f, realtype t0, N_Vector y0) {   // cvode
   m->qu = 0; m->hu = 0.0; m->tolsf = 1.0;
   VScale(1.0, y0, m->zn[0]);
   m->nst = m->nfe = m->ncfn = m->netf = m->nni = m->nli = m->nfes = m->nhnil = m->nscon = 0;
+  m->nstlp = m->nstlpre = m->npe = m->nps = m->nsetups = 0;
+  m->jcur = false;
   m->h0u = 0.0; m->next_h = 0.0; m->next_q = 0;
   m->indx_acor = m->qmax;
   for (double &t : m->tau) t = 0.0;
@@ -765,6 +868,8 @@ int PihmCVodeReInit(void *mem, realtype t0, N_Vector y0) {   // cvode.c:580-675
   m->qu = 0; m->hu = 0.0; m->tolsf = 1.0;
   VScale(1.0, y0, m->zn[0]);
   m->nst = m->nfe = m->ncfn = m->netf = m->nni = m->nli = m->nfes = m->nhnil = m->nscon = 0;
+  m->nstlp = m->nstlpre = m->npe = m->nps = m->nsetups = 0;
+  m->jcur = false;
   m->h0u = 0.0; m->next_h = 0.0; m->next_q = 0;
   m->indx_acor = m->qmax;
   for (double &t : m->tau) t = 0.0;
@@ -821,12 +926,22 @@ int PihmCVodeSetGraphRhs(void *mem, int onoff) {
   free_graphs(m);
   return PIHM_CV_SUCCESS;
 }
+int PihmCVSpilsSetPreconditioner(void *mem, PihmPrecSetupFn pset, PihmPrecSolveFn psolve) {
+  if (!mem) return PIHM_CV_MEM_NULL;
+  Mem *m = (Mem *)mem;
+  m->pset = pset;
+  m->psolve = psolve;
+  free_graphs(m);
+  return PIHM_CV_SUCCESS;
+}
 int PihmCVodeSetStabLimDet(void *mem, int onoff) { if (!mem) return PIHM_CV_MEM_NULL; ((Mem *)mem)->sldet = onoff != 0; return 0; }
 
 int PihmCVSpgmr(void *mem, int pretype, int maxl) {
   if (!mem) return PIHM_CV_MEM_NULL;
-  if (pretype != 0) return PIHM_CV_ILL_INPUT;   // only PREC_NONE, as MM-PIHM (src/ode.c:523)
+  if (pretype != PIHM_PREC_NONE && pretype != PIHM_PREC_RIGHT) return PIHM_CV_ILL_INPUT;
   Mem *m = (Mem *)mem;
+  m->pretype = pretype;
+  free_graphs(m);
   m->maxl = (maxl <= 0) ? 5 : (maxl > MAXL_CAP ? MAXL_CAP : maxl);
   m->ls_set = true;
   if (m->zn[0]) {
@@ -1014,6 +1129,9 @@ GETTER(PihmCVodeGetLastStep, realtype, hu)
 GETTER(PihmCVodeGetLastOrder, int, qu)
 GETTER(PihmCVodeGetCurrentTime, realtype, tn)
 GETTER(PihmCVodeGetCurrentStep, realtype, next_h)
+GETTER(PihmCVodeGetNumLinSolvSetups, long int, nsetups)
+GETTER(PihmCVSpilsGetNumPrecEvals, long int, npe)
+GETTER(PihmCVSpilsGetNumPrecSolves, long int, nps)
 #undef GETTER
 
 }  // extern "C"

@@ -34,8 +34,16 @@ constexpr int DQ_MAX_ITERS = 3;        // cvode_spils.c:36
 enum { DO_ERROR_TEST = 2, PREDICT_AGAIN = 3, CONV_FAIL = 4, TRY_AGAIN = 5, FIRST_CALL = 6,
        PREV_CONV_FAIL = 7, PREV_ERR_FAIL = 8, RHSFUNC_RECVR = 9 };
 // SPGMR outcomes (sundials_spgmr.h)
-enum { SP_SUCCESS = 0, SP_RES_REDUCED = 1, SP_CONV_FAIL = 2, SP_QRFACT_FAIL = 3,
-       SP_ATIMES_FAIL_REC = 5, SP_ATIMES_FAIL_UNREC = -2, SP_GS_FAIL = -4, SP_QRSOL_FAIL = -5 };
+enum { SP_SUCCESS = 0, SP_RES_REDUCED = 1, SP_CONV_FAIL = 2, SP_QRFACT_FAIL = 3, SP_PSOLVE_FAIL_REC = 4,
+       SP_ATIMES_FAIL_REC = 5, SP_ATIMES_FAIL_UNREC = -2, SP_PSOLVE_FAIL_UNREC = -3, SP_GS_FAIL = -4,
+       SP_QRSOL_FAIL = -5 };
+// linear-solver setup decisions: cvode.c:237-240 (MSBP, DGMAX), cvode_spils.h:69-70
+// (CVSPILS_MSBPRE, CVSPILS_DGMAX), convfail values cvode_impl.h
+constexpr int MSBP = 20;
+constexpr double DGMAX = 0.3;
+constexpr long SPILS_MSBPRE = 50;
+constexpr double SPILS_DGMAX = 0.2;
+enum { CV_NO_FAILURES = 0, CV_FAIL_BAD_J = 1, CV_FAIL_OTHER = 2 };
 
 template <class M>
 void set_tq_bdf(M *m, double hsum, double alpha0, double alpha0_hat, double xi_inv,

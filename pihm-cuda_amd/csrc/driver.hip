@@ -38,7 +38,10 @@ extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_con
   if (!flag) flag = PihmCVodeSetMaxStep(cv, maxstep);
   if (!flag) flag = PihmCVodeSetMaxNumSteps(cv, cfg->max_steps_per_call > 0 ? cfg->max_steps_per_call
                                                                             : (long)(cfg->stepsize * 10));
-  if (!flag) flag = PihmCVSpgmr(cv, 0, cfg->maxl);
+  const bool pre = cfg->precond && be->prec_setup && be->prec_solve;
+  if (cfg->precond && !pre) flag = PIHM_ERR_ARG;
+  if (!flag) flag = PihmCVSpgmr(cv, pre ? PIHM_PREC_RIGHT : PIHM_PREC_NONE, cfg->maxl);
+  if (!flag && pre) flag = PihmCVSpilsSetPreconditioner(cv, be->prec_setup, be->prec_solve);
   if (!flag && cfg->graphs && be->rhs == gpu_rhs_fn()) flag = PihmCVodeSetGraphRhs(cv, 1);
   if (flag) { PihmCVodeFree(&cv); return flag; }
 
@@ -88,6 +91,9 @@ extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_con
   PihmCVodeGetNumLinIters(cv, &local.nli);
   PihmCVodeGetNumNonlinSolvConvFails(cv, &local.ncfn);
   PihmCVodeGetNumErrTestFails(cv, &local.netf);
+  PihmCVodeGetNumLinSolvSetups(cv, &local.nsetups);
+  PihmCVSpilsGetNumPrecEvals(cv, &local.npe);
+  PihmCVSpilsGetNumPrecSolves(cv, &local.nps);
   PihmCVodeFree(&cv);
   local.flag = flag;
   local.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0w).count();

@@ -43,6 +43,7 @@ struct PrepArgs {
   double coef[kMaxBasis];
   const double *y, *s;
   double *work;
+  double *vt_out;                                   // right preconditioning: V_l / s, no work vector
 };
 
 __global__ void k_prep(long n, PrepArgs a) {
@@ -53,7 +54,8 @@ __global__ void k_prep(long n, PrepArgs a) {
   v = a.c * v;                                      // Scale(1/h, V_l)
   a.vl[i] = v;
   const double vt = v / a.s[i];                     // Div(V_l, s2)
-  a.work[i] = (a.sig * vt) + a.y[i];                // LinearSum(sig, v, 1, y)
+  if (a.vt_out) a.vt_out[i] = vt;
+  else a.work[i] = (a.sig * vt) + a.y[i];           // LinearSum(sig, v, 1, y)
 }
 
 // k_prep with its coefficients read from the device copy of the previous
@@ -79,12 +81,14 @@ __global__ void k_prep_dev(long n, int l, const double *kry, PrepArgs a) {
   v = c * v;
   a.vl[i] = v;
   const double vt = v / a.s[i];
-  a.work[i] = (a.sig * vt) + a.y[i];
+  if (a.vt_out) a.vt_out[i] = vt;
+  else a.work[i] = (a.sig * vt) + a.y[i];
 }
 
 template <int NV>
 struct FArnoldi {
   const double *vl, *s, *jf, *fy;
+  const double *zp;                                 // preconditioned vector (right preconditioning)
   double siginv, gamma;
   const double *gp;                                 // gamma from device memory when set
   double *w;
@@ -92,7 +96,7 @@ struct FArnoldi {
   __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const {
     const double g = gp ? *gp : gamma;
     const double si = s[i];
-    const double vt = vl[i] / si;
+    const double vt = zp ? zp[i] : vl[i] / si;      // the vector atimes is applied to
     const double jv = siginv * (jf[i] - fy[i]);     // LinearSum(siginv, Jv, -siginv, fy)
     const double z = (-g * jv) + vt;                // LinearSum(1, v, -gamma, Jv)
     const double wi = si * z;                       // Prod(s1, z)
@@ -123,11 +127,13 @@ struct FFinish {
   int mode, k;
   double yc[kMaxBasis];
   const double *vk[kMaxBasis];
-  const double *s, *b, *zn0;
+  const double *s, *b, *zn0, *xsrc;
   double *acor, *y;
   __device__ __forceinline__ void operator()(long i, double (&acc)[2]) const {
     double x;
-    if (mode == 0) {
+    if (mode == 3) {
+      x = 0.0 + xsrc[i];                                     // x = 0 + P^-1 (xcor / s2)
+    } else if (mode == 0) {
       double xc = 0.0;
       for (int j = 0; j < k; j++) xc += yc[j] * vk[j][i];   // LinearSum(yg, V, 1, xcor, xcor)
       xc = xc / s[i];                                        // Div(xcor, s2)
@@ -147,12 +153,31 @@ struct FFinish {
   }
 };
 
+// xcor = sum_j y_j V_j (LinearSum(y_j, V_j, 1, xcor, xcor) from xcor = 0), then
+// Div(xcor, s2): the operand of the right preconditioner's final solve.
+struct CombineArgs {
+  int k;
+  double yc[kMaxBasis];
+  const double *vk[kMaxBasis];
+  const double *s;
+  double *out;
+};
+
+__global__ void k_combine(long n, CombineArgs a) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xc = 0.0;
+  for (int j = 0; j < a.k; j++) xc += a.yc[j] * a.vk[j][i];
+  a.out[i] = xc / a.s[i];
+}
+
 // sums == nullptr: launch only (gamma from kry, nothing awaited)
 template <int NV>
 int arnoldi_n(N_Vector *V, int l, N_Vector s, N_Vector jf, N_Vector fy, double siginv, double gamma,
-              double *sums, double *kry) {
+              double *sums, double *kry, N_Vector z) {
   FArnoldi<NV> f;
   f.vl = D(V[l]); f.s = D(s); f.jf = D(jf); f.fy = D(fy);
+  f.zp = z ? D(z) : nullptr;
   f.siginv = siginv; f.gamma = gamma;
   f.gp = sums ? nullptr : kry + PIHM_KRY_GAMMA;
   f.w = D(V[l + 1]);
@@ -186,46 +211,49 @@ int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vecto
                       kry ? kry + PIHM_KRY_GAMMA : nullptr, gamma);
 }
 
-int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig, N_Vector work) {
+int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig, N_Vector work,
+                     N_Vector vt) {
   if (l < 0 || l >= kMaxBasis) return -1;
   PrepArgs a;
   a.l = l; a.c = 0.0; a.sig = sig;
   a.vl = D(V[l]);
   for (int k = 0; k < l; k++) { a.vk[k] = D(V[k]); a.coef[k] = 0.0; }
   a.y = D(y); a.s = D(s); a.work = D(work);
+  a.vt_out = vt ? D(vt) : nullptr;
   const long n = L(s);
   hipLaunchKernelGGL(k_prep_dev, dim3(div_up(n, kBlock)), dim3(kBlock), 0, S(s), n, l, kry, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s, double sig,
-                 N_Vector work) {
+                 N_Vector work, N_Vector vt) {
   if (l < 0 || l >= kMaxBasis) return -1;
   PrepArgs a;
   a.l = l; a.c = c; a.sig = sig;
   a.vl = D(V[l]);
   for (int k = 0; k < l; k++) { a.vk[k] = D(V[k]); a.coef[k] = coef[k]; }
   a.y = D(y); a.s = D(s); a.work = D(work);
+  a.vt_out = vt ? D(vt) : nullptr;
   const long n = L(s);
   hipLaunchKernelGGL(k_prep, dim3(div_up(n, kBlock)), dim3(kBlock), 0, S(s), n, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
-                    double gamma, double *sums, double *kry) {
+                    double gamma, double *sums, double *kry, N_Vector z) {
   if (!sums && !kry) return -1;
   switch (l + 2) {
-    case 2: return arnoldi_n<2>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 3: return arnoldi_n<3>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 4: return arnoldi_n<4>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 5: return arnoldi_n<5>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 6: return arnoldi_n<6>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 7: return arnoldi_n<7>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 8: return arnoldi_n<8>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 9: return arnoldi_n<9>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 10: return arnoldi_n<10>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 11: return arnoldi_n<11>(V, l, s, jf, fy, siginv, gamma, sums, kry);
-    case 12: return arnoldi_n<12>(V, l, s, jf, fy, siginv, gamma, sums, kry);
+    case 2: return arnoldi_n<2>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 3: return arnoldi_n<3>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 4: return arnoldi_n<4>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 5: return arnoldi_n<5>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 6: return arnoldi_n<6>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 7: return arnoldi_n<7>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 8: return arnoldi_n<8>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 9: return arnoldi_n<9>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 10: return arnoldi_n<10>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 11: return arnoldi_n<11>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
+    case 12: return arnoldi_n<12>(V, l, s, jf, fy, siginv, gamma, sums, kry, z);
     default: return -1;
   }
 }
@@ -248,13 +276,24 @@ int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums, double *k
 }
 
 int pihm_gm_finish(int mode, int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector b,
-                   N_Vector zn0, N_Vector acor, N_Vector y, double *sums) {
-  if (k < 0 || k > kMaxBasis) return -1;
+                   N_Vector zn0, N_Vector acor, N_Vector y, const double *xsrc, double *sums) {
+  if (k < 0 || k > kMaxBasis || (mode == 3 && !xsrc)) return -1;
   FFinish f;
   f.mode = mode; f.k = k;
   for (int j = 0; j < k; j++) { f.yc[j] = ycoef[j]; f.vk[j] = D(V[j]); }
-  f.s = D(s); f.b = D(b); f.zn0 = D(zn0); f.acor = D(acor); f.y = D(y);
+  f.s = D(s); f.b = D(b); f.zn0 = D(zn0); f.acor = D(acor); f.y = D(y); f.xsrc = xsrc;
   return fused_sum<2>(L(s), f, pihm_nv_redwork(s), S(s), sums);
+}
+
+int pihm_gm_combine(int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector out) {
+  if (k < 0 || k > kMaxBasis) return -1;
+  CombineArgs a;
+  a.k = k;
+  for (int j = 0; j < k; j++) { a.yc[j] = ycoef[j]; a.vk[j] = D(V[j]); }
+  a.s = D(s); a.out = D(out);
+  const long n = L(s);
+  hipLaunchKernelGGL(k_combine, dim3(div_up(n, kBlock)), dim3(kBlock), 0, S(s), n, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 }  // extern "C"

@@ -51,15 +51,20 @@ int pihm_red_wait(pihm_redwork w, void *stream);
 #define PIHM_KRY_SIZE (PIHM_KRY_START + 4)
 int pihm_gm_start(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_Vector f,
                   N_Vector s, N_Vector b, N_Vector v0, double *sumsq, double *kry);
+// With right preconditioning: prep writes V_l / s into vt (and not the work
+// vector), the caller applies P^-1 and forms the RHS argument, and arnoldi
+// takes the preconditioned vector z for the A*z step.  finish mode 3 takes the
+// correction x from xsrc; combine forms sum_j y_j V_j / s for it.
 int pihm_gm_prep(int l, N_Vector *V, const double *coef, double c, N_Vector y, N_Vector s,
-                 double sig, N_Vector work);
+                 double sig, N_Vector work, N_Vector vt);
 int pihm_gm_prep_dev(int l, N_Vector *V, const double *kry, N_Vector y, N_Vector s, double sig,
-                     N_Vector work);
+                     N_Vector work, N_Vector vt);
 int pihm_gm_arnoldi(int l, N_Vector *V, N_Vector s, N_Vector jf, N_Vector fy, double siginv,
-                    double gamma, double *sums, double *kry);
+                    double gamma, double *sums, double *kry, N_Vector z);
 int pihm_gm_orth(int l, N_Vector *V, const double *coef, double *sums, double *kry);
 int pihm_gm_finish(int mode, int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector b,
-                   N_Vector zn0, N_Vector acor, N_Vector y, double *sums);
+                   N_Vector zn0, N_Vector acor, N_Vector y, const double *xsrc, double *sums);
+int pihm_gm_combine(int k, const double *ycoef, N_Vector *V, N_Vector s, N_Vector out);
 
 // Fused Nordsieck-array updates used by the integrator (one launch each).
 // zn holds q+1 (or more) device vectors of equal length.

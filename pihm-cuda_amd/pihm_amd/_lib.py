@@ -58,7 +58,7 @@ class RunConfig(C.Structure):
                 ("initstep", C.c_double), ("nncfn", C.c_double), ("nnimax", C.c_double),
                 ("nnimin", C.c_double), ("decr", C.c_double), ("incr", C.c_double),
                 ("stmin", C.c_double), ("maxl", C.c_int), ("max_steps_per_call", C.c_long),
-                ("graphs", C.c_int)]
+                ("graphs", C.c_int), ("precond", C.c_int)]
 
 
 class Forcing(C.Structure):
@@ -74,7 +74,8 @@ class RunStats(C.Structure):
 
     _fields_ = [("nst", C.c_long), ("nfe", C.c_long), ("nfes", C.c_long), ("nni", C.c_long),
                 ("nli", C.c_long), ("ncfn", C.c_long), ("netf", C.c_long),
-                ("model_steps", C.c_long), ("wall_s", C.c_double), ("flag", C.c_int)]
+                ("model_steps", C.c_long), ("wall_s", C.c_double), ("flag", C.c_int),
+                ("nsetups", C.c_long), ("npe", C.c_long), ("nps", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +106,12 @@ class Meteo(C.Structure):
 
 
 RhsFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
+PrecSetupFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_double,
+                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+PrecSolveFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                          C.c_double, C.c_int, C.c_void_p, C.c_void_p)
+PIHM_PREC_NONE = 0
+PIHM_PREC_RIGHT = 2
 ForcingFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)
 SummaryFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
 StepEndFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
@@ -115,7 +122,7 @@ class Backend(C.Structure):
 
     _fields_ = [("rhs", C.c_void_p), ("rhs_data", C.c_void_p), ("apply_forcing", C.c_void_p),
                 ("summary", C.c_void_p), ("model_step_end", C.c_void_p), ("user", C.c_void_p),
-                ("apply_bc", C.c_void_p)]
+                ("apply_bc", C.c_void_p), ("prec_setup", C.c_void_p), ("prec_solve", C.c_void_p)]
 
 
 _lib = None
@@ -206,6 +213,10 @@ _PROTOS = [
     ("PihmCVodeSetStabLimDet", C.c_int, [C.c_void_p, C.c_int]),
     ("PihmCVodeSetGraphRhs", C.c_int, [C.c_void_p, C.c_int]),
     ("PihmCVSpgmr", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("PihmCVSpilsSetPreconditioner", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("PihmCVodeGetNumLinSolvSetups", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVSpilsGetNumPrecEvals", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    ("PihmCVSpilsGetNumPrecSolves", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
     ("PihmCVode", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     ("PihmCVodeGetDky", C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
     ("PihmCVodeFree", None, [C.POINTER(C.c_void_p)]),

@@ -56,10 +56,11 @@ class StormForcing:
 
 
 def run_config(t_end, t_start=0.0, stepsize=60.0, etstep=900.0, reltol=1e-3, abstol=1e-4,
-               initstep=5e-5, maxl=0, max_steps_per_call=0, graphs=1):
+               initstep=5e-5, maxl=0, max_steps_per_call=0, graphs=1, precond=0):
     """Driver settings of the reference example (benchmark/input/example/example.para):
     60 s model step, 900 s LSM step, reltol 1e-3, abstol 1e-4, initial step 5e-5,
-    NUM_NONCOV_FAIL 0, MAX_NONLIN_ITER 3, MIN_NONLIN_ITER 1, DECR/INCR 1.2, MIN_MAXSTEP 1."""
+    NUM_NONCOV_FAIL 0, MAX_NONLIN_ITER 3, MIN_NONLIN_ITER 1, DECR/INCR 1.2, MIN_MAXSTEP 1.
+    precond=1 adds the block-Jacobi right preconditioner to SPGMR (the reference has none)."""
     c = _lib.RunConfig()
     c.t_start, c.t_end, c.stepsize, c.etstep = t_start, t_end, stepsize, etstep
     c.reltol, c.abstol, c.initstep = reltol, abstol, initstep
@@ -67,4 +68,5 @@ def run_config(t_end, t_start=0.0, stepsize=60.0, etstep=900.0, reltol=1e-3, abs
     c.maxl = maxl
     c.max_steps_per_call = max_steps_per_call
     c.graphs = graphs
+    c.precond = precond
     return c
