@@ -72,6 +72,11 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     y0 = mesh.initial_state()
     assert L.pihm_ctx_state_to_device(ctx, y0.ctypes.data_as(_lib.c_double_p), y) == 0
     _lib.SummaryFn(be.summary)(be.user, y)        # ws0 = initial surface state
+    # launch shape of the RHS (element-tile placement over the XCDs), chosen on the
+    # initial state before the timed run; the results do not depend on it
+    span, span_ms = C.c_int(), np.zeros(len(_lib.PIHM_TUNE_SPANS))
+    assert L.pihm_ctx_autotune(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), 20,
+                               C.byref(span), span_ms.ctypes.data_as(_lib.c_double_p)) == 0
     marks = {}
     want = {int(round(h * 60)) for h in mark_hours}     # in model steps of 60 s
 
@@ -106,7 +111,9 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     L.pihm_ctx_destroy(ctx)
     L.pihm_stream_destroy(stream)
     return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t2],
-                ms_eval=ms_eval.value, yend=yend)
+                ms_eval=ms_eval.value, yend=yend,
+                tuning={"tile_span": span.value,
+                        "rhs_ms_by_span": dict(zip(_lib.PIHM_TUNE_SPANS, span_ms.round(6).tolist()))})
 
 
 def ensemble_run(nmem, hours, sample, seed):
@@ -123,6 +130,7 @@ def ensemble_run(nmem, hours, sample, seed):
     e = Ensemble(meshes, forc, cfg)
     for k, m in enumerate(meshes):
         e.set_state(k, m.initial_state())
+    tuning = e.autotune(10)          # launch shapes, before the timed run
     t0 = time.perf_counter()
     nfail = e.run(hours * 3600.0)
     wall = time.perf_counter() - t0
@@ -161,7 +169,8 @@ def ensemble_run(nmem, hours, sample, seed):
                                  f"through pihm_run_gpu over the same window; total scaled by "
                                  f"{nmem}/{len(pick)}",
                        "sample_wall_s": float(np.sum(single_wall))},
-        speedup_vs_one_at_a_time=(member_days / wall) / (member_days / est_single_total))
+        speedup_vs_one_at_a_time=(member_days / wall) / (member_days / est_single_total),
+        tuning=tuning)
 
 
 SERVE_CLASSES = [(8192, 48), (32768, 32), (131072, 12)]   # slot capacity (elements) x count
@@ -433,7 +442,7 @@ def main():
                    "nelem": mesh.nelem, "nriver": mesh.nriver, "neq": mesh.neq},
         "solver": {k: st[k] for k in ("nst", "nfe", "nfes", "nni", "nli", "ncfn", "netf")},
         "rhs": {"ms_per_eval": rhs_ms, "evals_per_s": 1e3 / rhs_ms,
-                "ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}},
+                "ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}, "tuning": g["tuning"]},
         "roofline": {"kernel": "one RHS evaluation (k_prep + k_flux, back to back)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": RHS_TRAFFIC_BYTES,

@@ -68,6 +68,26 @@ int  pihm_ens_member_stats(pihm_ensemble *e, int k, pihm_run_stats *st);
  * right-hand-side evaluations since creation. */
 int  pihm_ens_counters(pihm_ensemble *e, long *ticks, long *op_launches, long *rhs_launches);
 
+/* Autotuner of the launch shapes, choices the results do not depend on:
+ * the placement of the batched RHS's element tiles over the XCDs (the
+ * candidates of PIHM_TUNE_SPANS, pihm_ctx_set_tile_span) and the number of
+ * 2,048-entry chunks each workgroup of the vector-operation kernel runs in
+ * turn (1, 2, 4, 8).  It times reps batched RHS evaluations and reps
+ * vector-operation launches (scale and linear sums) over the members that
+ * are ready and not started, keeps the fastest of each, and reports them.
+ * No member may be running; the members' states, carried river-edge fluxes
+ * and NaN flags are left as they were.  pihm_ens_set_tuning sets the two
+ * choices directly.  No counterpart in the reference. */
+#define PIHM_ENS_TUNE_NG 4
+typedef struct pihm_ens_tuning {
+  int tile_span;
+  int ops_chunks_per_wg;
+  double rhs_ms[PIHM_TUNE_NSPAN];
+  double ops_ms[PIHM_ENS_TUNE_NG];
+} pihm_ens_tuning;
+int  pihm_ens_autotune(pihm_ensemble *e, int reps, pihm_ens_tuning *out);
+int  pihm_ens_set_tuning(pihm_ensemble *e, int tile_span, int ops_chunks_per_wg);
+
 /* ---- serving: members that come and go ----------------------------------
  * An ensemble over a slot context (pihm_ctx_create_slots) starts with every
  * slot empty; slot k takes forcing of up to nrec_cap[k] records and

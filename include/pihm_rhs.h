@@ -188,6 +188,21 @@ int  pihm_ctx_time_kernels(pihm_ctx *ctx, const double *y_dev, double *ydot_dev,
  * after the last, on the context's stream. */
 int  pihm_ctx_time_rhs(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters, double *ms_per_eval);
 
+/* Placement of the element tiles over the XCDs: 0 gives each XCD one
+ * contiguous range of Hilbert-ordered tiles; s >= 1 deals groups of s
+ * consecutive tiles to the XCDs in turn.  Results do not depend on it.
+ * pihm_ctx_autotune (single-solve contexts) times reps back-to-back RHS
+ * evaluations at y_dev for each candidate of PIHM_TUNE_SPANS, keeps the
+ * fastest and returns it (and, when ms is not NULL, the ms per evaluation of
+ * each candidate).  The carried river-edge flux is restored afterwards, so the
+ * next call reads what it would have read.  No counterpart in the reference. */
+#define PIHM_TUNE_NSPAN 8
+#define PIHM_TUNE_SPANS {0, 1, 2, 4, 8, 16, 32, 64}
+int  pihm_ctx_set_tile_span(pihm_ctx *ctx, int span);
+int  pihm_ctx_get_tile_span(const pihm_ctx *ctx);
+int  pihm_ctx_autotune(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int reps, int *best_span,
+                       double *ms);
+
 /* Carried state: the overland flux on river edges written at the end of one
  * RHS call and read by Infil in the next (src/river_flow.c:159-182 then
  * src/vert_flow.c:59-62).  Arrays are [3*nelem] in caller order; entries of

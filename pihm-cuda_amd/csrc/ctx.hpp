@@ -117,6 +117,10 @@ struct RhsArgs {
   // prefixes, bm_mem[i] the i-th active member.
   const int *bm, *bm_mem;
   int nact;
+  // Element-tile placement over the XCDs (xcd_tile): 0 places one contiguous
+  // range of tiles on each XCD; s >= 1 deals groups of s consecutive tiles to
+  // the XCDs in turn.  The arithmetic does not depend on it.
+  int tile_span;
 };
 
 }  // namespace pihm
@@ -146,6 +150,9 @@ struct pihm_ctx;
 int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
 // The same over the nact members of a block map (see RhsArgs::bm); nbe and
 // nbr are the map's element and river block totals.
+// Saves (dir 0) or restores (dir 1) the carried river-edge flux of every slot
+// (hid_ovl[0, 2 nr)) to or from the device buffer buf, on the context's stream.
+int pihm_ctx_carried_copy(pihm_ctx *c, double *buf, int dir);
 int pihm_rhs_launch_compact(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan, const int *bm,
                             const int *bm_mem, int nact, int nbe, int nbr);
 

@@ -97,12 +97,16 @@ struct Round {
   const int *rmem;    // [nred] members with a reduction in this round
   const int *rinfo;   // [3*nred] nsum, minfirst, rslot
   int nred;
+  int nchunks;        // chunks of all participants (cpre[np])
+  int g;              // chunks per workgroup, run one after another (tuned; the arithmetic does not depend on it)
 };
 
 // The round's operations (nblocks = chunks of all participants), then the
 // per-member totals of its reductions (a second launch, one workgroup per
 // member with a reduction).
-void launch_ops(const Dev &d, const Round &r, int nblocks, hipStream_t st);
+void launch_ops(const Dev &d, const Round &r, hipStream_t st);
+constexpr int kTuneNG = 4;
+constexpr int kTuneG[kTuneNG] = {1, 2, 4, 8};
 // One-lane kernel writing ++(*ctr) to *flag (host memory) after the work queued before it.
 void launch_signal(unsigned *ctr, volatile unsigned *flag, hipStream_t st);
 

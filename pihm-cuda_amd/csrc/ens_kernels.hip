@@ -370,7 +370,10 @@ __device__ __forceinline__ void op_ws0(const Ctx &x) {
 __global__ __launch_bounds__(kChunkThreads) void k_ens_ops(Dev d, Round r) {
   __shared__ double lds[4 * kSums];
   __shared__ int last;
-  const int b = blockIdx.x;
+  for (int gi = 0; gi < r.g; gi++) {
+  const int b = blockIdx.x * r.g + gi;
+  if (b >= r.nchunks) return;
+  if (gi > 0) __syncthreads();   // the previous chunk's reduction is done with lds
   int lo = 0, hi = r.np - 1;   // cpre[lo] <= b < cpre[lo+1]
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -438,6 +441,7 @@ __global__ __launch_bounds__(kChunkThreads) void k_ens_ops(Dev d, Round r) {
       default: break;
     }
   }
+  }
 }
 
 // Member totals: the chunk partials of member rmem[b] in chunk order.
@@ -485,8 +489,9 @@ void launch_signal(unsigned *ctr, volatile unsigned *flag, hipStream_t st) {
   hipLaunchKernelGGL(k_ens_signal, dim3(1), dim3(64), 0, st, ctr, flag);
 }
 
-void launch_ops(const Dev &d, const Round &r, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_ens_ops, dim3(nblocks), dim3(kChunkThreads), 0, st, d, r);
+void launch_ops(const Dev &d, const Round &r, hipStream_t st) {
+  const int g = r.g > 0 ? r.g : 1;
+  hipLaunchKernelGGL(k_ens_ops, dim3((r.nchunks + g - 1) / g), dim3(kChunkThreads), 0, st, d, r);
   if (r.nred > 0) hipLaunchKernelGGL(k_ens_final, dim3(r.nred), dim3(kChunkThreads), 0, st, d, r);
 }
 

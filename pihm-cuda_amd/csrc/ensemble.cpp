@@ -716,6 +716,7 @@ struct pihm_ensemble {
   unsigned *sig_ctr = nullptr;           // device tick count
   volatile unsigned *sig_flag = nullptr; // pinned host copy
   unsigned sig_expect = 0;
+  int ops_g = 1;               // chunks per workgroup of k_ens_ops (pihm_ens_autotune)
 
   double *P(int v, int m) const { return vecs + (long)v * stride + info[m].yoff; }
   int ensure_stage(size_t bytes);
@@ -871,7 +872,9 @@ int pihm_ensemble::flush() {
       rd.rmem = (const int *)(stage_d + r.off_rmem);
       rd.rinfo = (const int *)(stage_d + r.off_rinfo);
       rd.nred = r.nred;
-      launch_ops(dev, rd, r.nblocks, st);
+      rd.nchunks = r.nblocks;
+      rd.g = ops_g;
+      launch_ops(dev, rd, st);
       PIHM_HIP_TRY(hipGetLastError());
       op_launches++;
     }
@@ -1193,6 +1196,130 @@ void pihm_ens_destroy(pihm_ensemble *e) {
   if (e->stage_h) (void)hipHostFree(e->stage_h);
   (void)hipFree(e->stage_d);
   delete e;
+}
+
+int pihm_ens_set_tuning(pihm_ensemble *e, int tile_span, int g) {
+  if (!e || tile_span < 0 || (g != 1 && g != 2 && g != 4 && g != 8)) return PIHM_ERR_ARG;
+  e->ctx->args.tile_span = tile_span;
+  e->ops_g = g;
+  return PIHM_OK;
+}
+
+int pihm_ens_autotune(pihm_ensemble *e, int reps, pihm_ens_tuning *out) {
+  if (!e || reps <= 0) return PIHM_ERR_ARG;
+  std::vector<int> act;
+  for (int m = 0; m < e->nmem; m++) {
+    if (e->status[m] == ST_RUNNING) return PIHM_ERR_ARG;
+    if (e->status[m] == ST_READY) act.push_back(m);
+  }
+  if (act.empty()) return PIHM_ERR_ARG;
+  pihm_ctx *c = e->ctx;
+  const int na = (int)act.size();
+  // staged tables: the RHS member table and block map, then one round of
+  // vector operations (VTEMP = 1*ZN0; YTEMP = VTEMP - ZN0; YTEMP = .5 YTEMP + .5 VTEMP)
+  std::vector<MemberIO> io(e->nmem);
+  std::memset(io.data(), 0, e->nmem * sizeof(MemberIO));
+  std::vector<int> bm(2 * (na + 1), 0), cpre{0}, obeg{0};
+  std::vector<Op> ops;
+  for (int i = 0; i < na; i++) {
+    const int m = act[i];
+    io[m].y = e->P(ZN0, m);
+    io[m].ydot = e->P(VTEMP, m);
+    io[m].active = 1;
+    const auto &M = c->members[m];
+    bm[i + 1] = bm[i] + div_up(M.ne, kBlock);
+    bm[(na + 1) + i + 1] = bm[(na + 1) + i] + div_up(M.nr, kBlock);
+    Op o{};
+    o.kind = OP_SCALE; o.a0 = ZN0; o.a2 = VTEMP; o.s[0] = 1.0;
+    ops.push_back(o);
+    o = Op{};
+    o.kind = OP_LIN; o.a0 = VTEMP; o.a1 = ZN0; o.a2 = YTEMP; o.s[0] = 1.0; o.s[1] = -1.0;
+    ops.push_back(o);
+    o = Op{};
+    o.kind = OP_LIN; o.a0 = YTEMP; o.a1 = VTEMP; o.a2 = YTEMP; o.s[0] = 0.5; o.s[1] = 0.5;
+    ops.push_back(o);
+    obeg.push_back((int)ops.size());
+    cpre.push_back(cpre.back() + e->info[m].nch);
+  }
+  std::vector<char> buf;
+  auto put = [&](const void *src, size_t bytes) {
+    const size_t off = (buf.size() + 63) / 64 * 64;
+    buf.resize(off + std::max<size_t>(bytes, 1));
+    if (bytes) std::memcpy(buf.data() + off, src, bytes);
+    return off;
+  };
+  const size_t o_io = put(io.data(), io.size() * sizeof(MemberIO));
+  const size_t o_bm = put(bm.data(), bm.size() * sizeof(int));
+  const size_t o_act = put(act.data(), act.size() * sizeof(int));
+  const size_t o_cpre = put(cpre.data(), cpre.size() * sizeof(int));
+  const size_t o_obeg = put(obeg.data(), obeg.size() * sizeof(int));
+  const size_t o_ops = put(ops.data(), ops.size() * sizeof(Op));
+  int rc = e->ensure_stage(buf.size());
+  if (rc) return rc;
+  std::memcpy(e->stage_h, buf.data(), buf.size());
+  PIHM_HIP_TRY(hipMemcpyAsync(e->stage_d, e->stage_h, buf.size(), hipMemcpyHostToDevice, e->st));
+  double *save = nullptr;
+  if (c->nr > 0) PIHM_HIP_TRY(hipMalloc(&save, 2L * c->nr * sizeof(double)));
+  hipStream_t cs = c->stream;
+  c->stream = e->st;
+  rc = pihm_ctx_carried_copy(c, save, 0);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto &x : ev)
+    if (!rc && hipEventCreate(&x) != hipSuccess) rc = PIHM_ERR_HIP;
+  auto timed = [&](auto launch, double *ms) -> int {
+    for (int w = 0; w < 2; w++) if (int r = launch()) return r;
+    PIHM_HIP_TRY(hipEventRecord(ev[0], e->st));
+    for (int it = 0; it < reps; it++) if (int r = launch()) return r;
+    PIHM_HIP_TRY(hipEventRecord(ev[1], e->st));
+    PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+    float f = 0.0f;
+    PIHM_HIP_TRY(hipEventElapsedTime(&f, ev[0], ev[1]));
+    *ms = f / reps;
+    return PIHM_OK;
+  };
+  static const int spans[PIHM_TUNE_NSPAN] = PIHM_TUNE_SPANS;
+  pihm_ens_tuning t{};
+  double best = HUGE_VAL;
+  t.tile_span = c->args.tile_span;
+  for (int k = 0; k < PIHM_TUNE_NSPAN && !rc; k++) {
+    c->args.tile_span = spans[k];
+    rc = timed([&]() {
+      return pihm_rhs_launch_compact(c, (const MemberIO *)(e->stage_d + o_io), e->mnan,
+                                     (const int *)(e->stage_d + o_bm), (const int *)(e->stage_d + o_act), na,
+                                     bm[na], bm[(na + 1) + na]);
+    }, &t.rhs_ms[k]);
+    if (!rc && t.rhs_ms[k] < best) { best = t.rhs_ms[k]; t.tile_span = spans[k]; }
+  }
+  c->args.tile_span = t.tile_span;
+  Round rd{};
+  rd.part = (const int *)(e->stage_d + o_act);
+  rd.cpre = (const int *)(e->stage_d + o_cpre);
+  rd.obeg = (const int *)(e->stage_d + o_obeg);
+  rd.ops = (const Op *)(e->stage_d + o_ops);
+  rd.np = na;
+  rd.rmem = rd.part;
+  rd.rinfo = rd.part;
+  rd.nred = 0;
+  rd.nchunks = cpre.back();
+  best = HUGE_VAL;
+  t.ops_chunks_per_wg = e->ops_g;
+  for (int k = 0; k < kTuneNG && !rc; k++) {
+    rd.g = kTuneG[k];
+    rc = timed([&]() {
+      launch_ops(e->dev, rd, e->st);
+      return hipGetLastError() == hipSuccess ? (int)PIHM_OK : (int)PIHM_ERR_HIP;
+    }, &t.ops_ms[k]);
+    if (!rc && t.ops_ms[k] < best) { best = t.ops_ms[k]; t.ops_chunks_per_wg = kTuneG[k]; }
+  }
+  e->ops_g = t.ops_chunks_per_wg;
+  if (!rc) rc = pihm_ctx_carried_copy(c, save, 1);
+  if (!rc && hipStreamSynchronize(e->st) != hipSuccess) rc = PIHM_ERR_HIP;
+  c->stream = cs;
+  for (int m : act) e->mnan[m] = 0;
+  for (auto &x : ev) if (x) (void)hipEventDestroy(x);
+  if (save) (void)hipFree(save);
+  if (out) *out = t;
+  return rc;
 }
 
 int pihm_ens_set_state(pihm_ensemble *e, int k, const double *y) {

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
     if (r >= 0 && r < a.nr) river_flux<B>(a, r, part);
     return;
   }
-  const int tile = xcd_tile((int)blockIdx.x - nbr3, nbe);
+  const int tile = xcd_tile((int)blockIdx.x - nbr3, nbe, a.tile_span);
   const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
   const int ne = a.ne;
   if (p < 0 || p >= ne) return;
@@ -651,7 +651,7 @@ template <bool B>
 __global__ __launch_bounds__(kBlock, 3) void k_flux(RhsArgs a, int nbe) {
   const int nbr = (int)gridDim.x - nbe;
   if ((int)blockIdx.x >= nbr) {
-    const int tile = xcd_tile((int)blockIdx.x - nbr, nbe);
+    const int tile = xcd_tile((int)blockIdx.x - nbr, nbe, a.tile_span);
     const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
     if (p >= 0 && p < a.ne) elem_flux<B>(a, p);
   } else {
@@ -1592,6 +1592,47 @@ extern "C" int pihm_math_pow(int n, const double *x, const double *y, double *ou
 // Average device time of one whole RHS evaluation (both kernels back to
 // back, as the integrator issues them) over `iters` evaluations: one HIP event
 // before the first launch and one after the last, on the context's stream.
+extern "C" int pihm_ctx_set_tile_span(pihm_ctx *c, int span) {
+  if (!c || span < 0) return PIHM_ERR_ARG;
+  c->args.tile_span = span;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_tile_span(const pihm_ctx *c) { return c ? c->args.tile_span : PIHM_ERR_ARG; }
+
+// Saves (dir 0) or restores (dir 1) the carried river-edge flux, hid_ovl[0, 2 nr).
+int pihm_ctx_carried_copy(pihm_ctx *c, double *buf, int dir) {
+  const size_t bytes = 2L * c->nr * sizeof(double);
+  if (bytes == 0) return PIHM_OK;
+  double *hid = c->args.r.hid_ovl;
+  PIHM_HIP_TRY(hipMemcpyAsync(dir ? hid : buf, dir ? buf : hid, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_autotune(pihm_ctx *c, const double *y, double *ydot, int reps, int *best_span, double *ms) {
+  if (!c || !y || !ydot || reps <= 0 || c->nmem != 1) return PIHM_ERR_ARG;
+  static const int spans[PIHM_TUNE_NSPAN] = PIHM_TUNE_SPANS;
+  double *save = nullptr;
+  if (c->nr > 0) PIHM_HIP_TRY(hipMalloc(&save, 2L * c->nr * sizeof(double)));
+  int rc = pihm_ctx_carried_copy(c, save, 0);
+  int best = c->args.tile_span;
+  double tbest = HUGE_VAL;
+  for (int k = 0; k < PIHM_TUNE_NSPAN && !rc; k++) {
+    c->args.tile_span = spans[k];
+    double t = 0.0, w = 0.0;
+    rc = pihm_ctx_time_rhs(c, y, ydot, 2, &w);
+    if (!rc) rc = pihm_ctx_time_rhs(c, y, ydot, reps, &t);
+    if (ms) ms[k] = t;
+    if (!rc && t < tbest) { tbest = t; best = spans[k]; }
+  }
+  c->args.tile_span = best;
+  if (!rc) rc = pihm_ctx_carried_copy(c, save, 1);
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PIHM_ERR_HIP;
+  if (save) (void)hipFree(save);
+  if (best_span) *best_span = best;
+  return rc;
+}
+
 extern "C" int pihm_ctx_time_rhs(pihm_ctx *c, const double *y, double *ydot, int iters, double *ms_per_eval) {
   if (!c || !y || !ydot || iters <= 0 || !ms_per_eval) return PIHM_ERR_ARG;
   RhsArgs a = c->args;

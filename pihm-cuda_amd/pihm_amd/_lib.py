@@ -112,6 +112,20 @@ PrecSolveFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_
                           C.c_double, C.c_int, C.c_void_p, C.c_void_p)
 PIHM_PREC_NONE = 0
 PIHM_PREC_RIGHT = 2
+PIHM_TUNE_SPANS = (0, 1, 2, 4, 8, 16, 32, 64)
+PIHM_ENS_TUNE_G = (1, 2, 4, 8)
+
+
+class EnsTuning(C.Structure):
+    """pihm_ens_tuning (include/pihm_ensemble.h)."""
+
+    _fields_ = [("tile_span", C.c_int), ("ops_chunks_per_wg", C.c_int),
+                ("rhs_ms", C.c_double * 8), ("ops_ms", C.c_double * 4)]
+
+    def as_dict(self):
+        return {"tile_span": self.tile_span, "ops_chunks_per_wg": self.ops_chunks_per_wg,
+                "rhs_ms_by_span": dict(zip(PIHM_TUNE_SPANS, list(self.rhs_ms))),
+                "ops_ms_by_chunks_per_wg": dict(zip(PIHM_ENS_TUNE_G, list(self.ops_ms)))}
 ForcingFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double)
 SummaryFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
 StepEndFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
@@ -250,6 +264,11 @@ _PROTOS = [
                                        C.c_double]),
     ("pihm_ens_advance", C.c_int, [C.c_void_p, C.c_long, C.POINTER(C.c_int)]),
     ("pihm_ens_member_status", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_ens_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(EnsTuning)]),
+    ("pihm_ens_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("pihm_ctx_set_tile_span", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_ctx_get_tile_span", C.c_int, [C.c_void_p]),
+    ("pihm_ctx_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),
     ("pihm_lsm_create", C.c_int, [C.c_void_p, C.POINTER(LsmDesc), C.POINTER(Meteo), C.POINTER(C.c_void_p)]),
     ("pihm_lsm_destroy", None, [C.c_void_p]),
     ("pihm_lsm_set_veg", C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_double]),

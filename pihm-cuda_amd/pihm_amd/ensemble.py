@@ -65,6 +65,20 @@ class Ensemble:
         self.L.pihm_ens_member_stats(self.ens, k, C.byref(st))
         return st
 
+    def autotune(self, reps=20):
+        """Pick the launch shapes (pihm_ens_autotune) over the ready members; returns the
+        choices and the timings of every candidate."""
+        t = _lib.EnsTuning()
+        rc = self.L.pihm_ens_autotune(self.ens, int(reps), C.byref(t))
+        if rc:
+            raise RuntimeError(f"pihm_ens_autotune failed ({rc})")
+        return t.as_dict()
+
+    def set_tuning(self, tile_span, ops_chunks_per_wg):
+        rc = self.L.pihm_ens_set_tuning(self.ens, int(tile_span), int(ops_chunks_per_wg))
+        if rc:
+            raise RuntimeError(f"pihm_ens_set_tuning failed ({rc})")
+
     def counters(self):
         a, b, c = C.c_long(), C.c_long(), C.c_long()
         self.L.pihm_ens_counters(self.ens, C.byref(a), C.byref(b), C.byref(c))
