@@ -153,6 +153,9 @@ int pihm_rhs_launch_io(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan);
 // Saves (dir 0) or restores (dir 1) the carried river-edge flux of every slot
 // (hid_ovl[0, 2 nr)) to or from the device buffer buf, on the context's stream.
 int pihm_ctx_carried_copy(pihm_ctx *c, double *buf, int dir);
+// One RHS evaluation of a single-solve context at y (device) with the fluxes
+// recorded in args.rec / args.r.rf; the carried river-edge flux is restored.
+int pihm_ctx_eval_fluxes(pihm_ctx *c, const double *y, double *ydot, double *carried_save);
 int pihm_rhs_launch_compact(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan, const int *bm,
                             const int *bm_mem, int nact, int nbe, int nbr);
 

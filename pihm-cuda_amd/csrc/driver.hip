@@ -9,6 +9,7 @@
 #include "../../include/pihm_driver.h"
 #include "../../include/pihm_bc.h"
 #include "../../include/pihm_lsm.h"
+#include "../../include/pihm_output.h"
 #include "common.hpp"
 #include "ctx.hpp"
 #include "nvector_impl.hpp"
@@ -257,7 +258,7 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   return rc;
 }
 
-// ---------------------------------------------------------------- land-surface backend
+// ---------------------------------------------------------------- producer-driven runs
 namespace pihm {
 pihm_ctx *lsm_context(const pihm_lsm *l);   // lsm.hip
 pihm_ctx *bc_context(const pihm_bc *b);     // bc.hip
@@ -268,44 +269,61 @@ namespace {
 // Every LSM step the land-surface step runs on the state the last Summary saw
 // (IntcpSnowEt reads elem.ws, src/pihm.c:30-63); the driver integrates y in
 // place, so the vector Summary is handed is the one the next step reads.
-struct LsmRun {
+struct ProducersRun {
   pihm_ctx *ctx;
-  pihm_lsm *lsm;
-  pihm_bc *bc;
+  pihm_producers p;
   N_Vector y;
-  double etstep;
+  double etstep, t_start;
 };
 
-int lsm_bc(void *user, double t) {
-  LsmRun *r = (LsmRun *)user;
-  return pihm_bc_apply(r->bc, t);
+int run_bc(void *user, double t) {
+  ProducersRun *r = (ProducersRun *)user;
+  return pihm_bc_apply(r->p.bc, t);
 }
 
-int lsm_forcing(void *user, double t) {
-  LsmRun *r = (LsmRun *)user;
-  return pihm_lsm_step(r->lsm, t, r->etstep, r->y);
+int run_lsm(void *user, double t) {
+  ProducersRun *r = (ProducersRun *)user;
+  int rc = pihm_lsm_step(r->p.lsm, t, r->etstep, r->y);
+  if (!rc && r->p.out) rc = pihm_output_update(r->p.out, PIHM_OUT_LS_STEP, r->y);   // src/pihm.c:70-72
+  return rc;
 }
 
-int lsm_summary(void *user, N_Vector y) {
-  LsmRun *r = (LsmRun *)user;
+int run_summary(void *user, N_Vector y) {
+  ProducersRun *r = (ProducersRun *)user;
   r->y = y;
   return copy_ws0(r->ctx, y);
 }
 
+// UpdPrintVar(HYDROL_STEP) and PrintData after Summary (src/pihm.c:112-160)
+int run_step_end(void *user, double t, N_Vector y) {
+  ProducersRun *r = (ProducersRun *)user;
+  int rc = pihm_output_update(r->p.out, PIHM_OUT_HYDROL_STEP, y);
+  if (!rc) rc = pihm_output_print(r->p.out, t, t - r->t_start);
+  return rc;
+}
+
 }  // namespace
 
-extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, pihm_bc *bc, const pihm_run_config *cfg,
-                                double *y_inout, pihm_run_stats *st) {
-  if (!c || !lsm || !cfg || !y_inout || pihm::lsm_context(lsm) != c) return PIHM_ERR_ARG;
-  if (bc && pihm::bc_context(bc) != c) return PIHM_ERR_ARG;
-  LsmRun r{c, lsm, bc, nullptr, cfg->etstep};
+extern "C" int pihm_run_gpu_producers(pihm_ctx *c, const pihm_producers *p, const pihm_run_config *cfg,
+                                      double *y_inout, pihm_run_stats *st) {
+  if (!c || !p || !cfg || !y_inout) return PIHM_ERR_ARG;
+  if ((p->lsm && pihm::lsm_context(p->lsm) != c) || (p->bc && pihm::bc_context(p->bc) != c)) return PIHM_ERR_ARG;
+  ProducersRun r{c, *p, nullptr, cfg->etstep, cfg->t_start};
   pihm_backend be;
   std::memset(&be, 0, sizeof(be));
   be.rhs = gpu_rhs;
   be.rhs_data = c;
-  be.apply_forcing = lsm_forcing;
-  be.summary = lsm_summary;
-  be.apply_bc = bc ? lsm_bc : nullptr;
+  be.apply_forcing = p->lsm ? run_lsm : nullptr;
+  be.summary = run_summary;
+  be.apply_bc = p->bc ? run_bc : nullptr;
+  be.model_step_end = p->out ? run_step_end : nullptr;
   be.user = &r;
   return run_device(c, cfg, y_inout, st, &be);
+}
+
+extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, pihm_bc *bc, const pihm_run_config *cfg,
+                                double *y_inout, pihm_run_stats *st) {
+  if (!lsm) return PIHM_ERR_ARG;
+  const pihm_producers p{lsm, bc, nullptr};
+  return pihm_run_gpu_producers(c, &p, cfg, y_inout, st);
 }

@@ -245,6 +245,12 @@ int download_elem(pihm_lsm *l, const double *src, double *dst) {
 
 namespace pihm {
 pihm_ctx *lsm_context(const pihm_lsm *l) { return l ? l->ctx : nullptr; }
+// Device arrays of the carried storages and the last step's canopy
+// evaporation, device order: 0 cmc, 1 sneqv, 2 ec.
+const double *lsm_field(const pihm_lsm *l, int which) {
+  if (!l) return nullptr;
+  return which == 0 ? l->d.cmc : which == 1 ? l->d.sneqv : which == 2 ? l->d.ec : nullptr;
+}
 }
 
 extern "C" {

@@ -1592,6 +1592,25 @@ extern "C" int pihm_math_pow(int n, const double *x, const double *y, double *ou
 // Average device time of one whole RHS evaluation (both kernels back to
 // back, as the integrator issues them) over `iters` evaluations: one HIP event
 // before the first launch and one after the last, on the context's stream.
+// One RHS evaluation at y with the fluxes recorded (rec.*, r.rf) and the
+// carried river-edge flux put back afterwards (carried_save: 2 nr doubles of
+// device scratch), so the next integrator call reads what it would have read.
+int pihm_ctx_eval_fluxes(pihm_ctx *c, const double *y, double *ydot, double *carried_save) {
+  if (!c || !y || !ydot || c->nmem != 1) return PIHM_ERR_ARG;
+  int rc = pihm_ctx_carried_copy(c, carried_save, 0);
+  if (rc) return rc;
+  RhsArgs a = c->args;
+  a.y = y;
+  a.ydot = ydot;
+  a.record = 1;
+  a.closure = 0;
+  const int nbe = div_up(c->ne, kBlock), nbr = div_up(c->nr, kBlock);
+  hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  PIHM_HIP_TRY(hipGetLastError());
+  return pihm_ctx_carried_copy(c, carried_save, 1);
+}
+
 extern "C" int pihm_ctx_set_tile_span(pihm_ctx *c, int span) {
   if (!c || span < 0) return PIHM_ERR_ARG;
   c->args.tile_span = span;

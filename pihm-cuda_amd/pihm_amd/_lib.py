@@ -265,6 +265,14 @@ _PROTOS = [
     ("pihm_ens_advance", C.c_int, [C.c_void_p, C.c_long, C.POINTER(C.c_int)]),
     ("pihm_ens_member_status", C.c_int, [C.c_void_p, C.c_int]),
     ("pihm_ens_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(EnsTuning)]),
+    ("pihm_output_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.c_void_p, C.c_int, C.c_int,
+                                     C.POINTER(C.c_void_p)]),
+    ("pihm_output_destroy", None, [C.c_void_p]),
+    ("pihm_output_update", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("pihm_output_print", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    ("pihm_output_write_ic", C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    ("pihm_run_gpu_producers", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p,
+                                         C.POINTER(RunStats)]),
     ("pihm_ens_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("pihm_ctx_set_tile_span", C.c_int, [C.c_void_p, C.c_int]),
     ("pihm_ctx_get_tile_span", C.c_int, [C.c_void_p]),

@@ -1,9 +1,12 @@
 """Reader for the reference's input formats, on the reference's own example watershed
 (benchmark/input/example: 535 triangles, 20 river segments; files copied unchanged into
 tests/golden/example as fixtures), and the oracle's invariants on that real mesh."""
+import os
+
 import numpy as np
 
 import oracle as O
+from conftest import GOLDEN
 from test_oracle_pins import _inputs, _states
 
 
@@ -60,3 +63,15 @@ def test_example_oracle_invariants(example):
                - np.sum(sub[bnd]) - rf[a["down"] < 0, 1].sum())
         scale = np.sum(a["area"] * (inp["pcpdrp"] + inp["edir"] + inp["ett"]) + a["area"] * mag[:ne])
         assert abs(rate - src) <= 1e-11 * scale
+
+
+def test_print_controls_of_the_example_para():
+    """ReadPrtCtrl's keywords of the example's .para (src/read_para.c:111-180): every
+    control HOURLY, SUBFLX and SURFFLX one file per edge."""
+    from pihm_amd.output import EXTS, HOURLY, read_print_ctrl
+    v = read_print_ctrl(os.path.join(GOLDEN, "example.para"))
+    names = [EXTS[k] for k, _ in v]
+    assert len(v) == 29 and all(iv == HOURLY for _, iv in v)
+    assert names[:5] == ["surf", "unsat", "gw", "stage", "rivgw"]
+    assert [f"subflx{k}" for k in range(3)] == [n for n in names if n.startswith("subflx")]
+    assert "rivflx10" in names and "is" in names
