@@ -14,10 +14,14 @@
  *   <prefix>.<ext>.txt  with ascii: "YYYY-MM-DD HH:MM" then "\t%lf" per
  *                       value, one line per print.
  * Here the sums stay in HBM and only the printed averages cross to the host.
- * The flux variables (infil, recharge, surfflx0-2, subflx0-2, rivflx0-10) are
- * the fluxes of one RHS evaluation at the model step's final state, made for
- * the update with the carried river-edge flux left unchanged; the reference
- * prints whatever its integrator's last RHS call left in elem.wf.
+ * The flux variables (recharge, surfflx0-2, subflx0-2, rivflx0-10) are the
+ * fluxes of one RHS evaluation at the model step's final state, made for the
+ * update with the carried river-edge flux left unchanged; the reference
+ * prints whatever its integrator's last RHS call left in elem.wf.  infil is
+ * MassBalance's (src/update.c:107-160), which Summary writes over the RHS's:
+ * the storage change of unsat + gw since the previous model step times the
+ * porosity over the step, plus the subsurface outflow and the soil's share of
+ * evaporation and transpiration of that evaluation, floored at 0.
  * pihm_output_write_ic writes the restart file of src/print.c:250-300: per
  * element cmc, sneqv, surf, unsat, gw, then per segment stage, gw (doubles).
  */
@@ -74,6 +78,10 @@ typedef struct pihm_output pihm_output;
 int  pihm_output_create(pihm_ctx *ctx, pihm_lsm *lsm, const char *prefix, const pihm_output_var *vars,
                         int nvars, int ascii, pihm_output **out);
 void pihm_output_destroy(pihm_output *out);
+
+/* The initial state (ws0 after initialisation), needed by infil before the
+ * first model-step update; pihm_run_gpu_producers calls it. */
+int  pihm_output_start(pihm_output *out, N_Vector y0);
 
 /* UpdPrintVar: add the current values of the variables of this update step
  * (PIHM_OUT_HYDROL_STEP with the model step's final state y, or

@@ -290,6 +290,11 @@ int run_lsm(void *user, double t) {
 
 int run_summary(void *user, N_Vector y) {
   ProducersRun *r = (ProducersRun *)user;
+  if (!r->y && r->p.out) {
+    // the first call is the initial state (ws0 = ws after initialisation)
+    const int rc = pihm_output_start(r->p.out, y);
+    if (rc) return rc;
+  }
   r->y = y;
   return copy_ws0(r->ctx, y);
 }

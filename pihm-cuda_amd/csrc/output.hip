@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "ctx.hpp"
 #include "nvector_impl.hpp"
+#include "physics.hpp"
 
 using namespace pihm;
 
@@ -30,6 +31,44 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_out_add(long n, const double *src, double *sum) {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) sum[i] += src[i];
+}
+
+// MassBalance (src/update.c:107-160): the infiltration the step's storage
+// change and outflows imply, with the RHS's fluxes at the step's final state
+// (subsurface outflow, EtExtract's partition, src/hydrol.c:33-97); prev holds
+// unsat, gw of the previous model step (ws0), replaced by this step's.
+__global__ __launch_bounds__(kBlock) void k_mass_balance(int ne, ElemDev e, const double *y, double *prev,
+                                                        const double *sub, double dt, double *infil) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const double depth = e.depth[p], area = e.area[p];
+  const double u = y[ne + p], g = y[2L * ne + p];
+  double soilw0 = prev[ne + p] + prev[p];
+  soilw0 = (soilw0 > depth) ? depth : soilw0;
+  soilw0 = (soilw0 < 0.0) ? 0.0 : soilw0;
+  double soilw1 = g + u;
+  soilw1 = (soilw1 > depth) ? depth : soilw1;
+  soilw1 = (soilw1 < 0.0) ? 0.0 : soilw1;
+  double subrunoff = 0.0;
+  for (int j = 0; j < 3; j++) subrunoff += sub[(long)j * ne + p] / area;
+  // EtExtract at the final state (the clamped surface and groundwater)
+  const double surfh = surf_h((y[p] > 0.0) ? y[p] : 0.0);
+  const double gc = (g > 0.0) ? g : 0.0;
+  const double edir = e.edir[p], ett = e.ett[p];
+  double edir_u = 0.0, edir_g = 0.0, ett_u = 0.0, ett_g = 0.0;
+  if (surfh >= DEPRSTG) {
+  } else if (gc > depth - e.dinf[p]) {
+    edir_g = edir;
+  } else {
+    edir_u = edir;
+  }
+  if (gc > depth - e.rzd[p]) ett_g = ett;
+  else ett_u = ett;
+  double inf = (soilw1 - soilw0) * e.porosity[p] / dt + subrunoff + edir_u + edir_g + ett_u + ett_g;
+  if (inf < 0.0) inf = 0.0;
+  infil[p] = inf;
+  prev[p] = u;
+  prev[ne + p] = g;
 }
 
 struct OutVar {
@@ -84,8 +123,10 @@ struct pihm_output {
   pihm_ctx *ctx;
   pihm_lsm *lsm;
   std::vector<OutVar> v;
-  bool any_flux;
+  bool any_flux, need_mb, started;
   double *scratch;       // RHS output and the carried-flux copy of a flux evaluation
+  double *prev;          // MassBalance: unsat, gw of the previous model step [2 ne]
+  double *infil_mb;      // MassBalance's infiltration [ne]
   std::vector<double> host;
 };
 
@@ -99,7 +140,22 @@ void pihm_output_destroy(pihm_output *o) {
     if (x.txt) std::fclose(x.txt);
   }
   if (o->scratch) (void)hipFree(o->scratch);
+  if (o->prev) (void)hipFree(o->prev);
+  if (o->infil_mb) (void)hipFree(o->infil_mb);
   delete o;
+}
+
+int pihm_output_start(pihm_output *o, N_Vector yv) {
+  if (!o || !yv || N_VGetLength_Hip(yv) != o->ctx->neq) return PIHM_ERR_ARG;
+  if (o->prev) {
+    const double *y = N_VGetDeviceArrayPointer_Hip(yv);
+    const int ne = o->ctx->ne;
+    // prev = (unsat, gw) of the initial state: ws0 = ws after initialisation
+    PIHM_HIP_TRY(hipMemcpyAsync(o->prev, y + ne, 2L * ne * sizeof(double), hipMemcpyDeviceToDevice,
+                                o->ctx->stream));
+  }
+  o->started = true;
+  return PIHM_OK;
 }
 
 int pihm_output_create(pihm_ctx *ctx, pihm_lsm *lsm, const char *prefix, const pihm_output_var *vars, int nvars,
@@ -116,7 +172,11 @@ int pihm_output_create(pihm_ctx *ctx, pihm_lsm *lsm, const char *prefix, const p
   o->ctx = ctx;
   o->lsm = lsm;
   o->any_flux = false;
+  o->need_mb = false;
+  o->started = false;
   o->scratch = nullptr;
+  o->prev = nullptr;
+  o->infil_mb = nullptr;
   long nmax = 1;
   for (int i = 0; i < nvars; i++) {
     OutVar x{};
@@ -127,6 +187,7 @@ int pihm_output_create(pihm_ctx *ctx, pihm_lsm *lsm, const char *prefix, const p
     x.flux = is_flux(x.kind);
     x.n = x.river ? ctx->nr : ctx->ne;
     o->any_flux |= x.flux;
+    o->need_mb |= x.kind == PIHM_OUT_INFIL;
     nmax = std::max(nmax, x.n);
     char buf[32];
     const std::string name = std::string(prefix) + "." + ext_of(x.kind, buf);
@@ -146,6 +207,11 @@ int pihm_output_create(pihm_ctx *ctx, pihm_lsm *lsm, const char *prefix, const p
   }
   if (o->any_flux &&
       hipMalloc(&o->scratch, ((size_t)ctx->neq + 2L * ctx->nr + 1) * sizeof(double)) != hipSuccess) {
+    pihm_output_destroy(o);
+    return PIHM_ERR_HIP;
+  }
+  if (o->need_mb && (hipMalloc(&o->prev, 2L * std::max(1, ctx->ne) * sizeof(double)) != hipSuccess ||
+                     hipMalloc(&o->infil_mb, std::max(1, ctx->ne) * sizeof(double)) != hipSuccess)) {
     pihm_output_destroy(o);
     return PIHM_ERR_HIP;
   }
@@ -170,6 +236,11 @@ int pihm_output_update(pihm_output *o, int step, N_Vector yv) {
       const int rc = pihm_ctx_eval_fluxes(c, y, o->scratch, o->scratch + c->neq);
       if (rc) return rc;
       flux_done = true;
+      if (o->need_mb) {
+        if (!o->started) return PIHM_ERR_ARG;   // the previous step's storage is needed
+        hipLaunchKernelGGL(k_mass_balance, dim3(div_up(ne, kBlock)), dim3(kBlock), 0, c->stream, ne, c->args.e, y,
+                           o->prev, c->args.rec.sub, c->args.dt, o->infil_mb);
+      }
     }
     const double *src = nullptr;
     const int k = x.kind;
@@ -183,7 +254,7 @@ int pihm_output_update(pihm_output *o, int step, N_Vector yv) {
     else if (k == PIHM_OUT_EC) src = lsm_field(o->lsm, 2);
     else if (k == PIHM_OUT_ETT) src = c->args.e.ett;
     else if (k == PIHM_OUT_EDIR) src = c->args.e.edir;
-    else if (k == PIHM_OUT_INFIL) src = c->args.rec.infil;
+    else if (k == PIHM_OUT_INFIL) src = o->infil_mb;
     else if (k == PIHM_OUT_RECHARGE) src = c->args.rec.rechg;
     else if (k < PIHM_OUT_SUBFLX0) src = c->args.r.rf + (long)(k - PIHM_OUT_RIVFLX0) * nr;
     else if (k < PIHM_OUT_SURFFLX0) src = c->args.rec.sub + (long)(k - PIHM_OUT_SUBFLX0) * ne;

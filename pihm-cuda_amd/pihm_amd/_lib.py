@@ -269,6 +269,7 @@ _PROTOS = [
                                      C.POINTER(C.c_void_p)]),
     ("pihm_output_destroy", None, [C.c_void_p]),
     ("pihm_output_update", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("pihm_output_start", C.c_int, [C.c_void_p, C.c_void_p]),
     ("pihm_output_print", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     ("pihm_output_write_ic", C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     ("pihm_run_gpu_producers", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p,

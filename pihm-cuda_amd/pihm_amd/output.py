@@ -66,6 +66,18 @@ class HipOutput:
             raise RuntimeError(f"pihm_output_create failed ({rc})")
         self.h = h
 
+    def start(self, y0):
+        """The initial state (caller order), which infil's storage change starts from."""
+        self._upload(y0)
+        rc = self.L.pihm_output_start(self.h, self.rhs.y)
+        if rc:
+            raise RuntimeError(f"pihm_output_start failed ({rc})")
+
+    def _upload(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if self.L.pihm_ctx_state_to_device(self.rhs.ctx, y.ctypes.data_as(_lib.c_double_p), self.rhs.y):
+            raise RuntimeError("state_to_device failed")
+
     def update(self, step, y=None):
         """UpdPrintVar; for the model step y (caller order) is uploaded first."""
         if y is not None:

@@ -60,6 +60,34 @@ class Checker:
             x["n"] = 0
 
 
+def _mass_balance(a, prev, y, sub, edir, ett, dt):
+    """MassBalance (src/update.c:107-160) with EtExtract's partition (src/hydrol.c:33-97)
+    at the state y, operation for operation."""
+    ne = len(a["area"])
+    depth = a["depth"]
+    s0 = prev[2 * ne:3 * ne] + prev[ne:2 * ne]
+    s0 = np.where(s0 > depth, depth, s0)
+    s0 = np.where(s0 < 0.0, 0.0, s0)
+    s1 = y[2 * ne:3 * ne] + y[ne:2 * ne]
+    s1 = np.where(s1 > depth, depth, s1)
+    s1 = np.where(s1 < 0.0, 0.0, s1)
+    sr = np.zeros(ne)
+    for j in range(3):
+        sr = sr + sub[:, j] / a["area"]
+    sc = np.maximum(y[:ne], 0.0)
+    dep = 1e-4
+    surfh = np.where(sc <= 0.5 * dep, np.sqrt(2.0 * dep * sc), dep + (sc - 0.5 * dep))
+    g = np.maximum(y[2 * ne:3 * ne], 0.0)
+    to_surf = surfh >= dep
+    to_gw = ~to_surf & (g > depth - a["dinf"])
+    eu = np.where(~to_surf & ~to_gw, edir, 0.0)
+    eg = np.where(to_gw, edir, 0.0)
+    tg = np.where(g > depth - a["rzd"], ett, 0.0)
+    tu = np.where(g > depth - a["rzd"], 0.0, ett)
+    inf = (s1 - s0) * a["porosity"] / dt + sr + eu + eg + tu + tg
+    return np.where(inf < 0.0, 0.0, inf)
+
+
 def test_output_matches_print_bookkeeping(c0, tmp_path):
     m = c0
     ne, nr = m.nelem, m.nriver
@@ -83,6 +111,9 @@ def test_output_matches_print_bookkeeping(c0, tmp_path):
     zero_carried = np.zeros(3 * ne)
     dt, et = 600.0, 1800.0
     y = m.initial_state()
+    out.start(y)
+    prev = y.copy()
+    a = m.a
     for k in range(30):                           # five simulated hours
         t = T0 + dt * k
         if k % int(et / dt) == 0:
@@ -103,8 +134,10 @@ def test_output_matches_print_bookkeeping(c0, tmp_path):
         ref.ode(y)
         fl = ref.fluxes()
         rf = fl["rivflow"].reshape(nr, 11)
+        infil = _mass_balance(a, prev, y, fl["subsurf"].reshape(ne, 3), ed, et_, m.stepsize)
+        prev = y.copy()
         chk.update(HYDROL_STEP, {KIND["surf"]: y[:ne], KIND["gw"]: y[2 * ne:3 * ne],
-                                 KIND["stage"]: y[3 * ne:3 * ne + nr], KIND["infil"]: fl["infil"],
+                                 KIND["stage"]: y[3 * ne:3 * ne + nr], KIND["infil"]: infil,
                                  KIND["rivflx1"]: rf[:, 1], KIND["subflx0"]: fl["subsurf"].reshape(ne, 3)[:, 0],
                                  KIND["surfflx2"]: fl["ovlflow"].reshape(ne, 3)[:, 2]})
         tn = t + dt
