@@ -91,6 +91,20 @@ int   PihmCVSpgmr(void *mem, int pretype, int maxl);
  * (sqrt(N) for the unit basis vector), and the preconditioner solve must only
  * enqueue device work on the vectors' stream when graphs are on. */
 int   PihmCVSpilsSetPreconditioner(void *mem, PihmPrecSetupFn pset, PihmPrecSolveFn psolve);
+/* Step record and replay; no counterpart in CVODE.
+ * Record: the size and order of every accepted step k go to h[k], q[k]
+ * (k < cap), *n counting them.  With iters non-null, iters[k*PIHM_REC_ITERS]
+ * holds the step's Newton iteration count and iters[k*PIHM_REC_ITERS + 1 + j]
+ * the Krylov iterations of Newton iteration j (0: the residual was already
+ * below the linear tolerance; -1: SPGMR returned the zero solution).
+ * Replay: step k is taken with h[k] and q[k] (k < n) in place of the step and
+ * order selection, and its error test is not applied; with iters non-null it
+ * also takes exactly the recorded Newton and Krylov counts in place of their
+ * convergence tests.  A Newton failure on a replayed step returns
+ * PIHM_CV_CONV_FAILURE.  Past step n the normal selection resumes. */
+#define PIHM_REC_ITERS 8
+int   PihmCVodeSetStepRecord(void *mem, double *h, int *q, int *iters, long cap, long *n);
+int   PihmCVodeSetStepReplay(void *mem, const double *h, const int *q, const int *iters, long n);
 int   PihmCVode(void *mem, realtype tout, N_Vector yout, realtype *tret, int itask);
 int   PihmCVodeGetDky(void *mem, realtype t, int k, N_Vector dky);
 void  PihmCVodeFree(void **mem);

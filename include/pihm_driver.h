@@ -39,6 +39,13 @@ typedef struct pihm_run_config {
   int precond;                  /* 0: SPGMR without preconditioning, as the reference
                                    (src/ode.c:523); 1: right block-Jacobi preconditioner
                                    from the backend's prec_setup / prec_solve */
+  /* optional step record / replay (PihmCVodeSetStepRecord / SetStepReplay) */
+  double *rec_h;
+  int32_t *rec_q, *rec_iters;   /* rec_iters may be NULL */
+  long rec_cap, *rec_n;
+  const double *replay_h;
+  const int32_t *replay_q, *replay_iters;   /* replay_iters may be NULL */
+  long replay_n;
 } pihm_run_config;
 
 typedef struct pihm_forcing {

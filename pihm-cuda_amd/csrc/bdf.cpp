@@ -47,6 +47,14 @@ struct Mem {
   bool jcur = false;
   long nstlp = 0, nstlpre = 0, npe = 0, nps = 0, nsetups = 0;
   double delta_ls = 0.0;
+  // step record and replay (PihmCVodeSetStepRecord / SetStepReplay)
+  double *rec_h = nullptr;
+  int *rec_q = nullptr, *rec_it = nullptr;   // rec_it: PIHM_REC_ITERS counts per step
+  long rec_cap = 0, *rec_n = nullptr;
+  const double *rp_h = nullptr;
+  const int *rp_q = nullptr, *rp_it = nullptr;
+  long rp_n = 0;
+  int att_it[PIHM_REC_ITERS] = {};   // Newton and Krylov counts of the step attempt in progress
   bool fused = false;
   // Krylov iterations replayed from graphs (device vectors on a non-null
   // stream and an RHS declared capturable, PihmCVodeSetGraphRhs)
@@ -90,6 +98,15 @@ inline void VDestroy(N_Vector v) { if (v) v->ops->nvdestroy(v); }
 bool is_hip(N_Vector v) { return v->ops->nvgetvectorid == N_VGetVectorID_Hip; }
 
 // cv_setupNonNull (cvode_spgmr.c CVSpgmr / CVSpilsSetPreconditioner)
+// The recorded Newton and Krylov counts of the step being replayed, or null.
+const int *replay_iters(const Mem *m) {
+  return (m->rp_it && m->nst < m->rp_n) ? m->rp_it + m->nst * PIHM_REC_ITERS : nullptr;
+}
+constexpr int FREE = -100;   // no forced Krylov count
+// Krylov count forced for Newton iteration mi of a replayed step (FREE past the record)
+int forced_krylov(const int *fi, int mi) { return (fi && mi + 1 < PIHM_REC_ITERS) ? fi[1 + mi] : FREE; }
+void note_krylov(Mem *m, int mi, int code) { if (mi + 1 < PIHM_REC_ITERS) m->att_it[1 + mi] = code; }
+
 bool setup_nonnull(const Mem *m) { return m->pretype != PIHM_PREC_NONE && m->pset != nullptr; }
 bool pre_right(const Mem *m) { return m->pretype == PIHM_PREC_RIGHT && m->psolve != nullptr; }
 
@@ -284,14 +301,15 @@ int atimes(Mem *m, N_Vector v, N_Vector z) {
 }
 
 // SpgmrSolve with x0 = 0, no restarts, s1 = s2 = ewt (sundials_spgmr.c:165-425)
-int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli) {
+// force: FREE, or the exact iteration count of a replayed solve (-1: the zero solution)
+int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli, int force) {
   N_Vector *V = m->V;
   N_Vector s = m->ewt;
   *nli = 0;
   VProd(s, b, V[0]);
   const double beta = std::sqrt(VDot(V[0], V[0]));
   double r_norm = beta, rho = beta;
-  if (r_norm <= delta) return SP_SUCCESS;
+  if (force == FREE ? r_norm <= delta : force < 0) { *nli = -1; return SP_SUCCESS; }
   VConst(0.0, m->xcor);
   for (int i = 0; i <= m->maxl; i++)
     for (int j = 0; j < m->maxl; j++) m->Hes[i][j] = 0.0;
@@ -316,7 +334,7 @@ int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli) {
     if (qr_update(m, krydim) != 0) return SP_QRFACT_FAIL;
     rotation_product *= m->givens[2 * l + 1];
     rho = std::fabs(rotation_product * r_norm);
-    if (rho <= delta) { converged = true; break; }
+    if (force == FREE ? rho <= delta : (l + 1 >= force || rho == 0.0)) { converged = true; break; }
     VScale(1.0 / m->Hes[l + 1][l], V[l + 1], V[l + 1]);
   }
   m->yg[0] = r_norm;
@@ -338,10 +356,12 @@ int spgmr(Mem *m, N_Vector x, N_Vector b, double delta, int *nli) {
 }
 
 // CVSpgmrSolve (cvode_spgmr.c:355-420)
-int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
+// *code: Krylov iterations taken, 0 for the small-residual exit, -1 for the zero solution
+int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow, int force, int *code) {
   const double deltar = m->eplifac * m->tq[4];
   const double bnorm = VWrms(b, m->ewt);
-  if (bnorm <= deltar) {
+  *code = 0;
+  if (force == FREE ? bnorm <= deltar : force == 0) {
     if (m->mnewt > 0) VConst(0.0, b);
     return 0;
   }
@@ -351,9 +371,10 @@ int lsolve(Mem *m, N_Vector b, N_Vector ynow, N_Vector fnow) {
   m->delta_ls = delta;
   VConst(0.0, m->xs);
   int nli = 0;
-  const int rv = spgmr(m, m->xs, b, delta, &nli);
+  const int rv = spgmr(m, m->xs, b, delta, &nli, force);
   VScale(1.0, m->xs, b);
-  m->nli += nli;
+  *code = nli;
+  if (nli > 0) m->nli += nli;
   switch (rv) {
     case SP_SUCCESS: return 0;
     case SP_RES_REDUCED: return (m->mnewt == 0) ? 0 : 1;
@@ -421,13 +442,13 @@ bool build_graph(Mem *m, int l) {
   return true;
 }
 
-int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
+int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out, int force) {
   N_Vector *V = m->V;
   N_Vector s = m->ewt;
   const double beta = std::sqrt(sumsq);
   double r_norm = beta, rho = beta;
   *krydim_out = 0;
-  if (r_norm <= delta) return SP_ZERO;
+  if (force == FREE ? r_norm <= delta : force < 0) return SP_ZERO;
   for (int i = 0; i <= m->maxl; i++)
     for (int j = 0; j < m->maxl; j++) m->Hes[i][j] = 0.0;
   double rotation_product = 1.0;
@@ -490,7 +511,7 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out) {
     if (qr_update(m, krydim) != 0) return SP_QRFACT_FAIL;
     rotation_product *= m->givens[2 * l + 1];
     rho = std::fabs(rotation_product * r_norm);
-    if (rho <= delta) { converged = true; break; }
+    if (force == FREE ? rho <= delta : (l + 1 >= force || rho == 0.0)) { converged = true; break; }
     cscale = 1.0 / m->Hes[l + 1][l];
   }
   m->yg[0] = r_norm;
@@ -507,6 +528,8 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
   int mi = 0;
   double del = 0.0, delp = 0.0;
   const double N = (double)m->neq;
+  const int *fi = replay_iters(m);
+  for (int k = 0; k < PIHM_REC_ITERS; k++) m->att_it[k] = 0;
   for (;;) {
     double sumsq = 0.0;
     if (pihm_gm_start(m->rl1, m->zn[1], m->acor, m->gamma, m->ftemp, m->ewt, m->tempv, m->V[0], &sumsq, m->kry))
@@ -514,14 +537,17 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
     const double deltar = m->eplifac * m->tq[4];
     const double bnorm = std::sqrt(sumsq / N);
     int mode = 0, krydim = 0, lret = 0;
-    if (bnorm <= deltar) {
+    const int force = forced_krylov(fi, mi);
+    if (force == FREE ? bnorm <= deltar : force == 0) {
       mode = (m->mnewt > 0) ? 2 : 1;
+      note_krylov(m, mi, 0);
     } else {
       m->ycur = m->y;
       m->fcur = m->ftemp;
       m->delta_ls = deltar * m->sqrtN;
-      const int sp = spgmr_fused(m, sumsq, deltar * m->sqrtN, &krydim);
+      const int sp = spgmr_fused(m, sumsq, deltar * m->sqrtN, &krydim, force);
       m->nli += krydim;
+      note_krylov(m, mi, sp == SP_ZERO ? -1 : krydim);
       switch (sp) {
         case SP_ZERO: mode = 2; break;
         case SP_SUCCESS: mode = 0; break;
@@ -549,12 +575,13 @@ int newton_iteration_fused(Mem *m) {   // cvode.c:2729-2832 with CVSpgmrSolve fu
     del = std::sqrt(sums[0] / N);
     if (mi > 0) m->crate = std::fmax(CRDOWN * m->crate, del / delp);
     const double dcon = del * std::fmin(1.0, m->crate) / m->tq[4];
-    if (dcon <= 1.0) {
+    if (fi ? mi + 1 >= fi[0] : dcon <= 1.0) {   // a replayed step takes its recorded count
       m->acnrm = (mi == 0) ? del : std::sqrt(sums[1] / N);
+      m->att_it[0] = mi + 1;
       return PIHM_CV_SUCCESS;
     }
     m->mnewt = ++mi;
-    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
+    if (mi == m->maxcor || (!fi && mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
     delp = del;
     const int fr = m->f(m->tn, m->y, m->ftemp, m->ud);
     m->nfe++;
@@ -568,6 +595,8 @@ int newton_iteration(Mem *m) {   // cvode.c:2729-2832
   m->mnewt = 0;
   int mi = 0;
   double del = 0.0, delp = 0.0;
+  const int *fi = replay_iters(m);
+  for (int k = 0; k < PIHM_REC_ITERS; k++) m->att_it[k] = 0;
   for (;;) {
     if (m->fused) pihm_nv_newton_rhs(m->rl1, m->zn[1], m->acor, m->gamma, m->ftemp, m->tempv);
     else {
@@ -575,7 +604,9 @@ int newton_iteration(Mem *m) {   // cvode.c:2729-2832
       VLin(m->gamma, m->ftemp, -1.0, m->tempv, m->tempv);
     }
     N_Vector b = m->tempv;
-    const int rv = lsolve(m, b, m->y, m->ftemp);
+    int code = 0;
+    const int rv = lsolve(m, b, m->y, m->ftemp, forced_krylov(fi, mi), &code);
+    note_krylov(m, mi, code);
     m->nni++;
     if (rv < 0) return PIHM_CV_LSOLVE_FAIL;
     if (rv > 0) return retry_or(m, CONV_FAIL);
@@ -584,12 +615,13 @@ int newton_iteration(Mem *m) {   // cvode.c:2729-2832
     VLin(1.0, m->zn[0], 1.0, m->acor, m->y);
     if (mi > 0) m->crate = std::fmax(CRDOWN * m->crate, del / delp);
     const double dcon = del * std::fmin(1.0, m->crate) / m->tq[4];
-    if (dcon <= 1.0) {
+    if (fi ? mi + 1 >= fi[0] : dcon <= 1.0) {   // a replayed step takes its recorded count
       m->acnrm = (mi == 0) ? del : VWrms(m->acor, m->ewt);
+      m->att_it[0] = mi + 1;
       return PIHM_CV_SUCCESS;
     }
     m->mnewt = ++mi;
-    if (mi == m->maxcor || (mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
+    if (mi == m->maxcor || (!fi && mi >= 2 && del > RDIV * delp)) return retry_or(m, CONV_FAIL);
     delp = del;
     const int fr = m->f(m->tn, m->y, m->ftemp, m->ud);
     m->nfe++;
@@ -655,6 +687,7 @@ int do_error_test(Mem *m, int *nflag, double saved_t, int *nef, double *dsmp) { 
   const double dsm = m->acnrm * m->tq[2];
   *dsmp = dsm;
   if (dsm <= 1.0) return PIHM_CV_SUCCESS;
+  if (m->rp_h && m->nst < m->rp_n) return PIHM_CV_SUCCESS;   // a replayed step is taken as recorded
   (*nef)++;
   m->netf++;
   *nflag = PREV_ERR_FAIL;
@@ -692,6 +725,13 @@ int do_error_test(Mem *m, int *nflag, double saved_t, int *nef, double *dsmp) { 
 }
 
 void complete_step(Mem *m) {   // cvode.c:2996-3027
+  if (m->rec_h && m->nst < m->rec_cap) {
+    m->rec_h[m->nst] = m->h;
+    m->rec_q[m->nst] = m->q;
+    if (m->rec_it)
+      for (int k = 0; k < PIHM_REC_ITERS; k++) m->rec_it[m->nst * PIHM_REC_ITERS + k] = m->att_it[k];
+    *m->rec_n = m->nst + 1;
+  }
   m->nst++;
   m->nscon++;
   m->hu = m->h;
@@ -779,12 +819,18 @@ int step(Mem *m) {   // cvode.c:2005-2066
   const double saved_t = m->tn;
   int ncf = 0, nef = 0, nflag = FIRST_CALL;
   double dsm = 0.0;
-  if ((m->nst > 0) && (m->hprime != m->h)) adjust_params(m);
+  if ((m->nst > 0) && (m->hprime != m->h || (m->rp_h && m->qprime != m->q))) adjust_params(m);
+  if (m->rp_h && m->nst > 0 && m->nst < m->rp_n) {   // the recorded size exactly, not h * (h'/h)
+    m->h = m->rp_h[m->nst];
+    m->hscale = m->h;
+    m->next_h = m->h;
+  }
   for (;;) {
     predict(m);
     set_bdf(m);
     nflag = nls_newton(m, nflag);
     const int kflag = handle_nflag(m, &nflag, saved_t, &ncf);
+    if (kflag == PREDICT_AGAIN && m->rp_h && m->nst < m->rp_n) return PIHM_CV_CONV_FAILURE;   // cannot follow
     if (kflag == PREDICT_AGAIN) continue;
     if (kflag != DO_ERROR_TEST) return kflag;
     const int eflag = do_error_test(m, &nflag, saved_t, &nef, &dsm);
@@ -794,6 +840,20 @@ int step(Mem *m) {   // cvode.c:2005-2066
   }
   complete_step(m);
   prepare_next_step(m, dsm);
+  if (m->rp_h && m->nst < m->rp_n) {
+    // the next step as recorded: its order and size replace the selection
+    const int qn = m->rp_q[m->nst];
+    if (qn < 1 || qn > m->qmax || qn > m->q + 1) return PIHM_CV_ILL_INPUT;
+    if (qn == m->q + 1 && m->qprime != qn) VScale(1.0, m->acor, m->zn[m->qmax]);   // as choose_eta
+    while (m->q > qn + 1) {   // a drop of several orders, as repeated error-test failures make it
+      adjust_order(m, -1);
+      m->q--;
+      m->L = m->q + 1;
+    }
+    m->qprime = qn;
+    m->hprime = m->rp_h[m->nst];
+    m->eta = m->hprime / m->h;
+  }
   m->etamax = (m->nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   VScale(m->tq[2], m->acor, m->acor);
   return PIHM_CV_SUCCESS;
@@ -926,6 +986,30 @@ int PihmCVodeSetGraphRhs(void *mem, int onoff) {
   free_graphs(m);
   return PIHM_CV_SUCCESS;
 }
+int PihmCVodeSetStepRecord(void *mem, double *h, int *q, int *iters, long cap, long *n) {
+  if (!mem) return PIHM_CV_MEM_NULL;
+  if (cap < 0 || (cap > 0 && (!h || !q || !n))) return PIHM_CV_ILL_INPUT;
+  Mem *m = (Mem *)mem;
+  m->rec_h = cap > 0 ? h : nullptr;
+  m->rec_q = q;
+  m->rec_it = cap > 0 ? iters : nullptr;
+  m->rec_cap = cap;
+  m->rec_n = n;
+  if (n) *n = 0;
+  return PIHM_CV_SUCCESS;
+}
+
+int PihmCVodeSetStepReplay(void *mem, const double *h, const int *q, const int *iters, long n) {
+  if (!mem) return PIHM_CV_MEM_NULL;
+  if (n < 0 || (n > 0 && (!h || !q))) return PIHM_CV_ILL_INPUT;
+  Mem *m = (Mem *)mem;
+  m->rp_h = n > 0 ? h : nullptr;
+  m->rp_q = q;
+  m->rp_it = n > 0 ? iters : nullptr;
+  m->rp_n = n;
+  return PIHM_CV_SUCCESS;
+}
+
 int PihmCVSpilsSetPreconditioner(void *mem, PihmPrecSetupFn pset, PihmPrecSolveFn psolve) {
   if (!mem) return PIHM_CV_MEM_NULL;
   Mem *m = (Mem *)mem;
@@ -991,6 +1075,7 @@ int PihmCVode(void *mem, realtype tout, N_Vector yout, realtype *tret, int itask
     if (std::fabs(m->h) < m->hmin) m->h *= m->hmin / std::fabs(m->h);
     if (m->tstopset && (m->tn + m->h - m->tstop) * m->h > 0.0)
       m->h = (m->tstop - m->tn) * (1.0 - 4.0 * UROUND);
+    if (m->rp_h && m->rp_n > 0) m->h = m->rp_h[0];   // the recorded first step
     m->hscale = m->h;
     m->h0u = m->h;
     m->hprime = m->h;

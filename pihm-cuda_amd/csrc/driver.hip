@@ -43,6 +43,10 @@ extern "C" int pihm_drive(const pihm_backend *be, N_Vector y, const pihm_run_con
   if (cfg->precond && !pre) flag = PIHM_ERR_ARG;
   if (!flag) flag = PihmCVSpgmr(cv, pre ? PIHM_PREC_RIGHT : PIHM_PREC_NONE, cfg->maxl);
   if (!flag && pre) flag = PihmCVSpilsSetPreconditioner(cv, be->prec_setup, be->prec_solve);
+  if (!flag && cfg->rec_cap > 0)
+    flag = PihmCVodeSetStepRecord(cv, cfg->rec_h, cfg->rec_q, cfg->rec_iters, cfg->rec_cap, cfg->rec_n);
+  if (!flag && cfg->replay_n > 0)
+    flag = PihmCVodeSetStepReplay(cv, cfg->replay_h, cfg->replay_q, cfg->replay_iters, cfg->replay_n);
   if (!flag && cfg->graphs && be->rhs == gpu_rhs_fn()) flag = PihmCVodeSetGraphRhs(cv, 1);
   if (flag) { PihmCVodeFree(&cv); return flag; }
 

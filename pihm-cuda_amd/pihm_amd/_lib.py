@@ -58,7 +58,10 @@ class RunConfig(C.Structure):
                 ("initstep", C.c_double), ("nncfn", C.c_double), ("nnimax", C.c_double),
                 ("nnimin", C.c_double), ("decr", C.c_double), ("incr", C.c_double),
                 ("stmin", C.c_double), ("maxl", C.c_int), ("max_steps_per_call", C.c_long),
-                ("graphs", C.c_int), ("precond", C.c_int)]
+                ("graphs", C.c_int), ("precond", C.c_int),
+                ("rec_h", c_double_p), ("rec_q", c_int32_p), ("rec_iters", c_int32_p), ("rec_cap", C.c_long),
+                ("rec_n", C.POINTER(C.c_long)),
+                ("replay_h", c_double_p), ("replay_q", c_int32_p), ("replay_iters", c_int32_p), ("replay_n", C.c_long)]
 
 
 class Forcing(C.Structure):
@@ -112,6 +115,7 @@ PrecSolveFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_
                           C.c_double, C.c_int, C.c_void_p, C.c_void_p)
 PIHM_PREC_NONE = 0
 PIHM_PREC_RIGHT = 2
+PIHM_REC_ITERS = 8
 PIHM_TUNE_SPANS = (0, 1, 2, 4, 8, 16, 32, 64)
 PIHM_ENS_TUNE_G = (1, 2, 4, 8)
 
@@ -228,6 +232,8 @@ _PROTOS = [
     ("PihmCVodeSetGraphRhs", C.c_int, [C.c_void_p, C.c_int]),
     ("PihmCVSpgmr", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("PihmCVSpilsSetPreconditioner", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("PihmCVodeSetStepRecord", C.c_int, [C.c_void_p, c_double_p, c_int32_p, c_int32_p, C.c_long, C.POINTER(C.c_long)]),
+    ("PihmCVodeSetStepReplay", C.c_int, [C.c_void_p, c_double_p, c_int32_p, c_int32_p, C.c_long]),
     ("PihmCVodeGetNumLinSolvSetups", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
     ("PihmCVSpilsGetNumPrecEvals", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
     ("PihmCVSpilsGetNumPrecSolves", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),

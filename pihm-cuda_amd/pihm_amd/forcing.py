@@ -70,3 +70,50 @@ def run_config(t_end, t_start=0.0, stepsize=60.0, etstep=900.0, reltol=1e-3, abs
     c.graphs = graphs
     c.precond = precond
     return c
+
+
+class StepRecord:
+    """Storage for the size, order and Newton/Krylov counts of up to ``cap`` accepted
+    integrator steps (PihmCVodeSetStepRecord through pihm_run_config)."""
+
+    def __init__(self, cap):
+        self.h = np.zeros(cap)
+        self.q = np.zeros(cap, dtype=np.int32)
+        self.iters = np.zeros((cap, _lib.PIHM_REC_ITERS), dtype=np.int32)
+        self.n = C.c_long(0)
+
+    def attach(self, cfg):
+        cfg.rec_h = self.h.ctypes.data_as(_lib.c_double_p)
+        cfg.rec_q = self.q.ctypes.data_as(_lib.c_int32_p)
+        cfg.rec_iters = self.iters.ctypes.data_as(_lib.c_int32_p)
+        cfg.rec_cap = len(self.h)
+        cfg.rec_n = C.pointer(self.n)
+        cfg._record = self            # the arrays stay alive with the configuration
+        return cfg
+
+    def steps(self):
+        """(h, q, iters) of the recorded steps."""
+        n = self.n.value
+        return self.h[:n].copy(), self.q[:n].copy(), self.iters[:n].copy()
+
+
+def set_replay(cfg, h, q, iters=None):
+    """Make the run take the recorded steps (h[k], q[k]) in place of the step-size and
+    order selection, and with ``iters`` (StepRecord.steps' third array) also the
+    recorded Newton and Krylov counts (PihmCVodeSetStepReplay through pihm_run_config)."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.int32)
+    if h.shape != q.shape or h.ndim != 1:
+        raise ValueError("h and q must be 1-D arrays of one length")
+    cfg.replay_h = h.ctypes.data_as(_lib.c_double_p)
+    cfg.replay_q = q.ctypes.data_as(_lib.c_int32_p)
+    keep = [h, q]
+    if iters is not None:
+        iters = np.ascontiguousarray(iters, dtype=np.int32)
+        if iters.shape != (len(h), _lib.PIHM_REC_ITERS):
+            raise ValueError(f"iters must have shape ({len(h)}, {_lib.PIHM_REC_ITERS})")
+        cfg.replay_iters = iters.ctypes.data_as(_lib.c_int32_p)
+        keep.append(iters)
+    cfg.replay_n = len(h)
+    cfg._replay = keep
+    return cfg

@@ -1,0 +1,79 @@
+"""Replay on the device of a step sequence recorded by the host run.
+
+The host run (the integrator over the oracle right-hand side and host vectors, through
+the model-step driver) records the size, order and Newton/Krylov counts of every
+accepted step; the device run (HIP right-hand side, device N_Vector, fused SPGMR) takes
+exactly that sequence (PihmCVodeSetStepReplay).  The two then differ only by the
+rounding of their right-hand sides and reductions, and the storages and outlet
+discharge must agree within 1e-8 relative.
+
+The window is ten minutes of model time.  Over longer windows neither this pair nor two
+host runs whose reductions round differently (one thread against eight) hold 1e-8:
+the right-hand side switches branches where a river stage or a ponding depth crosses a
+threshold, and a rounding-level difference that moves such a crossing grows to the
+solver tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pihm_amd.forcing import StepRecord, StormForcing, run_config, set_replay
+from pihm_amd.rhs import run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _stores(mesh, y):
+    a, ne, nr = mesh.a, mesh.nelem, mesh.nriver
+    ys, yu, yg = y[:ne], y[ne:2 * ne], y[2 * ne:3 * ne]
+    st, rg = y[3 * ne:3 * ne + nr], y[3 * ne + nr:3 * ne + 2 * nr]
+    return np.array([np.sum(a["area"] * ys), np.sum(a["area"] * a["porosity"] * yu),
+                     np.sum(a["area"] * a["porosity"] * yg), np.sum(a["rarea"] * st),
+                     np.sum(a["rarea"] * a["rporosity"] * rg)])
+
+
+def _outlet(mesh, y):
+    """Outlet discharge [m3 s-1] (OutletFlux, src/river_flow.c:333-390) at state y."""
+    o = O.OracleRHS(mesh)
+    o.ode(y)
+    rf = o.field("rivflow").reshape(mesh.nriver, 11)
+    return rf[np.asarray(mesh.rivers["down"]) < 0, 1].copy()
+
+
+def _check_replay(m, f, cfg_fn, label):
+    y0 = m.initial_state()
+    rec = StepRecord(500000)
+    rc_c, y_c, st_c = O.OracleRun(m, f).run(y0, rec.attach(cfg_fn()))
+    assert rc_c == 0
+    h, q, it = rec.steps()
+    rec_g = StepRecord(500000)
+    cfg = set_replay(rec_g.attach(cfg_fn()), h, q, it)
+    print(f"[replay] {label}: {len(h)} host steps, device replay", flush=True)
+    rc_g, y_g, st_g = run_gpu(m, f, cfg, y0)
+    assert rc_g == 0, st_g.as_dict()
+    h2, q2, it2 = rec_g.steps()
+    assert st_g.nst == st_c.nst
+    assert np.array_equal(h2, h) and np.array_equal(q2, q) and np.array_equal(it2, it)
+    S_c, S_g = _stores(m, y_c), _stores(m, y_g)
+    o_c, o_g = _outlet(m, y_c), _outlet(m, y_g)
+    comp = np.max(np.abs(y_g - y_c) / np.maximum(np.abs(y_c), 1e-4))
+    live = np.abs(S_c) > 0
+    srel = np.abs(S_g - S_c)[live] / np.abs(S_c)[live]
+    orel = np.abs(o_g - o_c) / np.abs(o_c)
+    print(f"  storages rel {srel}  outlet rel {orel}  max component {comp:.2e}", flush=True)
+    assert np.all(srel <= 1e-8)
+    assert np.all(orel <= 1e-8)
+
+
+def test_device_replays_host_sequence(c0):
+    f = StormForcing(24, seed=5, nelem=c0.nelem, p_wet=0.5)
+    _check_replay(c0, f, lambda: run_config(600.0), "535-element storm, 10 min")
+
+
+def test_device_replays_host_sequence_reference_example(example):
+    p = example.para
+    f = StormForcing(10, seed=9, nelem=example.nelem, p_wet=0.5)
+    _check_replay(example, f, lambda: run_config(600.0, stepsize=p["stepsize"], etstep=p["lsm_step"],
+                                                 reltol=p["reltol"], abstol=p["abstol"],
+                                                 initstep=p["initstep"]),
+                  "reference example, 10 min")
