@@ -316,6 +316,49 @@ def producers_run(mesh, seed, iters=200):
     }
 
 
+NOAH_EXTRA_BYTES_PER_ELEM = 32   # smcmin, sh2o[0], fcr, gwet read by the _NOAH_ branches
+
+
+def noah_rhs_run(seed, iters=200):
+    """The RHS with the reference's _NOAH_ branches on the 100,000-triangle / 2,047-segment
+    workload, timed with HIP events beside the plain RHS on the same mesh and state."""
+    from pihm_amd.rhs import HipRHS
+    L = _lib.lib()
+    m = config_mesh("c2", seed=seed)
+    ne = m.nelem
+    rng = np.random.default_rng(seed)
+    h = HipRHS(m, nan_check=False)
+    h.set_step_inputs(pcpdrp=rng.uniform(0, 3e-6, ne), edir=rng.uniform(0, 2e-8, ne),
+                      ett=rng.uniform(0, 4e-8, ne), ws0_surf=rng.uniform(0, 2e-3, ne))
+    y0 = m.initial_state()
+    rc, _ = h.ode(y0)
+    ydev, yddev = L.N_VGetDeviceArrayPointer_Hip(h.y), L.N_VGetDeviceArrayPointer_Hip(h.yd)
+    ms = {}
+    for mode in ("plain", "noah"):
+        if mode == "noah":
+            smcmin = rng.uniform(0.01, 0.06, ne)
+            h.set_noah(smcmin)
+            nl = 11
+            h.set_noah_step(smcmin + m.a["porosity"] * rng.uniform(0.1, 0.9, ne), rng.uniform(0.5, 1.0, ne),
+                            rng.uniform(0.0, 8e-9, (ne, nl)), rng.integers(1, nl + 1, ne).astype(np.int32),
+                            np.full(ne, 4, np.int32))
+        t = C.c_double()
+        L.pihm_ctx_time_rhs(h.ctx, ydev, yddev, 5, C.byref(t))
+        L.pihm_ctx_time_rhs(h.ctx, ydev, yddev, iters, C.byref(t))
+        ms[mode] = t.value
+    h.close()
+    bytes_plain = BYTES_PER_ELEM * ne + BYTES_PER_SEG * m.nriver
+    bytes_noah = bytes_plain + NOAH_EXTRA_BYTES_PER_ELEM * ne
+    ach = bytes_noah / (ms["noah"] * 1e-3) / 1e9
+    return {"workload": f"c2 mesh: {ne} triangles, {m.nriver} river segments, NEQ {m.neq}",
+            "ms_per_eval": ms["noah"], "plain_ms_per_eval": ms["plain"],
+            "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_prep + k_flux<noah>)",
+                         "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah},
+            "note": "the per-step hand-over (sh2o[0], fcr, per-layer et) comes from a host Noah step; "
+                    "the Noah land-surface model itself is not on the device"}
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -386,6 +429,7 @@ def main():
     ap.add_argument("--ensemble-hours", type=float, default=1.0)
     ap.add_argument("--ensemble-sample", type=int, default=16)
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-noah", action="store_true")
     ap.add_argument("--closure-days", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
@@ -461,6 +505,8 @@ def main():
             out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
     out["producers"] = producers_run(mesh, args.seed)
+    if not args.no_noah:
+        out["noah_rhs"] = noah_rhs_run(args.seed)
     if not args.no_closure:
         out["closure"] = closure_run(args.closure_days)
     if not args.no_ensemble:

@@ -137,6 +137,29 @@ int  pihm_ctx_set_step_inputs(pihm_ctx *ctx, const pihm_step_inputs *in);
  * what the device producers (land-surface step, boundary series) wrote. */
 int  pihm_ctx_get_step_inputs(pihm_ctx *ctx, double *pcpdrp, double *edir, double *ett,
                               double *ws0_surf, double *elem_bc, double *river_bc);
+/* The RHS of the reference built with _NOAH_ (Flux-PIHM), for single solves:
+ *  - EtExtract (src/hydrol.c:43-56,79-84): no direct evaporation from ponded
+ *    water, and transpiration split by GwTransp (src/noah/lsm_func.c:163-185);
+ *  - Infil (src/vert_flow.c:94-98): the unsaturated branch takes the top
+ *    layer's saturation (sh2o[0] - smcmin) / (smcmax - smcmin), where
+ *    smcmax - smcmin is the mesh's porosity (src/init_soil.c:34-36);
+ *  - VerticalFlow (src/vert_flow.c:16-19): infiltration times the frozen-soil
+ *    factor fcr, which Recharge then reads.
+ * pihm_ctx_set_noah switches it on with each element's smcmin (caller order;
+ * NULL switches it off).  Until the first pihm_ctx_set_noah_step, fcr is 1,
+ * sh2o[0] is 0 and no transpiration comes from the saturated zone.
+ * pihm_ctx_set_noah_step hands over what the Noah step (src/noah/noah.c) left
+ * for the next model steps: sh2o0[ne] (top layer liquid water, m3 m-3),
+ * fcr[ne], and optionally et[ne*nlayer] (per-layer transpiration, element-
+ * major as the reference's wf.et[MAXLYR]), nwtbl[ne] and nroot[ne] (1-based
+ * water-table and root layers, nroot <= nlayer).  The saturated zone's share
+ * of transpiration is evaluated on the device from these and the ett already
+ * set for the step, so the call follows pihm_ctx_set_step_inputs.  NULL
+ * pointers leave those values as they are. */
+#define PIHM_NOAH_MAXLYR 11
+int  pihm_ctx_set_noah(pihm_ctx *ctx, const double *smcmin);
+int  pihm_ctx_set_noah_step(pihm_ctx *ctx, const double *sh2o0, const double *fcr, const double *et,
+                            int nlayer, const int32_t *nwtbl, const int32_t *nroot);
 int  pihm_ctx_neq(const pihm_ctx *ctx);
 
 /* Exact permutations between caller order and device order. `perm_out`

@@ -31,6 +31,8 @@ def lib():
         ("oracle_state_new", C.c_void_p, [C.c_int, C.c_int]),
         ("oracle_state_free", None, [C.c_void_p]),
         ("oracle_set_step_inputs", None, [C.c_void_p, C.POINTER(P.StepInputs)]),
+        ("oracle_set_noah", None, [C.c_void_p, dp, dp, dp, dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        ("oracle_GwTransp", C.c_double, [C.c_double, dp, C.c_int, C.c_int]),
         ("oracle_ode", C.c_int, [C.POINTER(P.MeshDesc), C.c_void_p, C.c_double, dp, dp, dp]),
         ("oracle_SurfH", C.c_double, [C.c_double]),
         ("oracle_KrFunc", C.c_double, [C.c_double, C.c_double]),
@@ -121,6 +123,21 @@ class OracleRHS:
         keep = []
         s = step_inputs(keep=keep, **kw)
         self.L.oracle_set_step_inputs(self.st, C.byref(s))
+
+    def set_noah(self, smcmin, sh2o0=None, fcr=None, et=None, nwtbl=None, nroot=None):
+        """The _NOAH_ branches of EtExtract and Infil (smcmin None: off).  et is
+        [nelem, nlayer]; nwtbl, nroot 1-based layers per element."""
+        if smcmin is None:
+            self.L.oracle_set_noah(self.st, None, None, None, None, 0, None, None)
+            return
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        i = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.intc)
+        smcmin, sh2o0, fcr, et, nwtbl, nroot = f(smcmin), f(sh2o0), f(fcr), f(et), i(nwtbl), i(nroot)
+        nl = 0 if et is None else et.shape[1]
+        ip = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+        self.L.oracle_set_noah(self.st, _dptr(smcmin), _dptr(sh2o0) if sh2o0 is not None else None,
+                               _dptr(fcr) if fcr is not None else None, _dptr(et) if et is not None else None,
+                               nl, ip(nwtbl), ip(nroot))
 
     def _field(self, name, n):
         class _S(C.Structure):

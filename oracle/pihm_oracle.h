@@ -33,11 +33,23 @@ typedef struct oracle_state {
   /* river states and fluxes */
   double *stage, *rgw;
   double *rivflow;                          /* [11*nriver] */
+  /* _NOAH_ build (oracle_set_noah): soil.smcmin, ws.sh2o[0], ps.fcr,
+   * wf.et[MAXLYR] per element (element-major, nlayer used), ps.nwtbl, ps.nroot,
+   * and the ps.gwet EtExtract leaves */
+  int noah, nlayer;
+  double *smcmin, *sh2o0, *fcr, *et, *gwet;
+  int *nwtbl, *nroot;
 } oracle_state;
 
 oracle_state *oracle_state_new(int nelem, int nriver);
 void oracle_state_free(oracle_state *s);
 void oracle_set_step_inputs(oracle_state *s, const pihm_step_inputs *in);
+/* Switch the _NOAH_ branches on (smcmin non-NULL; copies every array given,
+ * et as et[i*nlayer + j], keeping the previous values and layer count of
+ * those that are NULL) or off (smcmin NULL). */
+void oracle_set_noah(oracle_state *s, const double *smcmin, const double *sh2o0, const double *fcr,
+                     const double *et, int nlayer, const int *nwtbl, const int *nroot);
+double oracle_GwTransp(double ett, const double *et, int nwtbl, int nroot);
 
 /* ODE (src/ode.c:4-399).  Returns 0, or 1 at the first NaN component (where
  * the reference prints and exits, src/ode.c:401-410).  When `mag` is not

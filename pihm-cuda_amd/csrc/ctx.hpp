@@ -49,6 +49,11 @@ struct ElemDev {
   // per-step inputs
   double *pcpdrp, *edir, *ett, *ws0s, *ebc;   // ebc [3*ne]
   NbrRec *rec;
+  // Noah coupling (single solves, pihm_ctx_set_noah): static smcmin, per-step
+  // top-layer liquid water sh2o0, frozen-soil factor fcr, and the saturated
+  // zone's share of transpiration gwet (GwTransp of et[nlayer][ne], nwtbl, nroot)
+  double *smcmin, *sh2o0, *fcr, *gwet, *et;
+  int32_t *nwtbl, *nroot;
 };
 
 struct RiverDev {
@@ -187,6 +192,8 @@ struct pihm_ctx {
   int *nanflag_dev;
   int *nanflag_host;             // pinned
   double *stage_host;            // scratch for host-side permutations
+  int noah;                      // RHS with the reference's _NOAH_ branches
+  void *noah_arena;              // the Noah fields of ElemDev
 };
 
 // Caller-order index of device element p / device river r (the loaded slots

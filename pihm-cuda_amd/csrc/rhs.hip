@@ -201,7 +201,9 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
   rp[2] = make_double2(effkh, sfmag);
 }
 
-template <bool B>
+// N: the reference built with _NOAH_ (src/hydrol.c:43-56,79-84;
+// src/vert_flow.c:16-19,94-98), single solves only.
+template <bool B, bool N = false>
 __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const int ne = a.ne;
   const ElemDev &e = a.e;
@@ -249,11 +251,22 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
 
   // EtExtract: src/hydrol.c:33-97
   double edir_s, edir_u, edir_g, ett_u, ett_g;
-  if (surfh >= DEPRSTG) { edir_s = edir; edir_u = 0.0; edir_g = 0.0; }
-  else if (g > depth - dinf) { edir_s = 0.0; edir_u = 0.0; edir_g = edir; }
-  else { edir_s = 0.0; edir_u = edir; edir_g = 0.0; }
-  if (g > depth - rzd) { ett_u = 0.0; ett_g = ett; }
-  else { ett_u = ett; ett_g = 0.0; }
+  if (N) {
+    // no direct evaporation from ponded water; the saturated zone's share of
+    // transpiration, GwTransp (src/noah/lsm_func.c:163-185), is fixed per step
+    edir_s = 0.0;
+    if (g > depth - dinf) { edir_u = 0.0; edir_g = edir; }
+    else { edir_u = edir; edir_g = 0.0; }
+    const double gwet = e.gwet[p];
+    ett_u = (1.0 - gwet) * ett;
+    ett_g = gwet * ett;
+  } else {
+    if (surfh >= DEPRSTG) { edir_s = edir; edir_u = 0.0; edir_g = 0.0; }
+    else if (g > depth - dinf) { edir_s = 0.0; edir_u = 0.0; edir_g = edir; }
+    else { edir_s = 0.0; edir_u = edir; edir_g = 0.0; }
+    if (g > depth - rzd) { ett_u = 0.0; ett_g = ett; }
+    else { ett_u = ett; ett_g = 0.0; }
+  }
 
   // LateralFlow: src/lat_flow.c:29-63
   double ovl[3], sub[3];
@@ -342,11 +355,21 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
       kinf = eff_kinf(kinfv, kmacv, areafh, dh_by_dz, satkfunc, 1.0, applrate, surfh);
       infil = kinf * dh_by_dz;
     } else {
-      const double psi_u = (psi_raw > PSIMIN) ? psi_raw : PSIMIN;
+      double isatn = satn, ipsi = psi_raw, ikr = kr;
+      if (N) {
+        // saturation of the top soil layer; smcmax - smcmin is the porosity
+        // (src/init_soil.c:34-36)
+        isatn = (e.sh2o0[p] - e.smcmin[p]) / porosity;
+        isatn = (isatn > 1.0) ? 1.0 : isatn;
+        isatn = (isatn < SATMIN) ? SATMIN : isatn;
+        ipsi = psi_vg(isatn, alpha, beta);
+        ikr = kr_func(beta, isatn);
+      }
+      const double psi_u = (ipsi > PSIMIN) ? ipsi : PSIMIN;
       const double h_u = psi_u + zmax - 0.5 * dinf;
       dh_by_dz = (surfh + zmax - h_u) / (0.5 * (surfh + dinf));
       dh_by_dz = (surfh <= 0.0 && dh_by_dz > 0.0) ? 0.0 : dh_by_dz;
-      kinf = eff_kinf(kinfv, kmacv, areafh, dh_by_dz, kr, satn, applrate, surfh);
+      kinf = eff_kinf(kinfv, kmacv, areafh, dh_by_dz, ikr, isatn, applrate, surfh);
       infil = kinf * dh_by_dz;
       infil = (infil > 0.0) ? infil : 0.0;
     }
@@ -354,6 +377,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     infil = (infil > infil_max) ? infil_max : infil;
     infil *= wetfrac;
   }
+  if (N) infil *= e.fcr[p];   // frozen top soil (src/vert_flow.c:16-19), before Recharge reads it
 
   // Recharge: src/vert_flow.c:132-170
   double rechg;
@@ -647,18 +671,24 @@ __device__ __forceinline__ void river_ode(const RhsArgs &a, int r) {
 
 // Grid: nbr river blocks (river ODEs, short), then nbe element blocks.  At
 // most 168 VGPRs: three wavefronts per SIMD.
-template <bool B>
+template <bool B, bool N = false>
 __global__ __launch_bounds__(kBlock, 3) void k_flux(RhsArgs a, int nbe) {
   const int nbr = (int)gridDim.x - nbe;
   if ((int)blockIdx.x >= nbr) {
     const int tile = xcd_tile((int)blockIdx.x - nbr, nbe, a.tile_span);
     const int p = (B && a.bm) ? compact_index(a, 0, tile) : tile * kBlock + threadIdx.x;
-    if (p >= 0 && p < a.ne) elem_flux<B>(a, p);
+    if (p >= 0 && p < a.ne) elem_flux<B, N>(a, p);
   } else {
     const int rb = blockIdx.x;
     const int r = (B && a.bm) ? compact_index(a, 1, rb) : rb * kBlock + threadIdx.x;
     if (r >= 0 && r < a.nr) river_ode<B>(a, r);
   }
+}
+
+// k_flux of a single solve, with the _NOAH_ branches when the context has them.
+static void launch_flux_single(const pihm_ctx *c, const RhsArgs &a, int nbe, int nbr) {
+  if (c->noah) hipLaunchKernelGGL((k_flux<false, true>), dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  else hipLaunchKernelGGL((k_flux<false, false>), dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
 }
 
 // Water-balance accumulators: the four volume rates summed over the solve in
@@ -1172,6 +1202,7 @@ extern "C" int pihm_ctx_load_member(pihm_ctx *c, int k, const pihm_mesh_desc *m)
 extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
   if (!c) return;
   if (c->arena) (void)hipFree(c->arena);
+  if (c->noah_arena) (void)hipFree(c->noah_arena);
   if (c->nanflag_host) (void)hipHostFree(c->nanflag_host);
   delete c;
 }
@@ -1290,6 +1321,102 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   return PIHM_OK;
 }
 
+// GwTransp (src/noah/lsm_func.c:163-185) per element: the share of
+// transpiration drawn from the saturated zone, from the layers between the
+// water table layer nwtbl and the root layer nroot, summed in ascending order.
+__global__ __launch_bounds__(kBlock) void k_gwtransp(ElemDev e, int ne) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const double ett = e.ett[p];
+  const int nwtbl = e.nwtbl[p], nroot = e.nroot[p];
+  double gw = 0.0;
+  if (ett > 0.0 && nwtbl <= nroot) {
+    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) gw += e.et[(long)j * ne + p];
+    gw = gw / ett;
+    gw = (gw > 1.0) ? 1.0 : gw;
+    gw = (gw < 0.0) ? 0.0 : gw;
+  }
+  e.gwet[p] = gw;
+}
+
+extern "C" int pihm_ctx_set_noah(pihm_ctx *c, const double *smcmin) {
+  if (!c || c->nmem != 1) return PIHM_ERR_ARG;
+  if (!smcmin) {
+    c->noah = 0;
+    return PIHM_OK;
+  }
+  const long ne = c->ne;
+  ElemDev &E = c->args.e;
+  if (!c->noah_arena) {
+    Arena ar;
+    double **f[] = {&E.smcmin, &E.sh2o0, &E.fcr, &E.gwet};
+    for (auto q : f) ar.add(q, ne);
+    ar.add(&E.et, (long)PIHM_NOAH_MAXLYR * ne);
+    ar.add(&E.nwtbl, ne);
+    ar.add(&E.nroot, ne);
+    PIHM_HIP_TRY(hipMalloc(&c->noah_arena, ar.total()));
+    ar.carve((char *)c->noah_arena);
+  }
+  std::vector<double> hd(ne);
+  for (int p = 0; p < ne; p++) {
+    const long ci = caller_elem(c, p);
+    hd[p] = (ci >= 0) ? smcmin[ci] : 0.0;
+  }
+  PIHM_HIP_TRY(hipMemcpyAsync(E.smcmin, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  // until the first step's values: unfrozen soil, no saturated-zone transpiration
+  std::fill(hd.begin(), hd.end(), 1.0);
+  PIHM_HIP_TRY(hipMemcpyAsync(E.fcr, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  PIHM_HIP_TRY(hipMemsetAsync(E.gwet, 0, ne * sizeof(double), c->stream));
+  PIHM_HIP_TRY(hipMemsetAsync(E.sh2o0, 0, ne * sizeof(double), c->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->noah = 1;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_set_noah_step(pihm_ctx *c, const double *sh2o0, const double *fcr, const double *et,
+                                      int nlayer, const int32_t *nwtbl, const int32_t *nroot) {
+  if (!c || !c->noah) return PIHM_ERR_ARG;
+  const bool gw = et || nwtbl || nroot;
+  if (gw && (!et || !nwtbl || !nroot || nlayer < 1 || nlayer > PIHM_NOAH_MAXLYR)) return PIHM_ERR_ARG;
+  const long ne = c->ne;
+  ElemDev &E = c->args.e;
+  std::vector<double> hd(ne);
+  auto up = [&](double *dst, const double *src, long stride_src) -> int {
+    for (int p = 0; p < ne; p++) {
+      const long ci = caller_elem(c, p);
+      hd[p] = (ci >= 0) ? src[ci * stride_src] : 0.0;
+    }
+    PIHM_HIP_TRY(hipMemcpyAsync(dst, hd.data(), ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    return PIHM_OK;
+  };
+  int rc;
+  if (sh2o0 && (rc = up(E.sh2o0, sh2o0, 1))) return rc;
+  if (fcr && (rc = up(E.fcr, fcr, 1))) return rc;
+  if (gw) {
+    for (int j = 0; j < nlayer; j++)
+      if ((rc = up(E.et + (long)j * ne, et + j, nlayer))) return rc;
+    std::vector<int32_t> hi(ne);
+    const int32_t *srcs[] = {nwtbl, nroot};
+    int32_t *dsts[] = {E.nwtbl, E.nroot};
+    for (int f = 0; f < 2; f++) {
+      for (int p = 0; p < ne; p++) {
+        const long ci = caller_elem(c, p);
+        const int32_t v = (ci >= 0) ? srcs[f][ci] : 0;
+        if (f == 1 && v > nlayer) return PIHM_ERR_ARG;   // a root layer past the given layers
+        hi[p] = v;
+      }
+      PIHM_HIP_TRY(hipMemcpyAsync(dsts[f], hi.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+      PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    hipLaunchKernelGGL(k_gwtransp, dim3(div_up((int)ne, kBlock)), dim3(kBlock), 0, c->stream, E, (int)ne);
+    PIHM_HIP_TRY(hipGetLastError());
+  }
+  return PIHM_OK;
+}
+
 extern "C" int pihm_ctx_get_step_inputs(pihm_ctx *c, double *pcpdrp, double *edir, double *ett,
                                         double *ws0_surf, double *elem_bc, double *river_bc) {
   if (!c) return PIHM_ERR_ARG;
@@ -1364,7 +1491,7 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
     hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   } else {
     hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    launch_flux_single(c, a, nbe, nbr);
     if (a.closure) {
       const int nb = std::max(1, std::min(kClosureBlocks, div_up(std::max(c->ne, c->nr), kBlock)));
       hipLaunchKernelGGL(k_closure_part, dim3(nb), dim3(kBlock), 0, c->stream, a);
@@ -1542,7 +1669,7 @@ extern "C" int pihm_ctx_time_kernels(pihm_ctx *c, const double *y, double *ydot,
     else hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
     PIHM_HIP_TRY(hipEventRecord(ev[1], c->stream));
     if (c->nmem > 1) hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    else hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+    else launch_flux_single(c, a, nbe, nbr);
     PIHM_HIP_TRY(hipEventRecord(ev[2], c->stream));
     PIHM_HIP_TRY(hipEventSynchronize(ev[2]));
     for (int k = 0; k < 2; k++) {
@@ -1606,7 +1733,7 @@ int pihm_ctx_eval_fluxes(pihm_ctx *c, const double *y, double *ydot, double *car
   a.closure = 0;
   const int nbe = div_up(c->ne, kBlock), nbr = div_up(c->nr, kBlock);
   hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-  hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  launch_flux_single(c, a, nbe, nbr);
   PIHM_HIP_TRY(hipGetLastError());
   return pihm_ctx_carried_copy(c, carried_save, 1);
 }
@@ -1670,7 +1797,7 @@ extern "C" int pihm_ctx_time_rhs(pihm_ctx *c, const double *y, double *ydot, int
       hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     } else {
       hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-      hipLaunchKernelGGL(k_flux<false>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+      launch_flux_single(c, a, nbe, nbr);
     }
   }
   PIHM_HIP_TRY(hipGetLastError());

@@ -116,6 +116,7 @@ PrecSolveFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_
 PIHM_PREC_NONE = 0
 PIHM_PREC_RIGHT = 2
 PIHM_REC_ITERS = 8
+PIHM_NOAH_MAXLYR = 11
 PIHM_TUNE_SPANS = (0, 1, 2, 4, 8, 16, 32, 64)
 PIHM_ENS_TUNE_G = (1, 2, 4, 8)
 
@@ -160,6 +161,8 @@ _PROTOS = [
     ("pihm_stream_create", C.c_int, [C.POINTER(C.c_void_p)]),
     ("pihm_stream_destroy", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
+    ("pihm_ctx_set_noah", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_ctx_set_noah_step", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, C.c_int, c_int32_p, c_int32_p]),
     ("pihm_ctx_neq", C.c_int, [C.c_void_p]),
     ("pihm_ctx_get_perm", C.c_int, [C.c_void_p, c_int32_p]),
     ("pihm_ctx_state_to_device", C.c_int, [C.c_void_p, c_double_p, C.c_void_p]),

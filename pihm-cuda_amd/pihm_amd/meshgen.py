@@ -330,10 +330,13 @@ def make_watershed(n_tri, seed=0, lx=None, ly=None, n_seg=20, network="valley",
 
 
 def config_mesh(name, seed=0):
-    """The named workloads: 'c0' (535 triangles, 20 segments along the valley) and
-    'c1' (200,000 triangles, 4,095-segment binary-branching network)."""
+    """The named workloads: 'c0' (535 triangles, 20 segments along the valley),
+    'c1' (200,000 triangles, 4,095-segment binary-branching network) and 'c2'
+    (100,000 triangles, 2,047 segments: the Noah-coupled workload's mesh)."""
     if name == "c0":
         return make_watershed(535, seed=seed, n_seg=20, network="valley")
     if name == "c1":
         return make_watershed(200_000, seed=seed, n_seg=4095, network="tree")
+    if name == "c2":
+        return make_watershed(100_000, seed=seed, n_seg=2047, network="tree")
     raise ValueError(name)

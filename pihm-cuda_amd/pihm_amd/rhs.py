@@ -70,6 +70,34 @@ class HipRHS:
         if rc:
             raise RuntimeError(f"set_step_inputs failed ({rc})")
 
+    def set_noah(self, smcmin):
+        """Switch on the reference's _NOAH_ branches of EtExtract and Infil with each
+        element's smcmin (pihm_ctx_set_noah); None switches them off."""
+        if smcmin is None:
+            rc = self.L.pihm_ctx_set_noah(self.ctx, None)
+        else:
+            a = np.ascontiguousarray(smcmin, dtype=np.float64)
+            rc = self.L.pihm_ctx_set_noah(self.ctx, _dp(a))
+        if rc:
+            raise RuntimeError(f"pihm_ctx_set_noah failed ({rc})")
+
+    def set_noah_step(self, sh2o0=None, fcr=None, et=None, nwtbl=None, nroot=None):
+        """The Noah step's hand-over (pihm_ctx_set_noah_step): top-layer liquid water,
+        frozen-soil factor, and per-layer transpiration et[nelem, nlayer] with 1-based
+        water-table and root layers.  Call after set_step_inputs (GwTransp reads ett)."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        i = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+        sh2o0, fcr, et, nwtbl, nroot = f(sh2o0), f(fcr), f(et), i(nwtbl), i(nroot)
+        nl = 0 if et is None else et.shape[1]
+        rc = self.L.pihm_ctx_set_noah_step(
+            self.ctx, None if sh2o0 is None else _dp(sh2o0), None if fcr is None else _dp(fcr),
+            None if et is None else _dp(et), nl,
+            None if nwtbl is None else nwtbl.ctypes.data_as(_lib.c_int32_p),
+            None if nroot is None else nroot.ctypes.data_as(_lib.c_int32_p))
+        if rc:
+            raise RuntimeError(f"pihm_ctx_set_noah_step failed ({rc})")
+        return rc
+
     def ode(self, y, t=0.0):
         """ydot = f(t, y) (caller order).  Returns (rc, ydot); rc 1 = NaN in ydot."""
         y = np.ascontiguousarray(y, dtype=np.float64)
