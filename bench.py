@@ -373,7 +373,8 @@ def profile_kernels(iters, seed):
     """Short fixed workload for kernel traces and hardware-counter passes: `iters` RHS
     evaluations at C1 on a mid-storm state (then `iters` more timed with HIP events), then
     `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB read and 512 MiB
-    written per launch), the calibration kernel for the byte counters."""
+    written per launch), the calibration kernel for the byte counters.  Between the two,
+    the producers at C1 and the plain and _NOAH_ RHS at C2 (100,000 triangles)."""
     L = _lib.lib()
     mesh = config_mesh("c1", seed=seed)
     desc = mesh.desc()
@@ -399,6 +400,7 @@ def profile_kernels(iters, seed):
     bc.close()
     lsm.close()
     h.close()
+    noah = noah_rhs_run(seed, iters)
     n = 64 << 20
     a = L.N_VNew_Hip(n)
     b = L.N_VNew_Hip(n)
@@ -412,7 +414,10 @@ def profile_kernels(iters, seed):
     print(json.dumps({"profile_kernels": iters, "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
                       "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8,
                       "lsm_ms_per_step_events": ms_l.value, "bc_ms_per_apply_events": ms_b.value,
-                      "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem}))
+                      "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem,
+                      "noah_rhs_c2_ms_per_eval_events": noah["ms_per_eval"],
+                      "plain_rhs_c2_ms_per_eval_events": noah["plain_ms_per_eval"],
+                      "noah_rhs_c2_bytes_algorithmic": noah["roofline"]["algorithmic_bytes"]}))
 
 
 def main():

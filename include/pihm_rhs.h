@@ -155,7 +155,9 @@ int  pihm_ctx_get_step_inputs(pihm_ctx *ctx, double *pcpdrp, double *edir, doubl
  * water-table and root layers, nroot <= nlayer).  The saturated zone's share
  * of transpiration is evaluated on the device from these and the ett already
  * set for the step, so the call follows pihm_ctx_set_step_inputs.  NULL
- * pointers leave those values as they are. */
+ * pointers leave those values as they are.  Switch the branches on or off
+ * before an integration starts: Krylov iterations captured as HIP graphs
+ * (PihmCVodeSetGraphRhs) keep the kernel they were captured with. */
 #define PIHM_NOAH_MAXLYR 11
 int  pihm_ctx_set_noah(pihm_ctx *ctx, const double *smcmin);
 int  pihm_ctx_set_noah_step(pihm_ctx *ctx, const double *sh2o0, const double *fcr, const double *et,
