@@ -43,6 +43,11 @@ LSM_BYTES_PER_ELEM = 228
 LSM_STATIONS = 64
 
 
+def _log(msg):
+    """Progress of the legs, on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _cfg(t_end):
     return run_config(t_end, graphs=GRAPHS)
 
@@ -462,7 +467,9 @@ def main():
     S = args.cpu_hours
     global GRAPHS
     GRAPHS = 0 if args.no_graphs else 1
+    _log(f"device run: {W} + {K} simulated hours at {mesh.nelem} triangles")
     g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S])
+    _log("device run done")
     if g["rc"] != 0 or g["nan"] != 0:
         raise RuntimeError(f"device run failed: rc={g['rc']} nan={g['nan']} {g['stats']}")
     elapsed = g["marks"][float(W + K)] - g["marks"][float(W)]
@@ -499,6 +506,7 @@ def main():
     }
     if not args.no_cpu:
         nthreads = args.cpu_threads or len(os.sched_getaffinity(0))
+        _log(f"CPU baseline: {S} simulated hours, {nthreads} threads")
         c = cpu_run(mesh, forcing, S, nthreads)
         cpu_rate = (S / 24.0) / c["wall"]
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "sim-days/s", "cores": nthreads, "kind": "port",
@@ -509,14 +517,19 @@ def main():
             gs = g["marks"][S] - g["marks"][0.0]
             out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
             out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
+    _log("producers")
     out["producers"] = producers_run(mesh, args.seed)
     if not args.no_noah:
+        _log("RHS with the Noah branches at 100,000 triangles")
         out["noah_rhs"] = noah_rhs_run(args.seed)
     if not args.no_closure:
+        _log(f"closure: {args.closure_days} days at 535 triangles")
         out["closure"] = closure_run(args.closure_days)
     if not args.no_ensemble:
+        _log(f"ensemble: {args.ensemble_members} members")
         out["ensemble"] = ensemble_run(args.ensemble_members, args.ensemble_hours, args.ensemble_sample, 2024)
     if not args.no_serve:
+        _log(f"serving: {args.serve_seconds} s of arrivals")
         out["serving"] = serve_run(args.serve_seconds, args.serve_rate, args.serve_horizon_unit,
                                    args.serve_check, 77)
     print(json.dumps(out))
