@@ -33,6 +33,13 @@ def lib():
         ("oracle_set_step_inputs", None, [C.c_void_p, C.POINTER(P.StepInputs)]),
         ("oracle_set_noah", None, [C.c_void_p, dp, dp, dp, dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         ("oracle_GwTransp", C.c_double, [C.c_double, dp, C.c_int, C.c_int]),
+        ("oracle_FindWaterTable", C.c_int, [dp, C.c_int, C.c_double, dp]),
+        ("oracle_CalcLatFlx", None, [C.c_int, dp, C.c_double, dp]),
+        ("oracle_WDfCnd", None, [dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double]),
+        ("oracle_Rosr12", None, [dp, dp, dp, dp, dp, dp, C.c_int]),
+        ("oracle_noah_soil_step", None, [C.c_int, C.c_double, C.POINTER(C.c_int)] + [dp] * 12 +
+         [dp, dp, dp, C.POINTER(C.c_int), dp, dp, dp, dp]),
         ("oracle_ode", C.c_int, [C.POINTER(P.MeshDesc), C.c_void_p, C.c_double, dp, dp, dp]),
         ("oracle_SurfH", C.c_double, [C.c_double]),
         ("oracle_KrFunc", C.c_double, [C.c_double, C.c_double]),
@@ -355,3 +362,30 @@ class OracleLsmRun:
             self.L.oracle_state_free(self.st)
         except Exception:
             pass
+
+
+NS_MAXLYR = 11
+
+
+def noah_soil_step(dt, nsoil, sldpth, soil, gw, infil, runoff2, edir, et, state):
+    """The soil-water step of the _NOAH_ build for every element (noahsoil_oracle.c).
+    Per-layer arrays are [nelem, 11]; soil holds smcmax, smcmin, ksatv, alpha, beta, frzx;
+    state holds smc, sh2o, fcr, smflxv, runoff3 and runoff2_lyr (updated in place) and
+    receives nwtbl and satdpth."""
+    L = lib()
+    ne = len(gw)
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    ns = np.ascontiguousarray(nsoil, dtype=np.intc)
+    arrs = [f(sldpth)] + [f(soil[k]) for k in ("smcmax", "smcmin", "ksatv", "alpha", "beta", "frzx")] + \
+        [f(gw), f(infil), f(runoff2), f(edir), f(et)]
+    for k in ("smc", "sh2o", "smflxv", "runoff2_lyr"):
+        state[k] = f(state[k]).copy()
+    for k in ("fcr", "runoff3"):
+        state[k] = f(state[k]).copy()
+    state["nwtbl"] = np.zeros(ne, dtype=np.intc)
+    state["satdpth"] = np.zeros((ne, NS_MAXLYR))
+    L.oracle_noah_soil_step(ne, float(dt), ns.ctypes.data_as(C.POINTER(C.c_int)), *[_dptr(a) for a in arrs],
+                            _dptr(state["smc"]), _dptr(state["sh2o"]), _dptr(state["fcr"]),
+                            state["nwtbl"].ctypes.data_as(C.POINTER(C.c_int)), _dptr(state["satdpth"]),
+                            _dptr(state["runoff2_lyr"]), _dptr(state["smflxv"]), _dptr(state["runoff3"]))
+    return state

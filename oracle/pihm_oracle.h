@@ -51,6 +51,37 @@ void oracle_set_noah(oracle_state *s, const double *smcmin, const double *sh2o0,
                      const double *et, int nlayer, const int *nwtbl, const int *nroot);
 double oracle_GwTransp(double ett, const double *et, int nwtbl, int nroot);
 
+/* The soil-water step of the _NOAH_ build after each model step
+ * (noahsoil_oracle.c): one element's column. */
+typedef struct oracle_noah_col {
+  int nsoil;
+  const double *sldpth;          /* [nsoil] layer thicknesses (m) */
+  double smcmax, smcmin, ksatv, alpha, beta, frzx;
+  double gw, infil, runoff2, edir;   /* runoff2: MassBalance's subsurface runoff (m s-1) */
+  const double *et;              /* [nsoil] layer transpiration (m s-1) */
+} oracle_noah_col;
+typedef struct oracle_noah_state {
+  double smc[11], sh2o[11];      /* carried */
+  double fcr;                    /* carried: kept when every layer is saturated */
+  double smflxv[11], runoff3;    /* carried likewise */
+  int nwtbl;
+  double satdpth[11], runoff2_lyr[11];
+} oracle_noah_state;
+void oracle_noah_soil_column(const oracle_noah_col *in, oracle_noah_state *st, double dt);
+int oracle_FindWaterTable(const double *sldpth, int nsoil, double gw, double *satdpth);
+void oracle_CalcLatFlx(int nsoil, const double *satdpth, double runoff2, double *runoff2_lyr);
+void oracle_WDfCnd(double *wdf, double *wcnd, double smc, double sicemax, double smcmax, double smcmin,
+                   double ksatv, double alpha, double beta);
+void oracle_Rosr12(double *p, const double *a, const double *b, double *c, const double *d, double *delta,
+                   int nsoil);
+/* Every element: per-layer arrays element-major [i*11 + k]; smc, sh2o, fcr,
+ * smflxv and runoff3 are read and written. */
+void oracle_noah_soil_step(int ne, double dt, const int *nsoil, const double *sldpth, const double *smcmax,
+                           const double *smcmin, const double *ksatv, const double *alpha, const double *beta,
+                           const double *frzx, const double *gw, const double *infil, const double *runoff2,
+                           const double *edir, const double *et, double *smc, double *sh2o, double *fcr,
+                           int *nwtbl, double *satdpth, double *runoff2_lyr, double *smflxv, double *runoff3);
+
 /* ODE (src/ode.c:4-399).  Returns 0, or 1 at the first NaN component (where
  * the reference prints and exits, src/ode.c:401-410).  When `mag` is not
  * NULL it receives, per component, the sum of magnitudes of the terms

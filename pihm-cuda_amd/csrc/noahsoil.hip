@@ -1,0 +1,445 @@
+// noahsoil.hip - the _NOAH_ build's soil-water step after each model step.
+//
+// One thread per element runs the element's soil column in the reference's
+// order: Summary's FindWaterTable and CalcLatFlx (src/update.c:37-43), then
+// NoahHydrol (src/noah/noah.c:57-94): FindWaterTable again, the moisture
+// clamps, and SmFlx with SRT, SStep, Rosr12 and AdjSmProf.  The layer arrays
+// live in HBM layer-major ([k * ne + p]) so that the threads of a wavefront
+// read one layer of 64 neighbouring elements together; inside the thread the
+// column (at most 11 layers) is worked in private arrays.  Every expression
+// keeps the reference's operation order; powers use xpow (as the RHS does) and
+// exp is the device library's.  At the end the thread leaves in the context
+// what the next model step's RHS reads: sh2o[0], fcr, nwtbl and GwTransp's
+// share (src/noah/lsm_func.c:163-185) for the new water-table layer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/pihm_noahsoil.h"
+#include "common.hpp"
+#include "ctx.hpp"
+#include "physics.hpp"
+
+using namespace pihm;
+
+namespace {
+
+constexpr int ML = PIHM_NOAH_MAXLYR;
+constexpr double SH2OMIN = 0.02;   // src/include/pihm_const.h:81
+
+struct SoilDev {
+  int32_t *nsoil;
+  double *sldpth, *smcmax, *frzx;               // static
+  double *smc, *sh2o, *smflxv, *runoff2_lyr;    // carried, [ML * ne]
+  double *satdpth;                              // [ML * ne]
+  double *runoff3;                              // carried
+  double *infil, *runoff2;                      // the step's MassBalance values
+};
+
+// FindWaterTable: src/noah/lsm_func.c:4-55
+__device__ int find_water_table(const double *sld, int nsoil, double gw, double *sat) {
+  int layer = -999;
+  double dsum = 0.0, depth = 0.0;
+  for (int j = 0; j < ML; j++) sat[j] = 0.0;
+  for (int j = 0; j < nsoil; j++) depth += sld[j];
+  if (gw <= 0.0) {
+    layer = nsoil;
+    sat[nsoil - 1] = 1.0E-3;
+  } else if (gw > depth) {
+    layer = 0;
+    for (int j = 0; j < nsoil; j++) sat[j] = sld[j];
+  } else {
+    for (int j = nsoil - 1; j >= 0; j--) {
+      if (dsum + sld[j] > gw) {
+        sat[j] = gw - dsum;
+        layer = j + 1;
+        break;
+      }
+      sat[j] = sld[j];
+      dsum += sld[j];
+    }
+  }
+  return layer;
+}
+
+// WDfCnd: src/noah/noah.c:2813-2861
+__device__ void wdf_cnd(double *wdf, double *wcnd, double smc, double sicemax, double smcmax, double smcmin,
+                        double ksatv, double alpha, double beta) {
+  double factr1 = 0.05 / (smcmax - smcmin);
+  double factr2 = (smc - smcmin) / (smcmax - smcmin);
+  factr2 = (factr2 > 1.0 - 5.0e-4) ? 1.0 - 5.0e-4 : factr2;
+  factr2 = (factr2 < 0.0 + 5.0e-4) ? 5.0e-4 : factr2;
+  factr1 = (factr1 < factr2) ? factr1 : factr2;
+  const double expon = 1.0 - 1.0 / beta;
+  double satkfunc = kr_func(beta, factr2);
+  double dpsidsm = (1.0 - expon) / alpha / expon / (smcmax - smcmin) *
+                   xpow(xpow(factr2, -1.0 / expon) - 1.0, -expon) * xpow(factr2, -(1.0 / expon + 1.0));
+  *wcnd = ksatv * satkfunc;
+  *wdf = *wcnd * dpsidsm;
+  if (sicemax > 0.0) {
+    const double vkwgt = 1.0 / (1.0 + xpow(500.0 * sicemax, 3.0));
+    satkfunc = kr_func(beta, factr1);
+    dpsidsm = (1.0 - expon) / alpha / expon / (smcmax - smcmin) *
+              xpow(xpow(factr1, -1.0 / expon) - 1.0, -expon) * xpow(factr1, -(1.0 / expon + 1.0));
+    *wdf = vkwgt * *wdf + (1.0 - vkwgt) * dpsidsm * satkfunc * ksatv;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_noah_soil(int ne, SoilDev s, ElemDev e, const double *y, double dt) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const int nsoil = s.nsoil[p];
+  const double smcmax = s.smcmax[p], smcmin = e.smcmin[p];
+  const double ksatv = e.ksatv[p], alpha = e.alpha[p], beta = e.beta[p];
+  const double gw = y[2L * ne + p];
+  double sld[ML], zsoil[ML], smc[ML], sh2o[ML], sat[ML], r2l[ML], smflxv[ML], et[ML], sice[ML];
+  for (int k = 0; k < ML; k++) {
+    sld[k] = s.sldpth[(long)k * ne + p];
+    r2l[k] = s.runoff2_lyr[(long)k * ne + p];
+    smflxv[k] = s.smflxv[(long)k * ne + p];
+    smc[k] = s.smc[(long)k * ne + p];
+    sh2o[k] = s.sh2o[(long)k * ne + p];
+    et[k] = e.et[(long)k * ne + p];
+  }
+  // layer depths below the surface (src/noah/lsm_func.c:150-160)
+  zsoil[0] = -sld[0];
+  for (int k = 1; k < nsoil; k++) zsoil[k] = -sld[k] + zsoil[k - 1];
+  double fcr = e.fcr[p], runoff3 = s.runoff3[p];
+
+  // Summary: src/update.c:37-43 (runoff2 is MassBalance's subsurface runoff)
+  int nwtbl = find_water_table(sld, nsoil, gw, sat);
+  {
+    double sattot = 0.0;   // CalcLatFlx: src/noah/lsm_func.c:226-255
+    for (int k = 0; k < nsoil; k++) sattot += sat[k];
+    const double runoff2 = s.runoff2[p];
+    if (sattot <= 0.0) r2l[nsoil - 1] = runoff2;
+    else for (int k = 0; k < nsoil; k++) r2l[k] = sat[k] / sattot * runoff2;
+  }
+  // NoahHydrol: src/noah/noah.c:68-83
+  nwtbl = find_water_table(sld, nsoil, gw, sat);
+  for (int k = 0; k < nsoil; k++) {
+    smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : smcmin + SH2OMIN;
+    smc[k] = (smc[k] < smcmax) ? smc[k] : smcmax;
+    sh2o[k] = (sh2o[k] < smc[k]) ? sh2o[k] : smc[k];
+  }
+  // SmFlx: src/noah/noah.c:1671-1711
+  for (int k = 0; k < nsoil; k++) sice[k] = smc[k] - sh2o[k];
+  if (0 == nwtbl) {
+    for (int k = 0; k < nsoil; k++) {
+      smc[k] = smcmax;
+      sh2o[k] = smc[k] - sice[k];
+    }
+  } else {
+    double ai[ML], bi[ML], ci[ML], rhstt[ML];
+    // SRT: src/noah/noah.c:2315-2425
+    {
+      const double CVFRZ = 3.0;
+      double sicemax = 0.0;
+      for (int ks = 0; ks < nsoil; ks++) sicemax = (sice[ks] > sicemax) ? sice[ks] : sicemax;
+      double dice = -zsoil[0] * sice[0];
+      for (int ks = 1; ks < nsoil; ks++) dice += (zsoil[ks - 1] - zsoil[ks]) * sice[ks];
+      fcr = 1.0;
+      if (dice > 1.0e-2) {
+        const double acrt = CVFRZ * s.frzx[p] / dice;
+        double sum = 1.0;
+        const double ialp1 = (int)round(CVFRZ) - 1;
+        for (int j = 1; j < ialp1 + 1; j++) {
+          int kf = 1;
+          for (int jj = j + 1; jj < ialp1; jj++) kf *= jj;
+          sum += xpow(acrt, CVFRZ - (double)j) / (double)kf;
+        }
+        fcr = 1.0 - exp(-acrt) * sum;
+      }
+      const double pddum = s.infil[p];
+      double wdf, wcnd, wdf2, wcnd2, dsmdz2;
+      double dsmdz = (sh2o[0] - sh2o[1]) / (-0.5 * zsoil[1]);
+      wdf_cnd(&wdf, &wcnd, sh2o[0], sicemax, smcmax, smcmin, ksatv, alpha, beta);
+      double ddz = 1.0 / (-0.5 * zsoil[1]);
+      ai[0] = 0.0;
+      bi[0] = wdf * ddz / (-zsoil[0]);
+      ci[0] = -bi[0];
+      rhstt[0] = (wdf * dsmdz + wcnd - pddum + e.edir[p] + et[0]) / zsoil[0];
+      rhstt[0] += r2l[0] / zsoil[0];
+      double ddz2 = 0.0;
+      for (int k = 1; k < nsoil; k++) {
+        const double denom2 = (zsoil[k - 1] - zsoil[k]);
+        if (k < nsoil - 1) {
+          const double denom = zsoil[k - 1] - zsoil[k + 1];
+          dsmdz2 = (sh2o[k] - sh2o[k + 1]) / (denom * 0.5);
+          wdf_cnd(&wdf2, &wcnd2, sh2o[k], sicemax, smcmax, smcmin, ksatv, alpha, beta);
+          ddz2 = 2.0 / denom;
+          ci[k] = -wdf2 * ddz2 / denom2;
+        } else {
+          wdf2 = 0.0;
+          wcnd2 = 0.0;
+          dsmdz2 = 0.0;
+          ci[k] = 0.0;
+        }
+        double numer = (wdf2 * dsmdz2) + wcnd2 - (wdf * dsmdz) - wcnd + et[k];
+        numer += r2l[k];
+        rhstt[k] = numer / (-denom2);
+        ai[k] = -wdf * ddz / denom2;
+        bi[k] = -(ai[k] + ci[k]);
+        if (k != nsoil - 1) {
+          wdf = wdf2;
+          wcnd = wcnd2;
+          dsmdz = dsmdz2;
+          ddz = ddz2;
+        }
+      }
+    }
+    // SStep: src/noah/noah.c:2437-2480
+    double sh2o0[ML];
+    for (int k = 0; k < nsoil; k++) {
+      rhstt[k] *= dt;
+      ai[k] *= dt;
+      bi[k] = 1.0 + bi[k] * dt;
+      ci[k] *= dt;
+      sh2o0[k] = sh2o[k];
+      smflxv[k] = 0.0;
+    }
+    {
+      // Rosr12 (src/noah/noah.c:1572-1620) with p = ci, c = a copy of ci, d = a copy of rhstt, delta = rhstt
+      double cin[ML], din[ML];
+      for (int k = 0; k < nsoil; k++) {
+        din[k] = rhstt[k];
+        cin[k] = ci[k];
+      }
+      cin[nsoil - 1] = 0.0;
+      ci[0] = -cin[0] / bi[0];
+      rhstt[0] = din[0] / bi[0];
+      for (int k = 1; k < nsoil; k++) {
+        ci[k] = -cin[k] * (1.0 / (bi[k] + ai[k] * ci[k - 1]));
+        rhstt[k] = (din[k] - ai[k] * rhstt[k - 1]) * (1.0 / (bi[k] + ai[k] * ci[k - 1]));
+      }
+      ci[nsoil - 1] = rhstt[nsoil - 1];
+      for (int k = 1; k < nsoil; k++) {
+        const int kk = nsoil - k - 1;
+        ci[kk] = ci[kk] * ci[kk + 1] + rhstt[kk];
+      }
+    }
+    for (int k = 0; k < nsoil; k++) sh2o[k] += ci[k];
+    // AdjSmProf: src/noah/noah.c:3081-3172
+    {
+      double sh2omid[ML];
+      double wplus = 0.0, ddz;
+      runoff3 = 0.0;
+      for (int k = nsoil - 1; k >= 0; k--) {
+        ddz = (k != 0) ? zsoil[k - 1] - zsoil[k] : -zsoil[0];
+        sh2omid[k] = sh2o[k] + wplus / ddz;
+        const double stot = sh2omid[k] + sice[k];
+        if (stot > smcmax) {
+          smc[k] = smcmax;
+          wplus = (stot - smcmax) * ddz;
+        } else {
+          double stotmin = (sat[k] * smcmax + (sld[k] - sat[k]) * (smcmin + SH2OMIN)) / sld[k];
+          stotmin = (stotmin > smcmax) ? smcmax : stotmin;
+          stotmin = (stotmin < smcmin + SH2OMIN) ? (smcmin + SH2OMIN) : stotmin;
+          if (stot < stotmin) {
+            smc[k] = stotmin;
+            wplus = (stot - stotmin) * ddz;
+          } else {
+            smc[k] = stot;
+            wplus = 0.0;
+          }
+        }
+        sh2omid[k] = smc[k] - sice[k];
+      }
+      ddz = -zsoil[0];
+      for (int k = 0; k < nsoil; k++) {
+        if (k != 0) ddz = zsoil[k - 1] - zsoil[k];
+        sh2o[k] = sh2omid[k] + wplus / ddz;
+        const double stot = sh2o[k] + sice[k];
+        wplus = (stot > smcmax) ? (stot - smcmax) * ddz : 0.0;
+        smc[k] = (stot < smcmax) ? stot : smcmax;
+        smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : (smcmin + SH2OMIN);
+        sh2o[k] = smc[k] - sice[k];
+        sh2o[k] = (sh2o[k] > 0.0) ? sh2o[k] : 0.0;
+      }
+      runoff3 = wplus / dt;
+    }
+    for (int k = nsoil - 1; k > 0; k--)
+      smflxv[k - 1] = (sh2o[k] - sh2o0[k]) * sld[k] / dt + r2l[k] + et[k] + smflxv[k];
+  }
+
+  for (int k = 0; k < ML; k++) {
+    s.smc[(long)k * ne + p] = smc[k];
+    s.sh2o[(long)k * ne + p] = sh2o[k];
+    s.satdpth[(long)k * ne + p] = sat[k];
+    s.runoff2_lyr[(long)k * ne + p] = r2l[k];
+    s.smflxv[(long)k * ne + p] = smflxv[k];
+  }
+  s.runoff3[p] = runoff3;
+  // what the next model step's RHS reads
+  e.sh2o0[p] = sh2o[0];
+  e.fcr[p] = fcr;
+  e.nwtbl[p] = nwtbl;
+  const double ett = e.ett[p];
+  const int nroot = e.nroot[p];
+  double g = 0.0;
+  if (ett > 0.0 && nwtbl <= nroot) {
+    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) g += et[j];
+    g = g / ett;
+    g = (g > 1.0) ? 1.0 : g;
+    g = (g < 0.0) ? 0.0 : g;
+  }
+  e.gwet[p] = g;
+}
+
+}  // namespace
+
+struct pihm_noah_soil {
+  pihm_ctx *ctx;
+  int ne;
+  SoilDev d;
+  void *arena;
+};
+
+namespace {
+
+// caller-order host values of one field -> device order at dst (layers: ML
+// per element, element-major on the host, layer-major on the device)
+int upload(pihm_noah_soil *s, double *dst, const double *src, int layers) {
+  const int ne = s->ne;
+  std::vector<double> h((size_t)layers * ne);
+  for (int p = 0; p < ne; p++) {
+    const long ci = s->ctx->perm[p];
+    for (int k = 0; k < layers; k++) h[(size_t)k * ne + p] = src[ci * layers + k];
+  }
+  PIHM_HIP_TRY(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return PIHM_OK;
+}
+
+template <class T>
+int download(pihm_noah_soil *s, const T *src, T *dst, int layers) {
+  const int ne = s->ne;
+  std::vector<T> h((size_t)layers * ne);
+  PIHM_HIP_TRY(hipMemcpyAsync(h.data(), src, h.size() * sizeof(T), hipMemcpyDeviceToHost, s->ctx->stream));
+  PIHM_HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  for (int p = 0; p < ne; p++) {
+    const long ci = s->ctx->perm[p];
+    for (int k = 0; k < layers; k++) dst[ci * layers + k] = h[(size_t)k * ne + p];
+  }
+  return PIHM_OK;
+}
+
+int launch(pihm_noah_soil *s, double dt, const double *y_dev) {
+  pihm_ctx *c = s->ctx;
+  hipLaunchKernelGGL(k_noah_soil, dim3(div_up(s->ne, kBlock)), dim3(kBlock), 0, c->stream, s->ne, s->d, c->args.e,
+                     y_dev, dt);
+  PIHM_HIP_TRY(hipGetLastError());
+  return PIHM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pihm_noah_soil_create(pihm_ctx *ctx, const pihm_noah_soil_desc *desc, pihm_noah_soil **out) {
+  if (!out) return PIHM_ERR_ARG;
+  *out = nullptr;
+  if (!ctx || !desc || ctx->nmem != 1 || !ctx->noah) return PIHM_ERR_ARG;
+  if (!desc->nsoil || !desc->sldpth || !desc->smcmax || !desc->frzx) return PIHM_ERR_ARG;
+  const int ne = ctx->ne;
+  for (int i = 0; i < ne; i++) {
+    const int n = desc->nsoil[i];
+    if (n < 2 || n > ML) return PIHM_ERR_ARG;
+    for (int k = 0; k < n; k++)
+      if (!(desc->sldpth[(long)i * ML + k] > 0.0)) return PIHM_ERR_ARG;
+  }
+  pihm_noah_soil *s = new pihm_noah_soil();
+  s->ctx = ctx;
+  s->ne = ne;
+  SoilDev &D = s->d;
+  const size_t b1 = ((size_t)ne * sizeof(double) + 255) & ~size_t(255);
+  const size_t bl = ((size_t)ML * ne * sizeof(double) + 255) & ~size_t(255);
+  const size_t bi = ((size_t)ne * sizeof(int32_t) + 255) & ~size_t(255);
+  double **one[] = {&D.smcmax, &D.frzx, &D.runoff3, &D.infil, &D.runoff2};
+  double **lay[] = {&D.sldpth, &D.smc, &D.sh2o, &D.smflxv, &D.runoff2_lyr, &D.satdpth};
+  const size_t total = 5 * b1 + 6 * bl + bi;
+  if (hipMalloc(&s->arena, total) != hipSuccess) { delete s; return PIHM_ERR_HIP; }
+  char *q = (char *)s->arena;
+  for (auto f : one) { *f = (double *)q; q += b1; }
+  for (auto f : lay) { *f = (double *)q; q += bl; }
+  D.nsoil = (int32_t *)q;
+  int rc = PIHM_OK;
+  if (hipMemsetAsync(s->arena, 0, total, ctx->stream) != hipSuccess) rc = PIHM_ERR_HIP;
+  std::vector<int32_t> hn(ne);
+  for (int p = 0; p < ne; p++) hn[p] = desc->nsoil[ctx->perm[p]];
+  if (!rc && hipMemcpyAsync(D.nsoil, hn.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = PIHM_ERR_HIP;
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = PIHM_ERR_HIP;
+  if (!rc) rc = upload(s, D.sldpth, desc->sldpth, ML);
+  if (!rc) rc = upload(s, D.smcmax, desc->smcmax, 1);
+  if (!rc) rc = upload(s, D.frzx, desc->frzx, 1);
+  if (rc) { pihm_noah_soil_destroy(s); return rc; }
+  *out = s;
+  return PIHM_OK;
+}
+
+void pihm_noah_soil_destroy(pihm_noah_soil *s) {
+  if (!s) return;
+  if (s->arena) (void)hipFree(s->arena);
+  delete s;
+}
+
+int pihm_noah_soil_set_state(pihm_noah_soil *s, const double *smc, const double *sh2o, const double *fcr) {
+  if (!s || !smc || !sh2o) return PIHM_ERR_ARG;
+  int rc;
+  if ((rc = upload(s, s->d.smc, smc, ML))) return rc;
+  if ((rc = upload(s, s->d.sh2o, sh2o, ML))) return rc;
+  ElemDev &E = s->ctx->args.e;
+  if (fcr && (rc = upload(s, E.fcr, fcr, 1))) return rc;
+  // the RHS's top-layer water: sh2o[0] of each element
+  std::vector<double> top(s->ne);
+  for (int i = 0; i < s->ne; i++) top[i] = sh2o[(long)i * ML];
+  return upload(s, E.sh2o0, top.data(), 1);
+}
+
+int pihm_noah_soil_get_state(pihm_noah_soil *s, double *smc, double *sh2o, double *fcr, int32_t *nwtbl,
+                             double *satdpth, double *runoff2_lyr, double *smflxv, double *runoff3) {
+  if (!s) return PIHM_ERR_ARG;
+  const ElemDev &E = s->ctx->args.e;
+  int rc = PIHM_OK;
+  if (!rc && smc) rc = download(s, s->d.smc, smc, ML);
+  if (!rc && sh2o) rc = download(s, s->d.sh2o, sh2o, ML);
+  if (!rc && fcr) rc = download(s, (const double *)E.fcr, fcr, 1);
+  if (!rc && nwtbl) rc = download(s, (const int32_t *)E.nwtbl, nwtbl, 1);
+  if (!rc && satdpth) rc = download(s, s->d.satdpth, satdpth, ML);
+  if (!rc && runoff2_lyr) rc = download(s, s->d.runoff2_lyr, runoff2_lyr, ML);
+  if (!rc && smflxv) rc = download(s, s->d.smflxv, smflxv, ML);
+  if (!rc && runoff3) rc = download(s, s->d.runoff3, runoff3, 1);
+  return rc;
+}
+
+int pihm_noah_soil_step(pihm_noah_soil *s, double dt, const double *y_dev, const double *infil,
+                        const double *runoff2) {
+  if (!s || !y_dev || !infil || !runoff2 || !(dt > 0.0) || !s->ctx->noah) return PIHM_ERR_ARG;
+  int rc;
+  if ((rc = upload(s, s->d.infil, infil, 1))) return rc;
+  if ((rc = upload(s, s->d.runoff2, runoff2, 1))) return rc;
+  return launch(s, dt, y_dev);
+}
+
+int pihm_noah_soil_time(pihm_noah_soil *s, double dt, const double *y_dev, int iters, double *ms) {
+  if (!s || !y_dev || iters < 1 || !ms || !(dt > 0.0) || !s->ctx->noah) return PIHM_ERR_ARG;
+  hipStream_t st = s->ctx->stream;
+  hipEvent_t ev[2];
+  for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
+  PIHM_HIP_TRY(hipEventRecord(ev[0], st));
+  for (int it = 0; it < iters; it++) {
+    const int rc = launch(s, dt, y_dev);
+    if (rc) return rc;
+  }
+  PIHM_HIP_TRY(hipEventRecord(ev[1], st));
+  PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+  float t = 0.0f;
+  PIHM_HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms = (double)t / iters;
+  return PIHM_OK;
+}
+
+}  // extern "C"

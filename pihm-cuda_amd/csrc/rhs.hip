@@ -1356,6 +1356,8 @@ extern "C" int pihm_ctx_set_noah(pihm_ctx *c, const double *smcmin) {
     ar.add(&E.nroot, ne);
     PIHM_HIP_TRY(hipMalloc(&c->noah_arena, ar.total()));
     ar.carve((char *)c->noah_arena);
+    // layers never handed over read as no transpiration; no water table or roots yet
+    PIHM_HIP_TRY(hipMemsetAsync(c->noah_arena, 0, ar.total(), c->stream));
   }
   std::vector<double> hd(ne);
   for (int p = 0; p < ne; p++) {

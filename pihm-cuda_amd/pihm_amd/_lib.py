@@ -84,6 +84,10 @@ class RunStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class NoahSoilDesc(C.Structure):
+    _fields_ = [("nsoil", c_int32_p), ("sldpth", c_double_p), ("smcmax", c_double_p), ("frzx", c_double_p)]
+
+
 class LsmDesc(C.Structure):
     """pihm_lsm_desc (include/pihm_lsm.h)."""
 
@@ -162,6 +166,13 @@ _PROTOS = [
     ("pihm_stream_destroy", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
     ("pihm_ctx_set_noah", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_noah_soil_create", C.c_int, [C.c_void_p, C.POINTER(NoahSoilDesc), C.POINTER(C.c_void_p)]),
+    ("pihm_noah_soil_destroy", None, [C.c_void_p]),
+    ("pihm_noah_soil_set_state", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p]),
+    ("pihm_noah_soil_get_state", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_int32_p, c_double_p,
+                                           c_double_p, c_double_p, c_double_p]),
+    ("pihm_noah_soil_step", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, c_double_p, c_double_p]),
+    ("pihm_noah_soil_time", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     ("pihm_ctx_set_noah_step", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, C.c_int, c_int32_p, c_int32_p]),
     ("pihm_ctx_neq", C.c_int, [C.c_void_p]),
     ("pihm_ctx_get_perm", C.c_int, [C.c_void_p, c_int32_p]),

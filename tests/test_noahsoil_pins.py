@@ -1,0 +1,233 @@
+"""The soil-water step of the _NOAH_ build (Summary's FindWaterTable and CalcLatFlx, then
+NoahHydrol: SmFlx with SRT, SStep, Rosr12 and AdjSmProf), restated in
+oracle/noahsoil_oracle.c, pinned on the CPU.
+
+The reference ships no tests or outputs for these functions.  The pins:
+* Rosr12 against scipy's banded solver;
+* WDfCnd against the van Genuchten-Mualem closed forms: the conductivity is Ksat times
+  Mualem's Kr, the diffusivity is K times dpsi/dtheta (checked by differentiating the
+  van Genuchten retention curve numerically), and the ice weighting's limits;
+* FindWaterTable's three cases and CalcLatFlx's proportional split;
+* the column's water budget: when AdjSmProf has nothing to adjust, the change of storage
+  equals dt times (infiltration - soil evaporation - transpiration - lateral runoff), since
+  the bottom of the column is closed; the layer fluxes smflxv add up to the same;
+* the direction of drainage, the frozen-soil factor's closed form, and the saturated
+  column's special case.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.linalg import solve_banded
+
+import oracle as O
+
+OL = O.lib()
+M = 11
+
+
+def _p(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(O.dp)
+
+
+def test_rosr12_solves_tridiagonal():
+    rng = np.random.default_rng(1)
+    for n in range(2, 12):
+        a = rng.uniform(-1, 0, n)
+        c = rng.uniform(-1, 0, n)
+        a[0] = 0.0
+        c[-1] = 0.0
+        b = 1.0 + np.abs(a) + np.abs(c) + rng.uniform(0, 1, n)
+        d = rng.normal(size=n)
+        ab = np.zeros((3, n))
+        ab[0, 1:] = c[:-1]
+        ab[1] = b
+        ab[2, :-1] = a[1:]
+        want = solve_banded((1, 1), ab, d)
+        p = np.zeros(n)
+        cc = c.copy()
+        delta = np.zeros(n)
+        OL.oracle_Rosr12(p.ctypes.data_as(O.dp), _p(a), _p(b), cc.ctypes.data_as(O.dp), _p(d),
+                         delta.ctypes.data_as(O.dp), n)
+        assert np.allclose(p, want, rtol=1e-13, atol=1e-15 * np.max(np.abs(want)))
+
+
+def _wdfcnd(smc, sicemax, smcmax, smcmin, ksatv, alpha, beta):
+    wdf, wcnd = C.c_double(), C.c_double()
+    OL.oracle_WDfCnd(C.byref(wdf), C.byref(wcnd), smc, sicemax, smcmax, smcmin, ksatv, alpha, beta)
+    return wdf.value, wcnd.value
+
+
+def _mualem(s, n):
+    m = 1.0 - 1.0 / n
+    return np.sqrt(s) * (1.0 - (1.0 - s ** (1.0 / m)) ** m) ** 2
+
+
+def _psi(theta, thr, ths, alpha, n):
+    """van Genuchten retention: pressure head (m, negative) at water content theta."""
+    m = 1.0 - 1.0 / n
+    s = (theta - thr) / (ths - thr)
+    return -((s ** (-1.0 / m) - 1.0) ** (1.0 / n)) / alpha
+
+
+@pytest.mark.parametrize("alpha,n", [(1.5, 1.4), (3.0, 2.2), (0.8, 1.15)])
+def test_wdfcnd_is_van_genuchten_mualem(alpha, n):
+    smcmax, smcmin, ks = 0.45, 0.05, 2e-6
+    for s in (0.05, 0.3, 0.7, 0.95):
+        theta = smcmin + s * (smcmax - smcmin)
+        wdf, wcnd = _wdfcnd(theta, 0.0, smcmax, smcmin, ks, alpha, n)
+        assert wcnd == pytest.approx(ks * _mualem(s, n), rel=1e-12)
+        h = 1e-7
+        dpsi = (_psi(theta + h, smcmin, smcmax, alpha, n) - _psi(theta - h, smcmin, smcmax, alpha, n)) / (2 * h)
+        assert wdf / wcnd == pytest.approx(dpsi, rel=1e-6)
+    # saturation is held 5e-4 inside (0, 1)
+    lo = _wdfcnd(smcmin - 0.01, 0.0, smcmax, smcmin, ks, alpha, n)
+    assert lo == _wdfcnd(smcmin + 5e-4 * (smcmax - smcmin), 0.0, smcmax, smcmin, ks, alpha, n)
+    hi = _wdfcnd(smcmax + 0.01, 0.0, smcmax, smcmin, ks, alpha, n)
+    assert hi[1] == pytest.approx(ks * _mualem(1.0 - 5e-4, n), rel=1e-12)
+
+
+def test_wdfcnd_ice_weighting():
+    smcmax, smcmin, ks, alpha, n = 0.45, 0.05, 2e-6, 2.0, 1.6
+    theta = 0.3
+    liq = _wdfcnd(theta, 0.0, smcmax, smcmin, ks, alpha, n)
+    s1 = 0.05 / (smcmax - smcmin)
+    h = 1e-8
+    t1 = smcmin + s1 * (smcmax - smcmin)
+    dpsi1 = (_psi(t1 + h, smcmin, smcmax, alpha, n) - _psi(t1 - h, smcmin, smcmax, alpha, n)) / (2 * h)
+    d_ice = dpsi1 * _mualem(s1, n) * ks
+    for sice in (1e-4, 2e-3, 0.01, 0.2):
+        wdf, wcnd = _wdfcnd(theta, sice, smcmax, smcmin, ks, alpha, n)
+        assert wcnd == liq[1]                      # conductivity keeps the liquid content
+        w = 1.0 / (1.0 + (500.0 * sice) ** 3)
+        assert wdf == pytest.approx(w * liq[0] + (1 - w) * d_ice, rel=1e-6)
+    # much ice: the diffusivity of the 5 % content
+    assert _wdfcnd(theta, 0.5, smcmax, smcmin, ks, alpha, n)[0] == pytest.approx(d_ice, rel=1e-6)
+
+
+def _fwt(sld, gw):
+    sat = np.zeros(M)
+    s = np.zeros(M)
+    s[:len(sld)] = sld
+    layer = OL.oracle_FindWaterTable(s.ctypes.data_as(O.dp), len(sld), gw, sat.ctypes.data_as(O.dp))
+    return layer, sat
+
+
+def test_find_water_table_and_lateral_split():
+    sld = np.array([0.1, 0.3, 0.6, 1.0, 1.0])
+    assert _fwt(sld, -0.5) == (5, pytest.approx(np.r_[0, 0, 0, 0, 1e-3, np.zeros(6)]))
+    layer, sat = _fwt(sld, 3.5)
+    assert layer == 0 and np.array_equal(sat[:5], sld)
+    # gw is the saturated thickness above the bottom: 1.4 m fills the last layer (1 m)
+    # and 0.4 m of the fourth, which is the water-table layer (1-based)
+    layer, sat = _fwt(sld, 1.4)
+    assert layer == 4
+    assert np.allclose(sat[:5], [0.0, 0.0, 0.0, 0.4, 1.0], rtol=0, atol=1e-15)
+    assert sat.sum() == pytest.approx(1.4, abs=1e-15)
+    r = np.zeros(M)
+    OL.oracle_CalcLatFlx(5, sat.ctypes.data_as(O.dp), 3e-7, r.ctypes.data_as(O.dp))
+    assert r[:5].sum() == pytest.approx(3e-7, rel=1e-15)
+    assert np.allclose(r[:5], sat[:5] / sat[:5].sum() * 3e-7, rtol=1e-15, atol=0)
+
+
+def _column_inputs(ne, rng, nsoil=6):
+    sld = np.zeros((ne, M))
+    sld[:, :nsoil] = rng.uniform(0.1, 0.6, (ne, nsoil))
+    soil = dict(smcmax=np.full(ne, 0.45), smcmin=np.full(ne, 0.05), ksatv=rng.uniform(1e-7, 5e-6, ne),
+                alpha=rng.uniform(1.0, 3.0, ne), beta=rng.uniform(1.3, 2.0, ne), frzx=np.full(ne, 0.15))
+    return sld, soil
+
+
+def _state(ne, sh2o, smc=None, fcr=1.0):
+    return dict(smc=sh2o.copy() if smc is None else smc, sh2o=sh2o.copy(), fcr=np.full(ne, fcr),
+                smflxv=np.zeros((ne, M)), runoff3=np.zeros(ne), runoff2_lyr=np.zeros((ne, M)))
+
+
+def test_column_water_budget_closes():
+    """No clamp or profile adjustment active: storage change = dt * net boundary flux."""
+    rng = np.random.default_rng(5)
+    ne, ns, dt = 64, 6, 60.0
+    sld, soil = _column_inputs(ne, rng, ns)
+    sh2o = np.zeros((ne, M))
+    sh2o[:, :ns] = rng.uniform(0.15, 0.35, (ne, ns))
+    gw = 0.05 * sld[:, ns - 1]                       # the water table inside the bottom layer
+    infil = rng.uniform(0, 2e-6, ne)
+    edir = rng.uniform(0, 5e-8, ne)
+    runoff2 = rng.uniform(-1e-8, 1e-7, ne)
+    et = np.zeros((ne, M))
+    et[:, :ns] = rng.uniform(0, 2e-8, (ne, ns))
+    st0 = _state(ne, sh2o)
+    st = O.noah_soil_step(dt, np.full(ne, ns), sld, soil, gw, infil, runoff2, edir, et, _state(ne, sh2o))
+    assert np.all(st["nwtbl"] == ns)
+    assert np.all(st["runoff3"] == 0.0)
+    assert np.all(st["sh2o"][:, :ns] > 0.07) and np.all(st["sh2o"][:, :ns] < 0.45)
+    dstore = np.sum((st["sh2o"] - st0["sh2o"]) * sld, axis=1)
+    net = dt * (infil - edir - et.sum(axis=1) - st["runoff2_lyr"].sum(axis=1))
+    scale = dt * (infil + edir + et.sum(axis=1) + np.abs(runoff2))
+    assert np.all(np.abs(dstore - net) <= 1e-11 * scale)
+    # the lateral runoff leaves from the saturated layers only
+    assert np.allclose(st["runoff2_lyr"].sum(axis=1), runoff2, rtol=1e-14, atol=0)
+    assert np.all(st["runoff2_lyr"][:, :ns - 1] == 0.0)
+    # layer fluxes: the top layer's budget closes with the flux out of its bottom
+    top = (st["sh2o"][:, 0] - st0["sh2o"][:, 0]) * sld[:, 0] / dt + st["runoff2_lyr"][:, 0] + et[:, 0] + \
+        st["smflxv"][:, 0]
+    assert np.allclose(top, infil - edir, rtol=0, atol=1e-11 * np.max(scale / dt))
+
+
+def test_column_drains_downward():
+    """A uniform moisture profile with no boundary fluxes: gravity moves water down."""
+    rng = np.random.default_rng(6)
+    ne, ns = 16, 5
+    sld, soil = _column_inputs(ne, rng, ns)
+    sh2o = np.zeros((ne, M))
+    sh2o[:, :ns] = 0.3
+    z = np.zeros(ne)
+    st = O.noah_soil_step(600.0, np.full(ne, ns), sld, soil, 0.02 * sld[:, ns - 1], z, z, z, np.zeros((ne, M)),
+                          _state(ne, sh2o))
+    assert np.all(st["sh2o"][:, 0] < 0.3) and np.all(st["sh2o"][:, ns - 1] > 0.3)
+    assert np.all(st["smflxv"][:, :ns - 1] > 0.0)       # positive: out of the layer, downward
+
+
+def test_frozen_soil_factor_and_saturated_column():
+    rng = np.random.default_rng(7)
+    ne, ns = 8, 4
+    sld, soil = _column_inputs(ne, rng, ns)
+    sh2o = np.zeros((ne, M))
+    sh2o[:, :ns] = 0.2
+    smc = sh2o.copy()
+    ice = np.linspace(0.0, 0.15, ne)
+    smc[:, :ns] += ice[:, None]                      # smc - sh2o is ice
+    z = np.zeros(ne)
+    st = O.noah_soil_step(60.0, np.full(ne, ns), sld, soil, 0.1 * sld[:, ns - 1], z, z, z, np.zeros((ne, M)),
+                          _state(ne, sh2o, smc, fcr=0.123))
+    dice = ice * sld[:, :ns].sum(axis=1)
+    acrt = 3.0 * soil["frzx"] / np.where(dice > 0, dice, 1.0)
+    # SRT's series with CVFRZ = 3: 1 + acrt^2 + acrt (its factorial loop leaves k = 1)
+    want = np.where(dice > 1e-2, 1.0 - np.exp(-acrt) * (1.0 + acrt ** 2 + acrt), 1.0)
+    assert np.allclose(st["fcr"], want, rtol=1e-14, atol=0)
+    assert st["fcr"][0] == 1.0 and st["fcr"][-1] < 1.0
+    # a water table above the surface: every layer saturated, the ice kept, fcr and fluxes untouched
+    st2 = O.noah_soil_step(60.0, np.full(ne, ns), sld, soil, 2.0 * sld.sum(axis=1), z, z, z, np.zeros((ne, M)),
+                           _state(ne, sh2o, smc, fcr=0.123))
+    assert np.all(st2["nwtbl"] == 0)
+    assert np.all(st2["smc"][:, :ns] == 0.45)
+    assert np.allclose(st2["sh2o"][:, :ns], 0.45 - ice[:, None], rtol=0, atol=1e-15)
+    assert np.all(st2["fcr"] == 0.123) and np.all(st2["smflxv"] == 0.0)
+
+
+def test_interlayer_flux_is_darcy_richards():
+    """Over a short step the flux out of the top layer is K(theta0) + D(theta0) * (theta0 -
+    theta1) / (distance between the layer centres), the Richards flux of the upper layer's
+    conductivity and diffusivity."""
+    rng = np.random.default_rng(8)
+    ne, ns = 6, 3
+    sld, soil = _column_inputs(ne, rng, ns)
+    sh2o = np.zeros((ne, M))
+    sh2o[:, :ns] = [0.35, 0.25, 0.2]
+    z = np.zeros(ne)
+    st = O.noah_soil_step(1e-3, np.full(ne, ns), sld, soil, 0.05 * sld[:, ns - 1], z, z, z, np.zeros((ne, M)),
+                          _state(ne, sh2o))
+    for i in range(ne):
+        wdf, wcnd = _wdfcnd(0.35, 0.0, 0.45, 0.05, soil["ksatv"][i], soil["alpha"][i], soil["beta"][i])
+        want = wcnd + wdf * (0.35 - 0.25) / (0.5 * (sld[i, 0] + sld[i, 1]))
+        assert st["smflxv"][i, 0] == pytest.approx(want, rel=1e-6)
