@@ -22,6 +22,7 @@
 
 #include <stdint.h>
 
+#include "pihm_driver.h"
 #include "pihm_rhs.h"
 
 #ifdef __cplusplus
@@ -56,6 +57,21 @@ int  pihm_noah_soil_get_state(pihm_noah_soil *s, double *smc, double *sh2o, doub
  * 107-160; m s-1, caller order), which Summary hands to CalcLatFlx and SRT. */
 int  pihm_noah_soil_step(pihm_noah_soil *s, double dt, const double *y_dev, const double *infil,
                          const double *runoff2);
+/* The same step with its inputs taken on the device, as Summary does
+ * (src/update.c:3-60, MassBalance :107-160): the fluxes are evaluated at y_dev
+ * (the carried river-edge flux is restored afterwards), the infiltration and
+ * subsurface runoff are re-derived from the storage change since the previous
+ * call, with the _NOAH_ EtExtract split, and the soil-water step follows.
+ * pihm_noah_soil_start records the initial storage (the first Summary). */
+int  pihm_noah_soil_start(pihm_noah_soil *s, const double *y_dev);
+int  pihm_noah_soil_summary(pihm_noah_soil *s, double dt, const double *y_dev);
+/* Runs pihm_noah_soil_start at the driver's first Summary and
+ * pihm_noah_soil_summary (with ctrl.stepsize) at every later one; NULL detaches. */
+int  pihm_gpu_backend_set_noah_soil(pihm_gpu_backend *b, pihm_noah_soil *s);
+/* pihm_run_gpu with the soil-water step at every Summary (the context's
+ * _NOAH_ branches on); the land-surface values come from the forcing. */
+int  pihm_run_gpu_noah(pihm_ctx *ctx, const pihm_forcing *forc, pihm_noah_soil *soil,
+                       const pihm_run_config *cfg, double *y_inout, pihm_run_stats *stats);
 /* HIP-event time of iters back-to-back steps (kernel only), ms per step. */
 int  pihm_noah_soil_time(pihm_noah_soil *s, double dt, const double *y_dev, int iters, double *ms);
 

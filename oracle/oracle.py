@@ -66,6 +66,9 @@ def lib():
         ("oracle_backend_get", C.POINTER(P.Backend), [C.c_void_p]),
         ("oracle_backend_free", None, [C.c_void_p]),
         ("oracle_backend_enable_precond", None, [C.c_void_p, C.c_long]),
+        ("oracle_backend_set_noah_soil", None, [C.c_void_p, C.POINTER(C.c_int), dp, dp, dp, dp, dp]),
+        ("oracle_backend_noah_soil_state", None, [C.c_void_p, dp, dp]),
+        ("oracle_backend_start", C.c_int, [C.c_void_p, C.c_void_p]),
         ("oracle_backend_bj", C.c_void_p, [C.c_void_p]),
         ("oracle_bj_new", C.c_void_p, [C.POINTER(P.MeshDesc), C.c_void_p, C.c_long]),
         ("oracle_bj_free", None, [C.c_void_p]),
@@ -215,7 +218,7 @@ def init_geometry(mesh):
 class OracleRun:
     """The model-step driver (product integrator) over the oracle RHS on host vectors."""
 
-    def __init__(self, mesh, forcing, nthreads=1, precond=False):
+    def __init__(self, mesh, forcing, nthreads=1, precond=False, noah=None):
         self.L = lib()
         self.PL = P.lib()
         self.mesh = mesh
@@ -227,6 +230,26 @@ class OracleRun:
         self.nthreads = nthreads
         if precond:
             self.L.oracle_backend_enable_precond(self.be, mesh.neq)
+        self.noah = noah
+        if noah is not None:
+            # the _NOAH_ RHS branches (11-layer et) and the soil-water step at every Summary
+            f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+            ci = lambda a: np.ascontiguousarray(a, dtype=np.intc)
+            self._nk = [f(noah["smcmin"]), f(noah["sh2o"][:, 0]), f(np.ones(mesh.nelem)), f(noah["et"]),
+                        ci(noah["nwtbl"]), ci(noah["nroot"]), ci(noah["nsoil"]), f(noah["sldpth"]),
+                        f(noah["smcmax"]), f(noah["frzx"]), f(noah["smc"]), f(noah["sh2o"])]
+            k = self._nk
+            assert k[3].shape == (mesh.nelem, 11)
+            ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+            self.L.oracle_set_noah(self.st, _dptr(k[0]), _dptr(k[1]), _dptr(k[2]), _dptr(k[3]), 11, ip(k[4]), ip(k[5]))
+            self.L.oracle_backend_set_noah_soil(self.be, ip(k[6]), _dptr(k[7]), _dptr(k[8]), _dptr(k[9]),
+                                                _dptr(k[10]), _dptr(k[11]))
+
+    def noah_soil_state(self):
+        ne = self.mesh.nelem
+        smc, sh2o = np.zeros((ne, 11)), np.zeros((ne, 11))
+        self.L.oracle_backend_noah_soil_state(self.be, _dptr(smc), _dptr(sh2o))
+        return smc, sh2o
 
     def run(self, y0, cfg):
         n = self.mesh.neq
@@ -237,6 +260,8 @@ class OracleRun:
             # initial ws0 = initial surface state
             ws0 = self._ws0()
             ws0[:] = y0[:self.mesh.nelem]
+            if self.noah is not None:
+                self.L.oracle_backend_start(self.be, v)
             st = P.RunStats()
             rc = self.PL.pihm_drive(self.L.oracle_backend_get(self.be), v, C.byref(cfg), C.byref(st))
             return rc, data.copy(), st

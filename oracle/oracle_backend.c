@@ -25,10 +25,19 @@ int oracle_bj_psetup(realtype t, N_Vector y, N_Vector fy, int jok, int *jcur, re
 int oracle_bj_psolve(realtype t, N_Vector y, N_Vector fy, N_Vector r, N_Vector z, realtype gamma, realtype delta,
                      int lr, void *ud, N_Vector tmp);
 
+typedef struct oracle_noah_soil_host {
+    int *nsoil, *nroot;
+    double *sldpth, *smcmax, *frzx, *et;              /* per-layer arrays [i*11 + k] */
+    double *smc, *sh2o, *smflxv, *runoff2_lyr, *runoff3;
+    double *prev_u, *prev_g, *ydot, *ovl_save;
+    int started;
+} oracle_noah_soil_host;
+
 typedef struct oracle_backend {
     const pihm_mesh_desc *m;
     oracle_state *s;
     oracle_bj *bj;
+    oracle_noah_soil_host *ns;
     int nstation, nrec;
     double rec_dt;
     double *prcp, *edir, *ett;
@@ -68,12 +77,113 @@ static int oracle_forcing(void *user, double t)
     return 0;
 }
 
+/* Summary with _NOAH_ (src/update.c:3-60) and NoahHydrol (src/pihm.c:106-109):
+ * the fluxes at y (the carried river-edge flux kept), MassBalance with the
+ * _NOAH_ EtExtract split, the soil column, and what the next RHS reads. */
+static int oracle_noah_summary(oracle_backend *b, const double *y)
+{
+    oracle_noah_soil_host *n = b->ns;
+    oracle_state *s = b->s;
+    const pihm_mesh_desc *m = b->m;
+    const int ne = m->nelem;
+    int i;
+    if (!n->started) {
+        for (i = 0; i < ne; i++) { n->prev_u[i] = y[ne + i]; n->prev_g[i] = y[2 * ne + i]; }
+        n->started = 1;
+        return 0;
+    }
+    memcpy(n->ovl_save, s->ovlflow, 3 * (size_t)ne * sizeof(double));
+    oracle_ode(m, s, 0.0, y, n->ydot, NULL);
+    memcpy(s->ovlflow, n->ovl_save, 3 * (size_t)ne * sizeof(double));
+    for (i = 0; i < ne; i++) {
+        const size_t o = (size_t)i * 11;
+        double soilw0 = n->prev_g[i] + n->prev_u[i], soilw1 = y[2 * ne + i] + y[ne + i], subrunoff = 0.0, infil;
+        int j;
+        oracle_noah_col in;
+        oracle_noah_state st;
+        soilw0 = (soilw0 > m->depth[i]) ? m->depth[i] : soilw0;
+        soilw0 = (soilw0 < 0.0) ? 0.0 : soilw0;
+        soilw1 = (soilw1 > m->depth[i]) ? m->depth[i] : soilw1;
+        soilw1 = (soilw1 < 0.0) ? 0.0 : soilw1;
+        for (j = 0; j < 3; j++) subrunoff += s->subsurf[(size_t)i * 3 + j] / m->area[i];
+        infil = (soilw1 - soilw0) * m->porosity[i] / m->stepsize + subrunoff + s->edir_unsat[i] + s->edir_gw[i] +
+            s->ett_unsat[i] + s->ett_gw[i];
+        if (infil < 0.0) { subrunoff -= infil; infil = 0.0; }
+        in.nsoil = n->nsoil[i];
+        in.sldpth = n->sldpth + o;
+        in.smcmax = n->smcmax[i]; in.smcmin = s->smcmin[i]; in.ksatv = m->ksatv[i];
+        in.alpha = m->alpha[i]; in.beta = m->beta[i]; in.frzx = n->frzx[i];
+        in.gw = y[2 * ne + i]; in.infil = infil; in.runoff2 = subrunoff; in.edir = s->edir[i];
+        in.et = n->et + o;
+        memset(&st, 0, sizeof(st));
+        memcpy(st.smc, n->smc + o, sizeof(st.smc));
+        memcpy(st.sh2o, n->sh2o + o, sizeof(st.sh2o));
+        memcpy(st.smflxv, n->smflxv + o, sizeof(st.smflxv));
+        memcpy(st.runoff2_lyr, n->runoff2_lyr + o, sizeof(st.runoff2_lyr));
+        st.fcr = s->fcr[i];
+        st.runoff3 = n->runoff3[i];
+        oracle_noah_soil_column(&in, &st, m->stepsize);
+        memcpy(n->smc + o, st.smc, sizeof(st.smc));
+        memcpy(n->sh2o + o, st.sh2o, sizeof(st.sh2o));
+        memcpy(n->smflxv + o, st.smflxv, sizeof(st.smflxv));
+        memcpy(n->runoff2_lyr + o, st.runoff2_lyr, sizeof(st.runoff2_lyr));
+        n->runoff3[i] = st.runoff3;
+        s->sh2o0[i] = st.sh2o[0];
+        s->fcr[i] = st.fcr;
+        s->nwtbl[i] = st.nwtbl;
+        n->prev_u[i] = y[ne + i];
+        n->prev_g[i] = y[2 * ne + i];
+    }
+    return 0;
+}
+
 static int oracle_summary(void *user, N_Vector y)
 {
     oracle_backend *b = (oracle_backend *)user;
+    if (b->ns) {
+        const int rc = oracle_noah_summary(b, y->ops->nvgetarraypointer(y));
+        if (rc) return rc;
+    }
     memcpy(b->s->ws0_surf, y->ops->nvgetarraypointer(y), b->m->nelem * sizeof(double));
     return 0;
 }
+
+/* The soil-water step at every Summary, over the state's _NOAH_ RHS branches
+ * (oracle_set_noah with 11-layer et); per-layer arrays [i*11 + k]. */
+void oracle_backend_set_noah_soil(oracle_backend *b, const int *nsoil, const double *sldpth, const double *smcmax,
+                                  const double *frzx, const double *smc, const double *sh2o)
+{
+    const size_t ne = (size_t)b->m->nelem, nl = ne * 11;
+    oracle_noah_soil_host *n = (oracle_noah_soil_host *)calloc(1, sizeof(*n));
+    n->nsoil = (int *)malloc(ne * sizeof(int));
+    memcpy(n->nsoil, nsoil, ne * sizeof(int));
+    n->sldpth = (double *)malloc(nl * sizeof(double)); memcpy(n->sldpth, sldpth, nl * sizeof(double));
+    n->smcmax = (double *)malloc(ne * sizeof(double)); memcpy(n->smcmax, smcmax, ne * sizeof(double));
+    n->frzx = (double *)malloc(ne * sizeof(double)); memcpy(n->frzx, frzx, ne * sizeof(double));
+    n->smc = (double *)malloc(nl * sizeof(double)); memcpy(n->smc, smc, nl * sizeof(double));
+    n->sh2o = (double *)malloc(nl * sizeof(double)); memcpy(n->sh2o, sh2o, nl * sizeof(double));
+    n->et = b->s->et;     /* the RHS's per-layer transpiration, [i*11 + k] */
+    n->smflxv = (double *)calloc(nl, sizeof(double));
+    n->runoff2_lyr = (double *)calloc(nl, sizeof(double));
+    n->runoff3 = (double *)calloc(ne, sizeof(double));
+    n->prev_u = (double *)calloc(ne, sizeof(double));
+    n->prev_g = (double *)calloc(ne, sizeof(double));
+    n->ydot = (double *)calloc((size_t)b->m->nelem * 3 + 2 * (size_t)b->m->nriver + 8, sizeof(double));
+    n->ovl_save = (double *)calloc(3 * ne, sizeof(double));
+    b->ns = n;
+}
+
+/* smc and sh2o [i*11 + k] of the soil-water step */
+void oracle_backend_noah_soil_state(const oracle_backend *b, double *smc, double *sh2o)
+{
+    const size_t nl = (size_t)b->m->nelem * 11;
+    if (!b->ns) return;
+    memcpy(smc, b->ns->smc, nl * sizeof(double));
+    memcpy(sh2o, b->ns->sh2o, nl * sizeof(double));
+}
+
+/* the initial Summary (the storage the first balance starts from) */
+int oracle_backend_start(oracle_backend *b, N_Vector y) { return oracle_summary(b, y); }
 
 oracle_backend *oracle_backend_new(const pihm_mesh_desc *m, oracle_state *s,
                                    const pihm_forcing *f, int nthreads)
@@ -127,6 +237,13 @@ const pihm_backend *oracle_backend_get(oracle_backend *b) { return &b->be; }
 
 void oracle_backend_free(oracle_backend *b)
 {
+    if (b && b->ns) {
+        oracle_noah_soil_host *n = b->ns;
+        free(n->nsoil); free(n->sldpth); free(n->smcmax); free(n->frzx); free(n->smc); free(n->sh2o);
+        free(n->smflxv); free(n->runoff2_lyr); free(n->runoff3); free(n->prev_u); free(n->prev_g);
+        free(n->ydot); free(n->ovl_save); free(n);
+        b->ns = NULL;
+    }
     if (!b) return;
     oracle_bj_free(b->bj);
     free(b->prcp); free(b->edir); free(b->ett); free(b->station);

@@ -161,6 +161,9 @@ int pihm_ctx_carried_copy(pihm_ctx *c, double *buf, int dir);
 // One RHS evaluation of a single-solve context at y (device) with the fluxes
 // recorded in args.rec / args.r.rf; the carried river-edge flux is restored.
 int pihm_ctx_eval_fluxes(pihm_ctx *c, const double *y, double *ydot, double *carried_save);
+// With the _NOAH_ branches on: the saturated zone's share of transpiration
+// again from the current ett, et, nwtbl and nroot (after any of them changed).
+int pihm_ctx_noah_refresh(pihm_ctx *c);
 int pihm_rhs_launch_compact(pihm_ctx *c, const pihm::MemberIO *mio, int *mnan, const int *bm,
                             const int *bm_mem, int nact, int nbe, int nbr);
 

@@ -10,6 +10,7 @@
 #include "../../include/pihm_bc.h"
 #include "../../include/pihm_lsm.h"
 #include "../../include/pihm_output.h"
+#include "../../include/pihm_noahsoil.h"
 #include "common.hpp"
 #include "ctx.hpp"
 #include "nvector_impl.hpp"
@@ -139,6 +140,8 @@ struct pihm_gpu_backend {
   double rec_dt;
   int32_t *station;
   double *prcp, *edir, *ett;
+  pihm_noah_soil *noah_soil;     // the _NOAH_ build's soil-water step at each Summary
+  int summaries;
 };
 
 static int gpu_rhs(realtype t, N_Vector y, N_Vector ydot, void *ud) {
@@ -153,7 +156,8 @@ static int gpu_forcing(void *user, double t) {
   hipLaunchKernelGGL(k_forcing, dim3(div_up(c->ne, kBlock)), dim3(kBlock), 0, c->stream, c->ne,
                      b->station, b->prcp, b->edir, b->ett, b->nrec, b->rec_dt, t,
                      c->args.e.pcpdrp, c->args.e.edir, c->args.e.ett);
-  return hipGetLastError() == hipSuccess ? 0 : PIHM_ERR_HIP;
+  if (hipGetLastError() != hipSuccess) return PIHM_ERR_HIP;
+  return pihm_ctx_noah_refresh(c);
 }
 
 // ws0.surf <- y[SURF] (device order on both sides)
@@ -164,7 +168,23 @@ static int copy_ws0(pihm_ctx *c, N_Vector y) {
 }
 
 static int gpu_summary(void *user, N_Vector y) {
-  return copy_ws0(((pihm_gpu_backend *)user)->ctx, y);
+  pihm_gpu_backend *b = (pihm_gpu_backend *)user;
+  if (b->noah_soil) {
+    // Summary's MassBalance and NoahHydrol (src/pihm.c:95-110); the first call
+    // is the initial state, whose storage starts the balance
+    const double *yd = N_VGetDeviceArrayPointer_Hip(y);
+    const int rc = (b->summaries++ == 0) ? pihm_noah_soil_start(b->noah_soil, yd)
+                                         : pihm_noah_soil_summary(b->noah_soil, b->ctx->dt, yd);
+    if (rc) return rc;
+  }
+  return copy_ws0(b->ctx, y);
+}
+
+extern "C" int pihm_gpu_backend_set_noah_soil(pihm_gpu_backend *b, pihm_noah_soil *s) {
+  if (!b) return PIHM_ERR_ARG;
+  b->noah_soil = s;
+  b->summaries = 0;
+  return PIHM_OK;
 }
 
 extern "C" int pihm_gpu_backend_create(pihm_ctx *c, const pihm_forcing *f, pihm_gpu_backend **out) {
@@ -255,6 +275,20 @@ extern "C" int pihm_run_gpu(pihm_ctx *c, const pihm_forcing *f, const pihm_run_c
   pihm_gpu_backend *b = nullptr;
   int rc = pihm_gpu_backend_create(c, f, &b);
   if (rc) return rc;
+  pihm_backend be;
+  pihm_gpu_backend_get(b, &be);
+  rc = run_device(c, cfg, y_inout, st, &be);
+  pihm_gpu_backend_destroy(b);
+  return rc;
+}
+
+extern "C" int pihm_run_gpu_noah(pihm_ctx *c, const pihm_forcing *f, pihm_noah_soil *soil,
+                                 const pihm_run_config *cfg, double *y_inout, pihm_run_stats *st) {
+  if (!c || !f || !soil || !cfg || !y_inout || !c->noah) return PIHM_ERR_ARG;
+  pihm_gpu_backend *b = nullptr;
+  int rc = pihm_gpu_backend_create(c, f, &b);
+  if (rc) return rc;
+  pihm_gpu_backend_set_noah_soil(b, soil);
   pihm_backend be;
   pihm_gpu_backend_get(b, &be);
   rc = run_device(c, cfg, y_inout, st, &be);

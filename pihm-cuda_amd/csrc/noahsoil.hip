@@ -35,7 +35,50 @@ struct SoilDev {
   double *satdpth;                              // [ML * ne]
   double *runoff3;                              // carried
   double *infil, *runoff2;                      // the step's MassBalance values
+  double *prev;                                 // [2 * ne] the previous Summary's unsat, gw
 };
+
+// MassBalance (src/update.c:107-160) with the _NOAH_ EtExtract split of the
+// fluxes evaluated at y: infiltration re-derived from the storage change since
+// the previous Summary, and the subsurface runoff that CalcLatFlx spreads.
+__global__ __launch_bounds__(kBlock) void k_mass_balance_noah(int ne, SoilDev s, ElemDev e, const double *y,
+                                                              const double *sub, double dt) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const double depth = e.depth[p], area = e.area[p];
+  const double u = y[ne + p], g = y[2L * ne + p];
+  double soilw0 = s.prev[ne + p] + s.prev[p];
+  soilw0 = (soilw0 > depth) ? depth : soilw0;
+  soilw0 = (soilw0 < 0.0) ? 0.0 : soilw0;
+  double soilw1 = g + u;
+  soilw1 = (soilw1 > depth) ? depth : soilw1;
+  soilw1 = (soilw1 < 0.0) ? 0.0 : soilw1;
+  double subrunoff = 0.0;
+  for (int j = 0; j < 3; j++) subrunoff += sub[(long)j * ne + p] / area;
+  // EtExtract with _NOAH_ (src/hydrol.c:43-56,79-84) at the clamped groundwater
+  const double gc = (g > 0.0) ? g : 0.0;
+  const double edir = e.edir[p], ett = e.ett[p], gwet = e.gwet[p];
+  double edir_u, edir_g;
+  if (gc > depth - e.dinf[p]) { edir_u = 0.0; edir_g = edir; }
+  else { edir_u = edir; edir_g = 0.0; }
+  const double ett_u = (1.0 - gwet) * ett, ett_g = gwet * ett;
+  double infil = (soilw1 - soilw0) * e.porosity[p] / dt + subrunoff + edir_u + edir_g + ett_u + ett_g;
+  if (infil < 0.0) {
+    subrunoff -= infil;
+    infil = 0.0;
+  }
+  s.infil[p] = infil;
+  s.runoff2[p] = subrunoff;
+  s.prev[p] = u;
+  s.prev[ne + p] = g;
+}
+
+__global__ __launch_bounds__(kBlock) void k_soil_prev(int ne, SoilDev s, const double *y) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  s.prev[p] = y[ne + p];
+  s.prev[ne + p] = y[2L * ne + p];
+}
 
 // FindWaterTable: src/noah/lsm_func.c:4-55
 __device__ int find_water_table(const double *sld, int nsoil, double gw, double *sat) {
@@ -294,6 +337,8 @@ struct pihm_noah_soil {
   int ne;
   SoilDev d;
   void *arena;
+  double *scratch;     // [neq + 2 nr]: fluxes evaluated at the Summary state, carried flux saved
+  int started;
 };
 
 namespace {
@@ -358,12 +403,15 @@ int pihm_noah_soil_create(pihm_ctx *ctx, const pihm_noah_soil_desc *desc, pihm_n
   const size_t bi = ((size_t)ne * sizeof(int32_t) + 255) & ~size_t(255);
   double **one[] = {&D.smcmax, &D.frzx, &D.runoff3, &D.infil, &D.runoff2};
   double **lay[] = {&D.sldpth, &D.smc, &D.sh2o, &D.smflxv, &D.runoff2_lyr, &D.satdpth};
-  const size_t total = 5 * b1 + 6 * bl + bi;
+  const size_t bs = ((size_t)(ctx->neq + 2L * ctx->nr + 1) * sizeof(double) + 255) & ~size_t(255);
+  const size_t total = 7 * b1 + 6 * bl + bi + bs;
   if (hipMalloc(&s->arena, total) != hipSuccess) { delete s; return PIHM_ERR_HIP; }
   char *q = (char *)s->arena;
   for (auto f : one) { *f = (double *)q; q += b1; }
+  D.prev = (double *)q; q += 2 * b1;
   for (auto f : lay) { *f = (double *)q; q += bl; }
-  D.nsoil = (int32_t *)q;
+  D.nsoil = (int32_t *)q; q += bi;
+  s->scratch = (double *)q;
   int rc = PIHM_OK;
   if (hipMemsetAsync(s->arena, 0, total, ctx->stream) != hipSuccess) rc = PIHM_ERR_HIP;
   std::vector<int32_t> hn(ne);
@@ -420,6 +468,25 @@ int pihm_noah_soil_step(pihm_noah_soil *s, double dt, const double *y_dev, const
   int rc;
   if ((rc = upload(s, s->d.infil, infil, 1))) return rc;
   if ((rc = upload(s, s->d.runoff2, runoff2, 1))) return rc;
+  return launch(s, dt, y_dev);
+}
+
+int pihm_noah_soil_start(pihm_noah_soil *s, const double *y_dev) {
+  if (!s || !y_dev) return PIHM_ERR_ARG;
+  hipLaunchKernelGGL(k_soil_prev, dim3(div_up(s->ne, kBlock)), dim3(kBlock), 0, s->ctx->stream, s->ne, s->d, y_dev);
+  PIHM_HIP_TRY(hipGetLastError());
+  s->started = 1;
+  return PIHM_OK;
+}
+
+int pihm_noah_soil_summary(pihm_noah_soil *s, double dt, const double *y_dev) {
+  if (!s || !y_dev || !(dt > 0.0) || !s->ctx->noah || !s->started) return PIHM_ERR_ARG;
+  pihm_ctx *c = s->ctx;
+  int rc = pihm_ctx_eval_fluxes(c, y_dev, s->scratch, s->scratch + c->neq);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mass_balance_noah, dim3(div_up(s->ne, kBlock)), dim3(kBlock), 0, c->stream, s->ne, s->d,
+                     c->args.e, y_dev, (const double *)c->args.rec.sub, dt);
+  PIHM_HIP_TRY(hipGetLastError());
   return launch(s, dt, y_dev);
 }
 

@@ -1281,6 +1281,35 @@ extern "C" int pihm_ctx_get_perm(const pihm_ctx *c, int32_t *out) {
   return PIHM_OK;
 }
 
+// GwTransp (src/noah/lsm_func.c:163-185) per element: the share of
+// transpiration drawn from the saturated zone, from the layers between the
+// water table layer nwtbl and the root layer nroot, summed in ascending order.
+__global__ __launch_bounds__(kBlock) void k_gwtransp(ElemDev e, int ne) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ne) return;
+  const double ett = e.ett[p];
+  const int nwtbl = e.nwtbl[p], nroot = e.nroot[p];
+  double gw = 0.0;
+  if (ett > 0.0 && nwtbl <= nroot) {
+    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) gw += e.et[(long)j * ne + p];
+    gw = gw / ett;
+    gw = (gw > 1.0) ? 1.0 : gw;
+    gw = (gw < 0.0) ? 0.0 : gw;
+  }
+  e.gwet[p] = gw;
+}
+
+// The saturated zone's share of transpiration follows its inputs: ett (step
+// inputs, device forcing), et and nroot (Noah hand-over), nwtbl (hand-over,
+// soil-water step).  The reference evaluates it in every RHS call from the
+// current values; evaluating it whenever one of them changes gives the same.
+int pihm_ctx_noah_refresh(pihm_ctx *c) {
+  if (!c->noah) return PIHM_OK;
+  hipLaunchKernelGGL(k_gwtransp, dim3(div_up(c->ne, kBlock)), dim3(kBlock), 0, c->stream, c->args.e, c->ne);
+  PIHM_HIP_TRY(hipGetLastError());
+  return PIHM_OK;
+}
+
 extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in) {
   if (!c || !in) return PIHM_ERR_ARG;
   const int ne = c->ne, nr = c->nr;
@@ -1301,6 +1330,7 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
   if (in->pcpdrp && (rc = up1(E.pcpdrp, in->pcpdrp))) return rc;
   if (in->edir && (rc = up1(E.edir, in->edir))) return rc;
   if (in->ett && (rc = up1(E.ett, in->ett))) return rc;
+  if (in->ett && (rc = pihm_ctx_noah_refresh(c))) return rc;
   if (in->ws0_surf && (rc = up1(E.ws0s, in->ws0_surf))) return rc;
   if (in->elem_bc) {
     for (int j = 0; j < 3; j++)
@@ -1319,24 +1349,6 @@ extern "C" int pihm_ctx_set_step_inputs(pihm_ctx *c, const pihm_step_inputs *in)
     PIHM_HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return PIHM_OK;
-}
-
-// GwTransp (src/noah/lsm_func.c:163-185) per element: the share of
-// transpiration drawn from the saturated zone, from the layers between the
-// water table layer nwtbl and the root layer nroot, summed in ascending order.
-__global__ __launch_bounds__(kBlock) void k_gwtransp(ElemDev e, int ne) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= ne) return;
-  const double ett = e.ett[p];
-  const int nwtbl = e.nwtbl[p], nroot = e.nroot[p];
-  double gw = 0.0;
-  if (ett > 0.0 && nwtbl <= nroot) {
-    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) gw += e.et[(long)j * ne + p];
-    gw = gw / ett;
-    gw = (gw > 1.0) ? 1.0 : gw;
-    gw = (gw < 0.0) ? 0.0 : gw;
-  }
-  e.gwet[p] = gw;
 }
 
 extern "C" int pihm_ctx_set_noah(pihm_ctx *c, const double *smcmin) {

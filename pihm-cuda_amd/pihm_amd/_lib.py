@@ -173,6 +173,10 @@ _PROTOS = [
                                            c_double_p, c_double_p, c_double_p]),
     ("pihm_noah_soil_step", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, c_double_p, c_double_p]),
     ("pihm_noah_soil_time", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    ("pihm_noah_soil_start", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("pihm_noah_soil_summary", C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    ("pihm_gpu_backend_set_noah_soil", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("pihm_run_gpu_noah", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, C.c_void_p]),
     ("pihm_ctx_set_noah_step", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, C.c_int, c_int32_p, c_int32_p]),
     ("pihm_ctx_neq", C.c_int, [C.c_void_p]),
     ("pihm_ctx_get_perm", C.c_int, [C.c_void_p, c_int32_p]),

@@ -94,3 +94,28 @@ class HipNoahSoil:
         if self.L.pihm_noah_soil_time(self.h, float(dt), self._ydev(None), int(iters), C.byref(ms)):
             raise RuntimeError("pihm_noah_soil_time failed")
         return ms.value
+
+
+def run_gpu_noah(mesh, forcing, cfg, y0, noah):
+    """Integrate on the device with the _NOAH_ RHS branches and the soil-water step at every
+    Summary (pihm_run_gpu_noah).  ``noah`` holds smcmin, et [nelem, 11], nwtbl, nroot (the
+    initial hand-over) and nsoil, sldpth, smcmax, frzx, smc, sh2o (the soil columns).
+    Returns (rc, y, stats, smc, sh2o)."""
+    from .rhs import HipRHS
+    h = HipRHS(mesh, nan_check=False)
+    try:
+        h.set_noah(noah["smcmin"])
+        h.set_noah_step(et=noah["et"], nwtbl=noah["nwtbl"], nroot=noah["nroot"])
+        soil = HipNoahSoil(h, noah["nsoil"], noah["sldpth"], noah["smcmax"], noah["frzx"])
+        try:
+            soil.set_state(noah["smc"], noah["sh2o"])
+            y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+            st = _lib.RunStats()
+            fs = forcing.struct()
+            rc = h.L.pihm_run_gpu_noah(h.ctx, C.byref(fs), soil.h, C.byref(cfg), _dp(y), C.byref(st))
+            s = soil.get_state()
+            return rc, y, st, s["smc"], s["sh2o"]
+        finally:
+            soil.close()
+    finally:
+        h.close()

@@ -231,3 +231,48 @@ def test_interlayer_flux_is_darcy_richards():
         wdf, wcnd = _wdfcnd(0.35, 0.0, 0.45, 0.05, soil["ksatv"][i], soil["alpha"][i], soil["beta"][i])
         want = wcnd + wdf * (0.35 - 0.25) / (0.5 * (sld[i, 0] + sld[i, 1]))
         assert st["smflxv"][i, 0] == pytest.approx(want, rel=1e-6)
+
+
+def noah_run_setup(m, rng, y0):
+    """Soil columns and the initial Noah hand-over for a coupled run: 4-11 layers, moist
+    profiles with some ice, transpiration from the root layers."""
+    ne = m.nelem
+    por = m.a["porosity"]
+    smcmin = rng.uniform(0.01, 0.06, ne)
+    nsoil = rng.integers(4, M + 1, ne).astype(np.int32)
+    sld = np.zeros((ne, M))
+    for i in range(ne):
+        sld[i, :nsoil[i]] = m.a["depth"][i] / nsoil[i]
+    sh2o = np.zeros((ne, M))
+    smc = np.zeros((ne, M))
+    for i in range(ne):
+        n = nsoil[i]
+        sh2o[i, :n] = smcmin[i] + por[i] * rng.uniform(0.3, 0.8, n)
+        smc[i, :n] = sh2o[i, :n] + np.where(rng.random(n) < 0.1, 0.02, 0.0)
+    nroot = np.minimum(3, nsoil).astype(np.int32)
+    et = np.zeros((ne, M))
+    et[:, :3] = 1.0 / 3.0 * 2e-8
+    nwtbl = np.array([_fwt(sld[i, :nsoil[i]], y0[2 * ne + i])[0] for i in range(ne)], dtype=np.int32)
+    return dict(smcmin=smcmin, et=et, nwtbl=nwtbl, nroot=nroot, nsoil=nsoil, sldpth=sld,
+                smcmax=smcmin + por, frzx=np.full(ne, 0.15), smc=smc, sh2o=sh2o)
+
+
+def test_coupled_host_run_keeps_columns_physical(c0):
+    """The host run with the _NOAH_ RHS and the soil-water step at every Summary: the run
+    completes, and every layer's moisture stays between its bounds."""
+    from pihm_amd.forcing import StormForcing, run_config
+    m = c0
+    rng = np.random.default_rng(9)
+    y0 = m.initial_state()
+    nz = noah_run_setup(m, rng, y0)
+    f = StormForcing(8, seed=5, nelem=m.nelem, p_wet=0.5)
+    run = O.OracleRun(m, f, noah=nz)
+    rc, y, st = run.run(y0, run_config(2 * 3600.0))
+    assert rc == 0 and np.all(np.isfinite(y))
+    smc, sh2o = run.noah_soil_state()
+    for i in range(m.nelem):
+        n = nz["nsoil"][i]
+        assert np.all(smc[i, :n] >= nz["smcmin"][i] + 0.02 - 1e-12)
+        assert np.all(smc[i, :n] <= nz["smcmax"][i] + 1e-12)
+        assert np.all(sh2o[i, :n] >= 0.0) and np.all(sh2o[i, :n] <= smc[i, :n] + 1e-12)
+    assert not np.array_equal(sh2o, nz["sh2o"])
