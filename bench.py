@@ -322,6 +322,10 @@ def producers_run(mesh, seed, iters=200):
 
 
 NOAH_EXTRA_BYTES_PER_ELEM = 32   # smcmin, sh2o[0], fcr, gwet read by the _NOAH_ branches
+# HBM bytes per launch at the 100,000-triangle mesh from the FETCH_SIZE and WRITE_SIZE passes
+# (FETCH_SIZE x 2, calibrated on the copy kernel): one Noah RHS, one soil-water step
+NOAH_RHS_TRAFFIC_BYTES = 56153190.0
+NOAH_SOIL_TRAFFIC_BYTES = 224412774.0
 
 
 def noah_rhs_run(seed, iters=200):
@@ -372,10 +376,12 @@ def noah_rhs_run(seed, iters=200):
             "ms_per_eval": ms["noah"], "plain_ms_per_eval": ms["plain"],
             "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_prep + k_flux<noah>)",
                          "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah},
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah,
+                         "traffic": NOAH_RHS_TRAFFIC_BYTES},
             "soil_step": {"kernel": "k_noah_soil (Summary's water table and lateral split, NoahHydrol)",
                           "ms_per_step": soil_ms, "layers": MAXLYR, "algorithmic_bytes": soil_bytes,
                           "achieved_GBps": soil_bytes / (soil_ms * 1e-3) / 1e9,
+                          "traffic": NOAH_SOIL_TRAFFIC_BYTES,
                           "note": "once per model step; one thread per element column"},
             "note": "the per-LSM-step values (edir, ett, per-layer et) come from a host Noah step; "
                     "the Noah energy balance (SFlx) is not on the device"}
