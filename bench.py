@@ -325,7 +325,11 @@ NOAH_EXTRA_BYTES_PER_ELEM = 32   # smcmin, sh2o[0], fcr, gwet read by the _NOAH_
 # HBM bytes per launch at the 100,000-triangle mesh from the FETCH_SIZE and WRITE_SIZE passes
 # (FETCH_SIZE x 2, calibrated on the copy kernel): one Noah RHS, one soil-water step
 NOAH_RHS_TRAFFIC_BYTES = 56153190.0
-NOAH_SOIL_TRAFFIC_BYTES = 224412774.0
+NOAH_SOIL_TRAFFIC_BYTES = 122959155.0
+# algorithmic bytes of the soil-water step per element: reads 4 layer fields (sldpth, smc, sh2o, et;
+# 352 B), 12 FP64 scalars and 2 int32 (104 B); writes 5 layer fields (satdpth, runoff2_lyr, smflxv,
+# smc, sh2o; 440 B), 4 FP64 scalars and nwtbl (36 B)
+NOAH_SOIL_BYTES_PER_ELEM = 932
 
 
 def noah_rhs_run(seed, iters=200):
@@ -366,7 +370,7 @@ def noah_rhs_run(seed, iters=200):
     soil.step(60.0, rng.uniform(0, 2e-6, ne), rng.uniform(0, 1e-7, ne))
     soil.time(60.0, 5)
     soil_ms = soil.time(60.0, iters)
-    soil_bytes = ne * (8 * MAXLYR * 6 + 8 * 12)
+    soil_bytes = ne * NOAH_SOIL_BYTES_PER_ELEM
     soil.close()
     h.close()
     bytes_plain = BYTES_PER_ELEM * ne + BYTES_PER_SEG * m.nriver

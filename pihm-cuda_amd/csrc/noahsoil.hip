@@ -80,32 +80,6 @@ __global__ __launch_bounds__(kBlock) void k_soil_prev(int ne, SoilDev s, const d
   s.prev[ne + p] = y[2L * ne + p];
 }
 
-// FindWaterTable: src/noah/lsm_func.c:4-55
-__device__ int find_water_table(const double *sld, int nsoil, double gw, double *sat) {
-  int layer = -999;
-  double dsum = 0.0, depth = 0.0;
-  for (int j = 0; j < ML; j++) sat[j] = 0.0;
-  for (int j = 0; j < nsoil; j++) depth += sld[j];
-  if (gw <= 0.0) {
-    layer = nsoil;
-    sat[nsoil - 1] = 1.0E-3;
-  } else if (gw > depth) {
-    layer = 0;
-    for (int j = 0; j < nsoil; j++) sat[j] = sld[j];
-  } else {
-    for (int j = nsoil - 1; j >= 0; j--) {
-      if (dsum + sld[j] > gw) {
-        sat[j] = gw - dsum;
-        layer = j + 1;
-        break;
-      }
-      sat[j] = sld[j];
-      dsum += sld[j];
-    }
-  }
-  return layer;
-}
-
 // WDfCnd: src/noah/noah.c:2813-2861
 __device__ void wdf_cnd(double *wdf, double *wcnd, double smc, double sicemax, double smcmax, double smcmin,
                         double ksatv, double alpha, double beta) {
@@ -129,6 +103,11 @@ __device__ void wdf_cnd(double *wdf, double *wcnd, double smc, double sicemax, d
   }
 }
 
+// Every loop over the layers runs over the fixed 11 with a test for the
+// column's own nsoil, so each array index is a constant after unrolling and
+// the column stays in registers.  The water table, the lateral split and the
+// layer fluxes go straight to their HBM fields, which the thread reads back
+// where the reference reads them again.
 __global__ __launch_bounds__(kBlock) void k_noah_soil(int ne, SoilDev s, ElemDev e, const double *y, double dt) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= ne) return;
@@ -136,52 +115,101 @@ __global__ __launch_bounds__(kBlock) void k_noah_soil(int ne, SoilDev s, ElemDev
   const double smcmax = s.smcmax[p], smcmin = e.smcmin[p];
   const double ksatv = e.ksatv[p], alpha = e.alpha[p], beta = e.beta[p];
   const double gw = y[2L * ne + p];
-  double sld[ML], zsoil[ML], smc[ML], sh2o[ML], sat[ML], r2l[ML], smflxv[ML], et[ML], sice[ML];
-  for (int k = 0; k < ML; k++) {
-    sld[k] = s.sldpth[(long)k * ne + p];
-    r2l[k] = s.runoff2_lyr[(long)k * ne + p];
-    smflxv[k] = s.smflxv[(long)k * ne + p];
-    smc[k] = s.smc[(long)k * ne + p];
-    sh2o[k] = s.sh2o[(long)k * ne + p];
-    et[k] = e.et[(long)k * ne + p];
-  }
+  auto at = [ne, p](double *f, int k) -> double & { return f[(long)k * ne + p]; };
+  double zsoil[ML], smc[ML], sh2o[ML], sice[ML], ai[ML], bi[ML], ci[ML], rhstt[ML];
   // layer depths below the surface (src/noah/lsm_func.c:150-160)
-  zsoil[0] = -sld[0];
-  for (int k = 1; k < nsoil; k++) zsoil[k] = -sld[k] + zsoil[k - 1];
-  double fcr = e.fcr[p], runoff3 = s.runoff3[p];
+#pragma unroll
+  for (int k = 0; k < ML; k++) {
+    zsoil[k] = 0.0;
+    if (k < nsoil) zsoil[k] = (k == 0) ? -at(s.sldpth, 0) : -at(s.sldpth, k) + zsoil[k - 1];
+    smc[k] = at(s.smc, k);
+    sh2o[k] = at(s.sh2o, k);
+  }
+  double fcr = e.fcr[p];
 
-  // Summary: src/update.c:37-43 (runoff2 is MassBalance's subsurface runoff)
-  int nwtbl = find_water_table(sld, nsoil, gw, sat);
+  // FindWaterTable (src/noah/lsm_func.c:4-55); Summary and NoahHydrol call it
+  // with the same arguments (src/update.c:40, src/noah/noah.c:72), so once
+  int nwtbl = -999;
   {
-    double sattot = 0.0;   // CalcLatFlx: src/noah/lsm_func.c:226-255
-    for (int k = 0; k < nsoil; k++) sattot += sat[k];
+    double depth = 0.0;
+#pragma unroll
+    for (int k = 0; k < ML; k++)
+      if (k < nsoil) depth += at(s.sldpth, k);
+    if (gw <= 0.0) {
+      nwtbl = nsoil;
+#pragma unroll
+      for (int k = 0; k < ML; k++) at(s.satdpth, k) = (k == nsoil - 1) ? 1.0E-3 : 0.0;
+    } else if (gw > depth) {
+      nwtbl = 0;
+#pragma unroll
+      for (int k = 0; k < ML; k++) at(s.satdpth, k) = (k < nsoil) ? at(s.sldpth, k) : 0.0;
+    } else {
+      double dsum = 0.0;
+      bool found = false;
+#pragma unroll
+      for (int k = ML - 1; k >= 0; k--) {
+        double v = 0.0;
+        if (k < nsoil && !found) {
+          const double sl = at(s.sldpth, k);
+          if (dsum + sl > gw) {
+            v = gw - dsum;
+            nwtbl = k + 1;
+            found = true;
+          } else {
+            v = sl;
+            dsum += sl;
+          }
+        }
+        at(s.satdpth, k) = v;
+      }
+    }
+  }
+  // CalcLatFlx: src/noah/lsm_func.c:226-255 (runoff2 is MassBalance's subsurface runoff)
+  {
+    double sattot = 0.0;
+#pragma unroll
+    for (int k = 0; k < ML; k++)
+      if (k < nsoil) sattot += at(s.satdpth, k);
     const double runoff2 = s.runoff2[p];
-    if (sattot <= 0.0) r2l[nsoil - 1] = runoff2;
-    else for (int k = 0; k < nsoil; k++) r2l[k] = sat[k] / sattot * runoff2;
+#pragma unroll
+    for (int k = 0; k < ML; k++) {
+      if (sattot <= 0.0) {
+        if (k == nsoil - 1) at(s.runoff2_lyr, k) = runoff2;
+      } else if (k < nsoil) {
+        at(s.runoff2_lyr, k) = at(s.satdpth, k) / sattot * runoff2;
+      }
+    }
   }
-  // NoahHydrol: src/noah/noah.c:68-83
-  nwtbl = find_water_table(sld, nsoil, gw, sat);
-  for (int k = 0; k < nsoil; k++) {
-    smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : smcmin + SH2OMIN;
-    smc[k] = (smc[k] < smcmax) ? smc[k] : smcmax;
-    sh2o[k] = (sh2o[k] < smc[k]) ? sh2o[k] : smc[k];
+  // NoahHydrol's clamps: src/noah/noah.c:74-83
+#pragma unroll
+  for (int k = 0; k < ML; k++) {
+    if (k < nsoil) {
+      smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : smcmin + SH2OMIN;
+      smc[k] = (smc[k] < smcmax) ? smc[k] : smcmax;
+      sh2o[k] = (sh2o[k] < smc[k]) ? sh2o[k] : smc[k];
+    }
+    sice[k] = smc[k] - sh2o[k];   // SmFlx: src/noah/noah.c:1676-1679
   }
-  // SmFlx: src/noah/noah.c:1671-1711
-  for (int k = 0; k < nsoil; k++) sice[k] = smc[k] - sh2o[k];
   if (0 == nwtbl) {
-    for (int k = 0; k < nsoil; k++) {
-      smc[k] = smcmax;
-      sh2o[k] = smc[k] - sice[k];
+#pragma unroll
+    for (int k = 0; k < ML; k++) {
+      if (k < nsoil) {
+        smc[k] = smcmax;
+        sh2o[k] = smc[k] - sice[k];
+      }
     }
   } else {
-    double ai[ML], bi[ML], ci[ML], rhstt[ML];
     // SRT: src/noah/noah.c:2315-2425
     {
       const double CVFRZ = 3.0;
-      double sicemax = 0.0;
-      for (int ks = 0; ks < nsoil; ks++) sicemax = (sice[ks] > sicemax) ? sice[ks] : sicemax;
-      double dice = -zsoil[0] * sice[0];
-      for (int ks = 1; ks < nsoil; ks++) dice += (zsoil[ks - 1] - zsoil[ks]) * sice[ks];
+      double sicemax = 0.0, dice = 0.0;
+#pragma unroll
+      for (int k = 0; k < ML; k++) {
+        if (k < nsoil) {
+          sicemax = (sice[k] > sicemax) ? sice[k] : sicemax;
+          dice = (k == 0) ? -zsoil[0] * sice[0] : dice + (zsoil[k - 1] - zsoil[k]) * sice[k];
+        }
+      }
       fcr = 1.0;
       if (dice > 1.0e-2) {
         const double acrt = CVFRZ * s.frzx[p] / dice;
@@ -195,125 +223,147 @@ __global__ __launch_bounds__(kBlock) void k_noah_soil(int ne, SoilDev s, ElemDev
         fcr = 1.0 - exp(-acrt) * sum;
       }
       const double pddum = s.infil[p];
-      double wdf, wcnd, wdf2, wcnd2, dsmdz2;
+      double wdf, wcnd;
       double dsmdz = (sh2o[0] - sh2o[1]) / (-0.5 * zsoil[1]);
       wdf_cnd(&wdf, &wcnd, sh2o[0], sicemax, smcmax, smcmin, ksatv, alpha, beta);
       double ddz = 1.0 / (-0.5 * zsoil[1]);
       ai[0] = 0.0;
       bi[0] = wdf * ddz / (-zsoil[0]);
       ci[0] = -bi[0];
-      rhstt[0] = (wdf * dsmdz + wcnd - pddum + e.edir[p] + et[0]) / zsoil[0];
-      rhstt[0] += r2l[0] / zsoil[0];
-      double ddz2 = 0.0;
-      for (int k = 1; k < nsoil; k++) {
-        const double denom2 = (zsoil[k - 1] - zsoil[k]);
-        if (k < nsoil - 1) {
-          const double denom = zsoil[k - 1] - zsoil[k + 1];
-          dsmdz2 = (sh2o[k] - sh2o[k + 1]) / (denom * 0.5);
-          wdf_cnd(&wdf2, &wcnd2, sh2o[k], sicemax, smcmax, smcmin, ksatv, alpha, beta);
-          ddz2 = 2.0 / denom;
-          ci[k] = -wdf2 * ddz2 / denom2;
-        } else {
-          wdf2 = 0.0;
-          wcnd2 = 0.0;
-          dsmdz2 = 0.0;
-          ci[k] = 0.0;
-        }
-        double numer = (wdf2 * dsmdz2) + wcnd2 - (wdf * dsmdz) - wcnd + et[k];
-        numer += r2l[k];
-        rhstt[k] = numer / (-denom2);
-        ai[k] = -wdf * ddz / denom2;
-        bi[k] = -(ai[k] + ci[k]);
-        if (k != nsoil - 1) {
-          wdf = wdf2;
-          wcnd = wcnd2;
-          dsmdz = dsmdz2;
-          ddz = ddz2;
+      rhstt[0] = (wdf * dsmdz + wcnd - pddum + e.edir[p] + at(e.et, 0)) / zsoil[0];
+      rhstt[0] += at(s.runoff2_lyr, 0) / zsoil[0];
+#pragma unroll
+      for (int k = 1; k < ML; k++) {
+        ai[k] = bi[k] = ci[k] = rhstt[k] = 0.0;
+        if (k < nsoil) {
+          const double denom2 = (zsoil[k - 1] - zsoil[k]);
+          double wdf2 = 0.0, wcnd2 = 0.0, dsmdz2 = 0.0, ddz2 = 0.0;
+          if (k + 1 < ML && k < nsoil - 1) {
+            const double denom = zsoil[k - 1] - zsoil[k + 1];
+            dsmdz2 = (sh2o[k] - sh2o[k + 1]) / (denom * 0.5);
+            wdf_cnd(&wdf2, &wcnd2, sh2o[k], sicemax, smcmax, smcmin, ksatv, alpha, beta);
+            ddz2 = 2.0 / denom;
+            ci[k] = -wdf2 * ddz2 / denom2;
+          } else {
+            ci[k] = 0.0;
+          }
+          double numer = (wdf2 * dsmdz2) + wcnd2 - (wdf * dsmdz) - wcnd + at(e.et, k);
+          numer += at(s.runoff2_lyr, k);
+          rhstt[k] = numer / (-denom2);
+          ai[k] = -wdf * ddz / denom2;
+          bi[k] = -(ai[k] + ci[k]);
+          if (k != nsoil - 1) {
+            wdf = wdf2;
+            wcnd = wcnd2;
+            dsmdz = dsmdz2;
+            ddz = ddz2;
+          }
         }
       }
     }
     // SStep: src/noah/noah.c:2437-2480
-    double sh2o0[ML];
-    for (int k = 0; k < nsoil; k++) {
-      rhstt[k] *= dt;
-      ai[k] *= dt;
-      bi[k] = 1.0 + bi[k] * dt;
-      ci[k] *= dt;
-      sh2o0[k] = sh2o[k];
-      smflxv[k] = 0.0;
-    }
-    {
-      // Rosr12 (src/noah/noah.c:1572-1620) with p = ci, c = a copy of ci, d = a copy of rhstt, delta = rhstt
-      double cin[ML], din[ML];
-      for (int k = 0; k < nsoil; k++) {
-        din[k] = rhstt[k];
-        cin[k] = ci[k];
-      }
-      cin[nsoil - 1] = 0.0;
-      ci[0] = -cin[0] / bi[0];
-      rhstt[0] = din[0] / bi[0];
-      for (int k = 1; k < nsoil; k++) {
-        ci[k] = -cin[k] * (1.0 / (bi[k] + ai[k] * ci[k - 1]));
-        rhstt[k] = (din[k] - ai[k] * rhstt[k - 1]) * (1.0 / (bi[k] + ai[k] * ci[k - 1]));
-      }
-      ci[nsoil - 1] = rhstt[nsoil - 1];
-      for (int k = 1; k < nsoil; k++) {
-        const int kk = nsoil - k - 1;
-        ci[kk] = ci[kk] * ci[kk + 1] + rhstt[kk];
+#pragma unroll
+    for (int k = 0; k < ML; k++) {
+      if (k < nsoil) {
+        rhstt[k] *= dt;
+        ai[k] *= dt;
+        bi[k] = 1.0 + bi[k] * dt;
+        ci[k] *= dt;
+        at(s.smflxv, k) = 0.0;
       }
     }
-    for (int k = 0; k < nsoil; k++) sh2o[k] += ci[k];
-    // AdjSmProf: src/noah/noah.c:3081-3172
+    // Rosr12 (src/noah/noah.c:1572-1620) in place: ci becomes p, rhstt becomes
+    // delta; each entry is read before it is overwritten, so the arithmetic is
+    // the reference's, which works on copies
+#pragma unroll
+    for (int k = 0; k < ML; k++)
+      if (k == nsoil - 1) ci[k] = 0.0;
+    ci[0] = -ci[0] / bi[0];
+    rhstt[0] = rhstt[0] / bi[0];
+#pragma unroll
+    for (int k = 1; k < ML; k++) {
+      if (k < nsoil) {
+        const double inv = 1.0 / (bi[k] + ai[k] * ci[k - 1]);
+        const double pk = -ci[k] * inv;
+        rhstt[k] = (rhstt[k] - ai[k] * rhstt[k - 1]) * inv;
+        ci[k] = pk;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ML; k++)
+      if (k == nsoil - 1) ci[k] = rhstt[k];
+#pragma unroll
+    for (int k = ML - 2; k >= 0; k--)
+      if (k <= nsoil - 2) ci[k] = ci[k] * ci[k + 1] + rhstt[k];
+    // the solution is added to sh2o; rhstt keeps the old values (sh2o0 of SStep)
+#pragma unroll
+    for (int k = 0; k < ML; k++) {
+      if (k < nsoil) {
+        rhstt[k] = sh2o[k];
+        sh2o[k] += ci[k];
+      }
+    }
+    // AdjSmProf: src/noah/noah.c:3081-3172 (ai holds sh2omid)
     {
-      double sh2omid[ML];
-      double wplus = 0.0, ddz;
-      runoff3 = 0.0;
-      for (int k = nsoil - 1; k >= 0; k--) {
-        ddz = (k != 0) ? zsoil[k - 1] - zsoil[k] : -zsoil[0];
-        sh2omid[k] = sh2o[k] + wplus / ddz;
-        const double stot = sh2omid[k] + sice[k];
-        if (stot > smcmax) {
-          smc[k] = smcmax;
-          wplus = (stot - smcmax) * ddz;
-        } else {
-          double stotmin = (sat[k] * smcmax + (sld[k] - sat[k]) * (smcmin + SH2OMIN)) / sld[k];
-          stotmin = (stotmin > smcmax) ? smcmax : stotmin;
-          stotmin = (stotmin < smcmin + SH2OMIN) ? (smcmin + SH2OMIN) : stotmin;
-          if (stot < stotmin) {
-            smc[k] = stotmin;
-            wplus = (stot - stotmin) * ddz;
+      double wplus = 0.0;
+#pragma unroll
+      for (int k = ML - 1; k >= 0; k--) {
+        if (k < nsoil) {
+          const double ddz = (k != 0) ? zsoil[k - 1] - zsoil[k] : -zsoil[0];
+          ai[k] = sh2o[k] + wplus / ddz;
+          const double stot = ai[k] + sice[k];
+          if (stot > smcmax) {
+            smc[k] = smcmax;
+            wplus = (stot - smcmax) * ddz;
           } else {
-            smc[k] = stot;
-            wplus = 0.0;
+            const double sat = at(s.satdpth, k), sl = at(s.sldpth, k);
+            double stotmin = (sat * smcmax + (sl - sat) * (smcmin + SH2OMIN)) / sl;
+            stotmin = (stotmin > smcmax) ? smcmax : stotmin;
+            stotmin = (stotmin < smcmin + SH2OMIN) ? (smcmin + SH2OMIN) : stotmin;
+            if (stot < stotmin) {
+              smc[k] = stotmin;
+              wplus = (stot - stotmin) * ddz;
+            } else {
+              smc[k] = stot;
+              wplus = 0.0;
+            }
           }
+          ai[k] = smc[k] - sice[k];
         }
-        sh2omid[k] = smc[k] - sice[k];
       }
-      ddz = -zsoil[0];
-      for (int k = 0; k < nsoil; k++) {
-        if (k != 0) ddz = zsoil[k - 1] - zsoil[k];
-        sh2o[k] = sh2omid[k] + wplus / ddz;
-        const double stot = sh2o[k] + sice[k];
-        wplus = (stot > smcmax) ? (stot - smcmax) * ddz : 0.0;
-        smc[k] = (stot < smcmax) ? stot : smcmax;
-        smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : (smcmin + SH2OMIN);
-        sh2o[k] = smc[k] - sice[k];
-        sh2o[k] = (sh2o[k] > 0.0) ? sh2o[k] : 0.0;
+#pragma unroll
+      for (int k = 0; k < ML; k++) {
+        if (k < nsoil) {
+          const double ddz = (k != 0) ? zsoil[k - 1] - zsoil[k] : -zsoil[0];
+          sh2o[k] = ai[k] + wplus / ddz;
+          const double stot = sh2o[k] + sice[k];
+          wplus = (stot > smcmax) ? (stot - smcmax) * ddz : 0.0;
+          smc[k] = (stot < smcmax) ? stot : smcmax;
+          smc[k] = (smc[k] > smcmin + SH2OMIN) ? smc[k] : (smcmin + SH2OMIN);
+          sh2o[k] = smc[k] - sice[k];
+          sh2o[k] = (sh2o[k] > 0.0) ? sh2o[k] : 0.0;
+        }
       }
-      runoff3 = wplus / dt;
+      s.runoff3[p] = wplus / dt;
     }
-    for (int k = nsoil - 1; k > 0; k--)
-      smflxv[k - 1] = (sh2o[k] - sh2o0[k]) * sld[k] / dt + r2l[k] + et[k] + smflxv[k];
+    // layer fluxes, bottom up (smflxv[nsoil - 1] stays 0)
+    double below = 0.0;
+#pragma unroll
+    for (int k = ML - 1; k >= 1; k--) {
+      if (k <= nsoil - 1) {
+        below = (sh2o[k] - rhstt[k]) * at(s.sldpth, k) / dt + at(s.runoff2_lyr, k) + at(e.et, k) + below;
+        at(s.smflxv, k - 1) = below;
+      }
+    }
   }
 
+#pragma unroll
   for (int k = 0; k < ML; k++) {
-    s.smc[(long)k * ne + p] = smc[k];
-    s.sh2o[(long)k * ne + p] = sh2o[k];
-    s.satdpth[(long)k * ne + p] = sat[k];
-    s.runoff2_lyr[(long)k * ne + p] = r2l[k];
-    s.smflxv[(long)k * ne + p] = smflxv[k];
+    if (k < nsoil) {
+      at(s.smc, k) = smc[k];
+      at(s.sh2o, k) = sh2o[k];
+    }
   }
-  s.runoff3[p] = runoff3;
   // what the next model step's RHS reads
   e.sh2o0[p] = sh2o[0];
   e.fcr[p] = fcr;
@@ -322,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_noah_soil(int ne, SoilDev s, ElemDev
   const int nroot = e.nroot[p];
   double g = 0.0;
   if (ett > 0.0 && nwtbl <= nroot) {
-    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) g += et[j];
+    for (int j = (nwtbl <= 0 ? 0 : nwtbl - 1); j < nroot; j++) g += at(e.et, j);
     g = g / ett;
     g = (g > 1.0) ? 1.0 : g;
     g = (g < 0.0) ? 0.0 : g;
