@@ -484,7 +484,7 @@ def main():
         profile_kernels(args.profile_kernels, args.seed)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; runs over several GPUs are not provided",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
   const ElemDev &e = a.e;
   const bool diff = a.surf_mode == DIFF_WAVE;
 
-  // round 1
+  // first group of loads
   const double ys = y[lp], yg = y[2 * ix.ne + lp];
   const double zmax = e.zmax[p], zmin = e.zmin[p], depth = e.depth[p], dmac = e.dmac[p];
   const double kmach = e.kmach[p], areafv = e.areafv[p], ksath = e.ksath[p];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
       ny[j] = e.nabr_y[j * ne + p];
     }
   }
-  // round 2: element neighbour (zmax, surface storage), boundary value,
+  // second group of loads: element neighbour (zmax, surface storage), boundary value,
   // river edge (stage, depth, zbed, zmax).  Each case loads into
   // its own registers, left unset in the other cases (each is read only in
   // its own case): a register shared by two cases, or a default written into
@@ -214,7 +214,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   const double *y = ix.y;
   double *ydot = ix.ydot;
 
-  // Loads first, in two rounds as in k_prep: the element's own entries and
+  // Loads first, in two groups as in k_prep: the element's own entries and
   // edge codes, then the neighbours' records, boundary values and carried
   // river-edge fluxes the codes point at.
   const NbrRec me = e.rec[p];
