@@ -298,12 +298,13 @@ def make_watershed(n_tri, seed=0, lx=None, ly=None, n_seg=20, network="valley",
     zmax = (100.0 + cross_slope * np.abs(pts[:, 1] - yc) + along_slope * pts[:, 0]
             + rng.normal(0.0, noise, len(pts)))
     em = _EdgeMap(pts, simp)
-    reaches = _build_network(em, lx, ly, n_seg, network, rng)
+    reaches = _build_network(em, lx, ly, n_seg, network, rng) if n_seg > 0 else []
     # The zero-depth-gradient outlet takes sqrt of the bed slope between the outlet
     # segment's mid-point and its downstream node (src/river_flow.c:369-375): carve the
     # outlet node 0.2 m below its upstream neighbour so that slope is positive.
-    o_node, o_up = reaches[0]["nodes"][0], reaches[0]["nodes"][1]
-    zmax[o_node] = min(zmax[o_node], zmax[o_up] - 0.2)
+    if reaches:
+        o_node, o_up = reaches[0]["nodes"][0], reaches[0]["nodes"][1]
+        zmax[o_node] = min(zmax[o_node], zmax[o_up] - 0.2)
     zmin = zmax - soil_depth
     riv = _rivers_from_reaches(em, reaches, simp, depth_by_level=(1.0, 0.7, 0.5),
                                coeff_by_level=(3.0, 2.0, 1.5))
