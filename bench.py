@@ -373,6 +373,27 @@ def noah_rhs_run(seed, iters=200):
     soil_bytes = ne * NOAH_SOIL_BYTES_PER_ELEM
     soil.close()
     h.close()
+    # one simulated hour of the coupled run (the _NOAH_ RHS in the integrator, the soil-water step at
+    # every Summary) beside the plain run, same mesh and storm forcing
+    from pihm_amd.noahsoil import run_gpu_noah
+    from pihm_amd.rhs import run_gpu
+    f = StormForcing(4, seed=11, nelem=ne)
+    y0 = m.initial_state()
+    gw0 = y0[2 * ne:3 * ne]
+    depth = m.a["depth"]
+    sld6 = np.zeros((ne, MAXLYR))
+    sld6[:, :6] = depth[:, None] / 6.0
+    nw = np.clip(6 - np.floor(gw0 / (depth / 6.0)).astype(np.int32), 0, 6).astype(np.int32)
+    th6 = smcmin[:, None] + m.a["porosity"][:, None] * np.where(np.arange(MAXLYR) < 6, 0.5, 0.0)[None, :]
+    noah = dict(smcmin=smcmin, et=np.where(np.arange(MAXLYR) < 3, 2e-8 / 3, 0.0)[None, :].repeat(ne, 0),
+                nwtbl=nw, nroot=np.full(ne, 3, np.int32), nsoil=np.full(ne, 6, np.int32), sldpth=sld6,
+                smcmax=smcmin + m.a["porosity"], frzx=np.full(ne, 0.15), smc=th6, sh2o=th6)
+    t0 = time.perf_counter()
+    rc_n, _, st_n, _, _ = run_gpu_noah(m, f, run_config(3600.0), y0, noah)
+    wall_n = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc_p, _, st_p = run_gpu(m, f, run_config(3600.0), y0)
+    wall_p = time.perf_counter() - t0
     bytes_plain = BYTES_PER_ELEM * ne + BYTES_PER_SEG * m.nriver
     bytes_noah = bytes_plain + NOAH_EXTRA_BYTES_PER_ELEM * ne
     ach = bytes_noah / (ms["noah"] * 1e-3) / 1e9
@@ -382,6 +403,9 @@ def noah_rhs_run(seed, iters=200):
                          "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah,
                          "traffic": NOAH_RHS_TRAFFIC_BYTES},
+            "coupled_run": {"rc": rc_n, "sim_days_per_s": (1.0 / 24.0) / wall_n, "wall_s": wall_n, "nst": st_n.nst,
+                            "plain_sim_days_per_s": (1.0 / 24.0) / wall_p, "plain_rc": rc_p, "plain_nst": st_p.nst,
+                            "window": "first simulated hour, 60 model steps, storm forcing, 6 soil layers"},
             "soil_step": {"kernel": "k_noah_soil (Summary's water table and lateral split, NoahHydrol)",
                           "ms_per_step": soil_ms, "layers": MAXLYR, "algorithmic_bytes": soil_bytes,
                           "achieved_GBps": soil_bytes / (soil_ms * 1e-3) / 1e9,
