@@ -545,18 +545,16 @@ int pihm_noah_soil_time(pihm_noah_soil *s, double dt, const double *y_dev, int i
   hipStream_t st = s->ctx->stream;
   hipEvent_t ev[2];
   for (auto &e : ev) PIHM_HIP_TRY(hipEventCreate(&e));
-  PIHM_HIP_TRY(hipEventRecord(ev[0], st));
-  for (int it = 0; it < iters; it++) {
-    const int rc = launch(s, dt, y_dev);
-    if (rc) return rc;
-  }
-  PIHM_HIP_TRY(hipEventRecord(ev[1], st));
-  PIHM_HIP_TRY(hipEventSynchronize(ev[1]));
+  int rc = PIHM_OK;
   float t = 0.0f;
-  PIHM_HIP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+  if (hipEventRecord(ev[0], st) != hipSuccess) rc = PIHM_ERR_HIP;
+  for (int it = 0; !rc && it < iters; it++) rc = launch(s, dt, y_dev);
+  if (!rc && (hipEventRecord(ev[1], st) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+              hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess))
+    rc = PIHM_ERR_HIP;
   for (auto &e : ev) (void)hipEventDestroy(e);
-  *ms = (double)t / iters;
-  return PIHM_OK;
+  if (!rc) *ms = (double)t / iters;
+  return rc;
 }
 
 }  // extern "C"

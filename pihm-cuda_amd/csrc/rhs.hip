@@ -1368,9 +1368,11 @@ extern "C" int pihm_ctx_set_noah(pihm_ctx *c, const double *smcmin) {
     ar.add(&E.nroot, ne);
     PIHM_HIP_TRY(hipMalloc(&c->noah_arena, ar.total()));
     ar.carve((char *)c->noah_arena);
-    // layers never handed over read as no transpiration; no water table or roots yet
-    PIHM_HIP_TRY(hipMemsetAsync(c->noah_arena, 0, ar.total(), c->stream));
   }
+  // no layer transpiration, water table or roots until the first hand-over
+  PIHM_HIP_TRY(hipMemsetAsync(E.et, 0, (size_t)PIHM_NOAH_MAXLYR * ne * sizeof(double), c->stream));
+  PIHM_HIP_TRY(hipMemsetAsync(E.nwtbl, 0, ne * sizeof(int32_t), c->stream));
+  PIHM_HIP_TRY(hipMemsetAsync(E.nroot, 0, ne * sizeof(int32_t), c->stream));
   std::vector<double> hd(ne);
   for (int p = 0; p < ne; p++) {
     const long ci = caller_elem(c, p);
