@@ -332,9 +332,12 @@ NOAH_SOIL_TRAFFIC_BYTES = 122959155.0
 NOAH_SOIL_BYTES_PER_ELEM = 932
 
 
-def noah_rhs_run(seed, iters=200):
+def noah_rhs_run(seed, iters=200, coupled=True):
     """The RHS with the reference's _NOAH_ branches on the 100,000-triangle / 2,047-segment
-    workload, timed with HIP events beside the plain RHS on the same mesh and state."""
+    workload, timed with HIP events beside the plain RHS on the same mesh and state, the
+    soil-water step, and (coupled=True) one simulated hour of the coupled and the plain run.
+    The profile mode leaves the runs out: they replay Krylov iterations from HIP graphs,
+    which has crashed the host process under the kernel tracer."""
     from pihm_amd.rhs import HipRHS
     L = _lib.lib()
     m = config_mesh("c2", seed=seed)
@@ -373,6 +376,24 @@ def noah_rhs_run(seed, iters=200):
     soil_bytes = ne * NOAH_SOIL_BYTES_PER_ELEM
     soil.close()
     h.close()
+    bytes_plain = BYTES_PER_ELEM * ne + BYTES_PER_SEG * m.nriver
+    bytes_noah = bytes_plain + NOAH_EXTRA_BYTES_PER_ELEM * ne
+    ach = bytes_noah / (ms["noah"] * 1e-3) / 1e9
+    out = {"workload": f"c2 mesh: {ne} triangles, {m.nriver} river segments, NEQ {m.neq}",
+           "ms_per_eval": ms["noah"], "plain_ms_per_eval": ms["plain"],
+           "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_prep + k_flux<noah>)",
+                        "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah,
+                        "traffic": NOAH_RHS_TRAFFIC_BYTES},
+           "soil_step": {"kernel": "k_noah_soil (Summary's water table and lateral split, NoahHydrol)",
+                         "ms_per_step": soil_ms, "layers": MAXLYR, "algorithmic_bytes": soil_bytes,
+                         "achieved_GBps": soil_bytes / (soil_ms * 1e-3) / 1e9,
+                         "traffic": NOAH_SOIL_TRAFFIC_BYTES,
+                         "note": "once per model step; one thread per element column"},
+           "note": "the per-LSM-step values (edir, ett, per-layer et) come from a host Noah step; "
+                   "the Noah energy balance (SFlx) is not on the device"}
+    if not coupled:
+        return out
     # one simulated hour of the coupled run (the _NOAH_ RHS in the integrator, the soil-water step at
     # every Summary) beside the plain run, same mesh and storm forcing
     from pihm_amd.noahsoil import run_gpu_noah
@@ -394,25 +415,10 @@ def noah_rhs_run(seed, iters=200):
     t0 = time.perf_counter()
     rc_p, _, st_p = run_gpu(m, f, run_config(3600.0), y0)
     wall_p = time.perf_counter() - t0
-    bytes_plain = BYTES_PER_ELEM * ne + BYTES_PER_SEG * m.nriver
-    bytes_noah = bytes_plain + NOAH_EXTRA_BYTES_PER_ELEM * ne
-    ach = bytes_noah / (ms["noah"] * 1e-3) / 1e9
-    return {"workload": f"c2 mesh: {ne} triangles, {m.nriver} river segments, NEQ {m.neq}",
-            "ms_per_eval": ms["noah"], "plain_ms_per_eval": ms["plain"],
-            "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_prep + k_flux<noah>)",
-                         "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah,
-                         "traffic": NOAH_RHS_TRAFFIC_BYTES},
-            "coupled_run": {"rc": rc_n, "sim_days_per_s": (1.0 / 24.0) / wall_n, "wall_s": wall_n, "nst": st_n.nst,
-                            "plain_sim_days_per_s": (1.0 / 24.0) / wall_p, "plain_rc": rc_p, "plain_nst": st_p.nst,
-                            "window": "first simulated hour, 60 model steps, storm forcing, 6 soil layers"},
-            "soil_step": {"kernel": "k_noah_soil (Summary's water table and lateral split, NoahHydrol)",
-                          "ms_per_step": soil_ms, "layers": MAXLYR, "algorithmic_bytes": soil_bytes,
-                          "achieved_GBps": soil_bytes / (soil_ms * 1e-3) / 1e9,
-                          "traffic": NOAH_SOIL_TRAFFIC_BYTES,
-                          "note": "once per model step; one thread per element column"},
-            "note": "the per-LSM-step values (edir, ett, per-layer et) come from a host Noah step; "
-                    "the Noah energy balance (SFlx) is not on the device"}
+    out["coupled_run"] = {"rc": rc_n, "sim_days_per_s": (1.0 / 24.0) / wall_n, "wall_s": wall_n, "nst": st_n.nst,
+                          "plain_sim_days_per_s": (1.0 / 24.0) / wall_p, "plain_rc": rc_p, "plain_nst": st_p.nst,
+                          "window": "first simulated hour, 60 model steps, storm forcing, 6 soil layers"}
+    return out
 
 
 def cpu_run(mesh, forcing, hours, nthreads):
@@ -456,7 +462,7 @@ def profile_kernels(iters, seed):
     bc.close()
     lsm.close()
     h.close()
-    noah = noah_rhs_run(seed, iters)
+    noah = noah_rhs_run(seed, iters, coupled=False)
     n = 64 << 20
     a = L.N_VNew_Hip(n)
     b = L.N_VNew_Hip(n)
