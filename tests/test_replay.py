@@ -21,6 +21,7 @@ sequence differ only by the rounding of their right-hand sides.  Pins:
 import ctypes as C
 
 import numpy as np
+import pytest
 from scipy.linalg import expm
 
 import oracle as O
@@ -174,3 +175,28 @@ def test_model_run_replay_reproduces_record(c0):
         assert np.all(np.abs(S2 - S1)[live] <= 1e-8 * np.abs(S1)[live])
         o1, o2 = _outlet(m, y1), _outlet(m, y2)
         assert np.all(np.abs(o2 - o1) <= 1e-8 * np.abs(o1))
+
+
+def test_record_and_replay_argument_errors():
+    h = np.zeros(4)
+    q = np.zeros(4, np.int32)
+    n = C.c_long()
+    dp, ip = h.ctypes.data_as(_lib.c_double_p), q.ctypes.data_as(_lib.c_int32_p)
+    assert L.PihmCVodeSetStepRecord(None, dp, ip, None, 4, C.byref(n)) == -21     # PIHM_CV_MEM_NULL
+    assert L.PihmCVodeSetStepReplay(None, dp, ip, None, 4) == -21
+    mem = L.PihmCVodeCreate(2, 2)
+    try:
+        assert L.PihmCVodeSetStepRecord(mem, None, ip, None, 4, C.byref(n)) == -22   # PIHM_CV_ILL_INPUT
+        assert L.PihmCVodeSetStepRecord(mem, dp, ip, None, -1, C.byref(n)) == -22
+        assert L.PihmCVodeSetStepRecord(mem, dp, ip, None, 4, None) == -22
+        assert L.PihmCVodeSetStepRecord(mem, None, None, None, 0, None) == 0          # switched off
+        assert L.PihmCVodeSetStepReplay(mem, dp, None, None, 4) == -22
+        assert L.PihmCVodeSetStepReplay(mem, dp, ip, None, -2) == -22
+        assert L.PihmCVodeSetStepReplay(mem, None, None, None, 0) == 0                # switched off
+    finally:
+        p = C.c_void_p(mem)
+        L.PihmCVodeFree(C.byref(p))
+    with pytest.raises(ValueError):
+        set_replay(run_config(60.0), np.zeros(3), np.zeros(4, np.int32))
+    with pytest.raises(ValueError):
+        set_replay(run_config(60.0), np.zeros(3), np.zeros(3, np.int32), np.zeros((3, 5), np.int32))
