@@ -77,3 +77,27 @@ def test_device_replays_host_sequence_reference_example(example):
                                                  reltol=p["reltol"], abstol=p["abstol"],
                                                  initstep=p["initstep"]),
                   "reference example, 10 min")
+
+
+def test_host_replays_device_sequence(c0):
+    """The other direction: the device run records (the fused SPGMR's Krylov counts included)
+    and the host run replays; storages and outlet discharge within 1e-8 over ten minutes."""
+    m = c0
+    f = StormForcing(24, seed=5, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    rec = StepRecord(500000)
+    print("[replay] device records, host replays", flush=True)
+    rc_g, y_g, st_g = run_gpu(m, f, rec.attach(run_config(600.0)), y0)
+    assert rc_g == 0
+    h, q, it = rec.steps()
+    assert len(h) == st_g.nst and it[:, 0].min() >= 1
+    rec_c = StepRecord(500000)
+    rc_c, y_c, st_c = O.OracleRun(m, f).run(y0, set_replay(rec_c.attach(run_config(600.0)), h, q, it))
+    assert rc_c == 0 and st_c.nst == st_g.nst
+    h2, q2, it2 = rec_c.steps()
+    assert np.array_equal(h2, h) and np.array_equal(q2, q) and np.array_equal(it2, it)
+    S_g, S_c = _stores(m, y_g), _stores(m, y_c)
+    live = np.abs(S_g) > 0
+    assert np.all(np.abs(S_c - S_g)[live] <= 1e-8 * np.abs(S_g)[live])
+    o_g, o_c = _outlet(m, y_g), _outlet(m, y_c)
+    assert np.all(np.abs(o_c - o_g) <= 1e-8 * np.abs(o_g))
