@@ -40,6 +40,8 @@ RHS_TRAFFIC_BYTES = (44752.5 * 2.0 + 17737.5) * 1024
 # resistance's numerator and the carried cmc, sneqv (32) + unsat, gw of the state (16)
 # read; cmc, sneqv, ec, drip, pcpdrp, edir, ett (56) written.
 LSM_BYTES_PER_ELEM = 228
+# HBM bytes of one k_intcp_snow_et launch at 200,000 elements (FETCH_SIZE x 2 + WRITE_SIZE)
+LSM_TRAFFIC_BYTES = 45800755.0
 LSM_STATIONS = 64
 
 
@@ -314,7 +316,8 @@ def producers_run(mesh, seed, iters=200):
         "lsm_step": {"ms": ms_l.value, "elements": ne, "stations": LSM_STATIONS,
                      "roofline": {"kernel": "k_meteo + k_intcp_snow_et (one LSM step)", "bound": "hbm",
                                   "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": lsm_bytes}},
+                                  "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": lsm_bytes,
+                                  "traffic": LSM_TRAFFIC_BYTES}},
         "bc_apply": {"ms": ms_b.value, "algorithmic_bytes": bc_bytes,
                      "achieved_GBps": bc_bytes / (ms_b.value * 1e-3) / 1e9,
                      "note": "two launches over 600k edge codes: launch-latency bound"},
