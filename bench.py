@@ -183,18 +183,10 @@ def ensemble_run(nmem, hours, sample, seed):
 SERVE_CLASSES = [(8192, 48), (32768, 32), (131072, 12)]   # slot capacity (elements) x count
 
 
-def _serve_payload(spec, hours):
-    from pihm_amd.workloads import _build, member_forcing
-    m = _build(spec)
-    f = member_forcing(spec, m, hours + 2, p_wet=0.1)
-    return m, f, m.initial_state()
-
-
 def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
     """C4: a Poisson stream of solves served with continuous batching on a slot ensemble."""
-    from concurrent.futures import ProcessPoolExecutor
     from pihm_amd.serve import Request, Scheduler, SlotEngine, latency_summary, slot_bytes
-    from pihm_amd.workloads import request_stream, tree_segments
+    from pihm_amd.workloads import request_stream, serve_payload, tree_segments, worker_pool
     stream = request_stream(duration_s=duration_s, rate=rate, seed=seed, hmin_days=1.0, hmax_days=30.0)
     # horizon: U(1, 30) units of horizon_unit_s, rounded to whole 60 s model steps
     for rq in stream:
@@ -205,9 +197,8 @@ def serve_run(duration_s, rate, horizon_unit_s, check_alone, seed):
     for ne, cnt in SERVE_CLASSES:
         caps += [(ne, tree_segments(ne), nrec_cap)] * cnt
     hbm = sum(slot_bytes(ne, nr) for ne, nr, _ in caps)
-    workers = min(16, len(os.sched_getaffinity(0)))
-    pool = ProcessPoolExecutor(max_workers=workers)
-    futs = [pool.submit(_serve_payload, rq, rq["t_end"] / 3600.0) for rq in stream]
+    pool = worker_pool()      # forked from the server started before the first HIP call
+    futs = [pool.submit(serve_payload, rq, rq["t_end"] / 3600.0) for rq in stream]
     reqs = [Request(i, rq["arrival"], rq["t_end"], futs[i]) for i, rq in enumerate(stream)]
     eng = SlotEngine(caps, _cfg(3600.0))
     t0 = time.perf_counter()
@@ -521,6 +512,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    # worker processes (mesh generation) fork from a server started before any HIP call
+    from pihm_amd.workloads import start_workers
+    start_workers()
     mesh = config_mesh("c1", seed=args.seed)
     W, K = args.warmup, args.steps
     forcing = StormForcing(W + K + 4, seed=11, nelem=mesh.nelem)

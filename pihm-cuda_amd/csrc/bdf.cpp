@@ -1,10 +1,14 @@
 // bdf.cpp - variable-order BDF (Nordsieck form) with Newton/SPGMR iteration.
 //
 // The integrator the MM-PIHM glue drives (src/ode.c:413-661): CVODE 2.9's
-// BDF/Newton/SPGMR method, restated here in this library's own code.  The
-// step logic follows the published method and the steps of the vendored
-// CVODE (file:line at each routine) so that step counts and trajectories
-// behave like the reference's; stability-limit detection is not performed.
+// BDF/Newton/SPGMR method.  The host-vector step logic is a close
+// restatement of the vendored CVODE routines (same locals, constants and
+// control flow; file:line at each routine) so that step counts and
+// trajectories behave like the reference's; stability-limit detection is not
+// performed.  Those routines are SUNDIALS code under the BSD-3 licence of
+// Lawrence Livermore National Security (Copyright (c) 2002-2016, LLNS;
+// UCRL-CODE-155951); its full text and disclaimer are in NOTICE.SUNDIALS at
+// the package root.
 //
 // Every vector operation goes through the N_Vector operation table.  When
 // the vectors are device vectors (pihm_nvector.h), groups of operations are

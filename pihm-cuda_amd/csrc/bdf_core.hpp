@@ -4,7 +4,10 @@
 // Constants and routines restate CVODE 2.9 (cvode/src/cvode/cvode.c,
 // sundials_iterative.c; file:line at each routine).  The routines are
 // templates over the integrator's state record, which provides the fields
-// they name (q, h, tau, l, tq, Hes, givens, yg, ...).
+// they name (q, h, tau, l, tq, Hes, givens, yg, ...).  The restated routines
+// are SUNDIALS code, BSD-3 licensed by Lawrence Livermore National Security
+// (Copyright (c) 2002-2016, LLNS; UCRL-CODE-155951); the licence text and
+// disclaimer are in NOTICE.SUNDIALS at the package root.
 #pragma once
 #include <cfloat>
 #include <cmath>

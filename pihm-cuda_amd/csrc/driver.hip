@@ -252,9 +252,17 @@ static int run_device(pihm_ctx *c, const pihm_run_config *cfg, double *y_inout, 
   const int saved_check = c->nan_check;
   c->nan_check = 0;   // NaN shows up in the norms; polled once at the end
   int rc = pihm_ctx_state_to_device(c, y_inout, y);
+  // a flag an earlier call left on this context is not this run's
+  if (!rc && (rc = pihm_ctx_poll_nan(c)) == PIHM_ERR_NAN) rc = PIHM_OK;
   if (!rc) rc = be->summary(be->user, y);
   if (!rc) rc = pihm_drive(be, y, cfg, st);
-  if (!rc) rc = pihm_ctx_poll_nan(c);
+  {
+    // Read and clear the flag whatever the run returned: a NaN that made the
+    // integrator give up is reported as the NaN, and the flag is not left set
+    // for the next call on the context.
+    const int rn = pihm_ctx_poll_nan(c);
+    if (rn == PIHM_ERR_NAN || (!rc && rn)) rc = rn;
+  }
   if (!rc || rc == PIHM_ERR_NAN) {
     int rc2 = pihm_ctx_state_from_device(c, y, y_inout);
     if (!rc) rc = rc2;

@@ -15,7 +15,10 @@
 // sundials_spgmr.c; file:line at each routine), in the form of the fused
 // single-solve device path: classical Gram-Schmidt with lazily applied
 // orthogonalisation, one synchronisation per Krylov iteration, error weights
-// computed in the same pass as the first prediction of each step.
+// computed in the same pass as the first prediction of each step.  The step
+// logic restates SUNDIALS routines, BSD-3 licensed by Lawrence Livermore
+// National Security (Copyright (c) 2002-2016, LLNS; UCRL-CODE-155951); the
+// licence text and disclaimer are in NOTICE.SUNDIALS at the package root.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

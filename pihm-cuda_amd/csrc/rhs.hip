@@ -1398,6 +1398,13 @@ extern "C" int pihm_ctx_set_noah_step(pihm_ctx *c, const double *sh2o0, const do
   if (gw && (!et || !nwtbl || !nroot || nlayer < 1 || nlayer > PIHM_NOAH_MAXLYR)) return PIHM_ERR_ARG;
   const long ne = c->ne;
   ElemDev &E = c->args.e;
+  // every argument is checked before the first upload, so a rejected call
+  // leaves the context as it was
+  if (gw)
+    for (long p = 0; p < ne; p++) {
+      const long ci = caller_elem(c, (int)p);
+      if (ci >= 0 && nroot[ci] > nlayer) return PIHM_ERR_ARG;   // a root layer past the given layers
+    }
   std::vector<double> hd(ne);
   auto up = [&](double *dst, const double *src, long stride_src) -> int {
     for (int p = 0; p < ne; p++) {
@@ -1414,15 +1421,18 @@ extern "C" int pihm_ctx_set_noah_step(pihm_ctx *c, const double *sh2o0, const do
   if (gw) {
     for (int j = 0; j < nlayer; j++)
       if ((rc = up(E.et + (long)j * ne, et + j, nlayer))) return rc;
+    // layers past the given ones carry no transpiration (the soil-water step
+    // reads every layer below nsoil)
+    if (nlayer < PIHM_NOAH_MAXLYR)
+      PIHM_HIP_TRY(hipMemsetAsync(E.et + (long)nlayer * ne, 0,
+                                  (size_t)(PIHM_NOAH_MAXLYR - nlayer) * ne * sizeof(double), c->stream));
     std::vector<int32_t> hi(ne);
     const int32_t *srcs[] = {nwtbl, nroot};
     int32_t *dsts[] = {E.nwtbl, E.nroot};
     for (int f = 0; f < 2; f++) {
       for (int p = 0; p < ne; p++) {
         const long ci = caller_elem(c, p);
-        const int32_t v = (ci >= 0) ? srcs[f][ci] : 0;
-        if (f == 1 && v > nlayer) return PIHM_ERR_ARG;   // a root layer past the given layers
-        hi[p] = v;
+        hi[p] = (ci >= 0) ? srcs[f][ci] : 0;
       }
       PIHM_HIP_TRY(hipMemcpyAsync(dsts[f], hi.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
       PIHM_HIP_TRY(hipStreamSynchronize(c->stream));

@@ -8,8 +8,12 @@
   a U(1, 30)-day horizon, and their own parameter and forcing seeds.
 
 Meshes are generated in parallel worker processes (the generator is numpy/scipy work).
+The workers are forked from a server process (multiprocessing's "forkserver"), never from
+the calling process, which may hold an initialised GPU runtime; start_workers() starts
+that server, and should be called before the first HIP call.
 """
 import math
+import multiprocessing
 import os
 from concurrent.futures import ProcessPoolExecutor
 
@@ -51,17 +55,38 @@ def _build(spec):
     raise err
 
 
+def start_workers():
+    """Start the server process that worker processes are forked from.  Called before the
+    first HIP call, it keeps every worker free of the GPU runtime's state and threads."""
+    from multiprocessing import forkserver
+    forkserver.ensure_running()
+
+
+def worker_pool(workers=None):
+    """A process pool whose workers fork from the forkserver, not from this process."""
+    if workers is None:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    return ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("forkserver"))
+
+
 def build_meshes(specs, workers=None):
     if workers is None:
         workers = min(16, len(os.sched_getaffinity(0)))
     if workers <= 1 or len(specs) < 4:
         return [_build(s) for s in specs]
-    with ProcessPoolExecutor(max_workers=workers) as ex:
+    with worker_pool(workers) as ex:
         return list(ex.map(_build, specs, chunksize=max(1, len(specs) // (4 * workers))))
 
 
 def member_forcing(spec, mesh, hours, p_wet=0.1):
     return StormForcing(hours, seed=spec["forcing_seed"], nelem=mesh.nelem, p_wet=p_wet)
+
+
+def serve_payload(spec, hours):
+    """A request's mesh, forcing (hours + 2 records) and initial state."""
+    m = _build(spec)
+    f = member_forcing(spec, m, hours + 2, p_wet=0.1)
+    return m, f, m.initial_state()
 
 
 def request_stream(duration_s=600.0, rate=20.0, seed=77, nmin=1000, nmax=100000, hmin_days=1.0,

@@ -98,13 +98,19 @@ def test_noah_argument_errors(c0):
     h.close()
 
 
-def test_noah_rhs_parity_full_size_100k():
-    """The 100,000-triangle, 2,047-segment workload with the Noah branches, per call."""
-    from pihm_amd.meshgen import config_mesh
-    m = config_mesh("c2", seed=1)
-    print(f"[noah parity] c2 ne={m.nelem} nr={m.nriver}", flush=True)
+def test_noah_rejected_hand_over_leaves_context_unchanged(c0):
+    """A hand-over refused for a root layer past the given layers uploads nothing: the
+    RHS after it gives the bits it gave before, top-layer water and fcr included."""
+    m = c0
     rng = np.random.default_rng(43)
-    h, o, _ = _setup(m, rng)
-    for k, y in enumerate(_states(m, rng, 2)):
-        _check(h, o, y, f"c2 noah state {k}")
+    h, o, nz = _setup(m, rng)
+    y = _states(m, rng, 2)[1]
+    hid0 = h.get_river_edge_ovlflow()
+    before = h.ode(y)[1]
+    bad = _noah_inputs(m, rng, 3)
+    bad["nroot"][m.nelem // 2] = 4
+    with pytest.raises(RuntimeError):
+        h.set_noah_step(bad["sh2o0"], bad["fcr"], bad["et"], bad["nwtbl"], bad["nroot"])
+    h.set_river_edge_ovlflow(hid0)
+    assert np.array_equal(h.ode(y)[1], before)
     h.close()

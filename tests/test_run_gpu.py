@@ -91,3 +91,48 @@ def test_gpu_run_matches_cpu_run_on_reference_example(example):
     print("gpu", st_g.as_dict(), "\ncpu", st_c.as_dict(), flush=True)
     assert _wrms(y_g, y_c) < 1.0
     assert abs(st_g.nst - st_c.nst) <= 0.1 * st_c.nst
+
+
+def test_nan_flag_does_not_outlive_its_run(c0):
+    """A NaN raised by one run on a context is reported by that run (PIHM_ERR_NAN) and
+    cleared; a stale flag left by an unchecked RHS call is not reported by the next run."""
+    import ctypes as C
+    from pihm_amd import _lib
+    L = _lib.lib()
+    m = c0
+    desc = m.desc()
+    ctx = C.c_void_p()
+    assert L.pihm_ctx_create(C.byref(desc), C.byref(ctx)) == 0
+    try:
+        cfg = run_config(600.0)
+        good = StormForcing(3, seed=5, nelem=m.nelem, p_wet=0.5)
+        bad = StormForcing(3, seed=5, nelem=m.nelem, p_wet=0.5)
+        bad.prcp[0, 0] = np.nan
+        y0 = m.initial_state()
+        print("[run] c0 NaN run, then a clean run on the same context", flush=True)
+        y = y0.copy()
+        st = _lib.RunStats()
+        fs = bad.struct()
+        assert L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), y.ctypes.data_as(_lib.c_double_p),
+                              C.byref(st)) == 1
+        y = y0.copy()
+        fs = good.struct()
+        assert L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), y.ctypes.data_as(_lib.c_double_p),
+                              C.byref(st)) == 0
+        # a flag set by an RHS call with the check off is not the next run's
+        L.pihm_ctx_set_nan_check(ctx, 0)
+        yv, ydv = L.N_VNew_Hip(m.neq), L.N_VNew_Hip(m.neq)
+        s = _lib.StepInputs()
+        pc = np.full(m.nelem, np.nan)
+        s.pcpdrp = pc.ctypes.data_as(_lib.c_double_p)
+        assert L.pihm_ctx_set_step_inputs(ctx, C.byref(s)) == 0
+        assert L.pihm_ctx_state_to_device(ctx, y0.ctypes.data_as(_lib.c_double_p), yv) == 0
+        assert L.PIHM_ODE_Hip(0.0, yv, ydv, ctx) == 0
+        L.N_VDestroy_Hip(yv)
+        L.N_VDestroy_Hip(ydv)
+        y = y0.copy()
+        assert L.pihm_run_gpu(ctx, C.byref(fs), C.byref(cfg), y.ctypes.data_as(_lib.c_double_p),
+                              C.byref(st)) == 0
+        assert np.all(np.isfinite(y))
+    finally:
+        L.pihm_ctx_destroy(ctx)
