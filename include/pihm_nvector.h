@@ -150,6 +150,20 @@ int      N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector *Y, double *d);
 double   N_VLinearSumWrms_Hip(double a, N_Vector x, double b, N_Vector y,
                               N_Vector z, N_Vector w);
 
+/* No counterpart in the reference or in SUNDIALS.  For callers that cannot
+ * allocate device or pinned memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): N_VDotProd_Hip on raw
+ * device arrays, *result_dev = <x, y>, launched on hip_stream (NULL = the null
+ * stream) without waiting.  Same summation tree, so the same bits as
+ * N_VDotProd_Hip.  The workspace is a fixed 8 KiB array in the library's
+ * device memory (each GPU holds its own until the process ends), so no call
+ * allocates; calls that overlap on one GPU share it and race.  Returns 0,
+ * -1 for n < 0, n > PIHM_NOALLOC_MAX_LENGTH or a NULL pointer, -2 on a
+ * runtime error. */
+#define PIHM_NOALLOC_MAX_LENGTH (1L << 33)
+int      pihm_nv_dotprod_noalloc(long int n, const double *x_dev, const double *y_dev,
+                                 double *result_dev, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -621,6 +621,68 @@ int pihm_nv_newton_rhs(double rl1, N_Vector zn1, N_Vector acor, double gamma, N_
   return 0;
 }
 
+// ---------------------------------------------------------------- no allocation
+// N_VDotProd_Hip on raw device arrays for callers that cannot allocate.  Its
+// workspace is a fixed device array, `scratch`: the first pass's partials
+// (one per workgroup, at most PIHM_RED_MAXBLOCKS) and, in the last slot, the
+// two counters the second pass updates.  The second pass writes the result to
+// the caller's device pointer instead of pinned host memory.  Each GPU holds
+// its own copy of the array until the process ends; its address on a device
+// comes from hipGetSymbolAddress, once per device, with that device current.
+}  // extern "C"
+
+namespace {
+constexpr int kScratchDoubles = PIHM_RED_MAXBLOCKS + 1;   // partials, then ctr|flag
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::atomic<double *> g_scratch[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up on the
+// device's first call).  The array's name in host code is not a device
+// address, so only this pointer is handed to kernels.
+int scratch_ptr(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -2;
+  double *p = g_scratch[dev].load(std::memory_order_acquire);
+  if (!p) {
+    void *q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return -2;
+    p = (double *)q;
+    g_scratch[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int pihm_nv_dotprod_noalloc(long int n, const double *x_dev, const double *y_dev, double *result_dev,
+                            void *hip_stream) {
+  if (n < 0 || n > PIHM_NOALLOC_MAX_LENGTH || !result_dev || (n > 0 && (!x_dev || !y_dev))) return -1;
+  double *ws = nullptr;
+  if (scratch_ptr(&ws)) return -2;
+  hipStream_t st = (hipStream_t)hip_stream;
+  RedArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n;
+  a.kind = R_DOT;
+  a.nvec = 1;
+  a.x = x_dev;
+  a.ys[0] = y_dev;
+  a.work.partials = ws;
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(kRedBlock), 0, st, a);
+  // second pass: the same tree as N_VDotProd_Hip's; its counters live in the
+  // last slot of the array, the result goes to the caller's device memory
+  unsigned *ctr = reinterpret_cast<unsigned *>(ws + PIHM_RED_MAXBLOCKS);
+  hipLaunchKernelGGL(k_partials_final, dim3(1), dim3(kRedThreads), 0, st, nb, 1, (unsigned)COMB_SUM,
+                     (const double *)ws, result_dev, (double *)nullptr, ctr, (volatile unsigned *)(ctr + 1),
+                     (double *)nullptr, 0.0);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 }  // extern "C"
 
 namespace {
