@@ -11,7 +11,9 @@ import sys
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# PIHM_ORACLE_LIB points the tests at another build of the same sources (a copy with a
+# planted mistake, to show that the pins catch it)
+LIB_PATH = os.environ.get("PIHM_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "pihm-cuda_amd"))
 from pihm_amd import _lib as P  # noqa: E402  (structure definitions only)
 
