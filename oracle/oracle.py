@@ -416,3 +416,53 @@ def noah_soil_step(dt, nsoil, sldpth, soil, gw, infil, runoff2, edir, et, state)
                             state["nwtbl"].ctypes.data_as(C.POINTER(C.c_int)), _dptr(state["satdpth"]),
                             _dptr(state["runoff2_lyr"]), _dptr(state["smflxv"]), _dptr(state["runoff3"]))
     return state
+
+
+# ----------------------------------------------------------------------------- Noah land-surface step
+_NP_SCALARS = ["shdfac", "cmcfactr", "laimin", "laimax", "emissmin", "emissmax", "albedomin", "albedomax",
+               "z0min", "z0max", "snup", "cfactr", "rgl", "rsmin", "rsmax", "topt", "hs", "smcmax", "smcmin",
+               "smcref", "smcwlt", "smcdry", "quartz", "csoil", "alpha", "beta", "sbeta", "salp", "fxexp", "czil",
+               "lvcoef", "zbot", "tbot", "snoalb", "zlvl", "zlvl_wind"]
+NOAH_PAR_DT = np.dtype([("nsoil", np.intc), ("nroot", np.intc), ("sldpth", np.float64, 11)] +
+                       [(k, np.float64) for k in _NP_SCALARS], align=True)
+NOAH_STATE_DT = np.dtype([("smc", np.float64, 11), ("sh2o", np.float64, 11), ("stc", np.float64, 11)] +
+                         [(k, np.float64) for k in ("t1", "sneqv", "snowh", "sndens", "cmc", "ch", "cm",
+                                                    "snotime1", "albedo", "emissi", "q1")], align=True)
+NOAH_FORC_DT = np.dtype([(k, np.float64) for k in ("sfctmp", "rh", "sfcspd", "soldn", "longwave", "sfcprs",
+                                                   "prcp", "proj_lai")], align=True)
+NOAH_OUT_DT = np.dtype([(k, np.float64) for k in ("edir", "ett", "ec")] + [("et", np.float64, 11)] +
+                       [(k, np.float64) for k in ("pcpdrp", "drip", "esnow", "etp", "snomlt", "ssoil", "sheat",
+                                                  "eta", "fdown", "sncovr", "beta", "rc", "pc", "flx1", "flx2",
+                                                  "flx3", "dew", "soilw", "soilm", "cmcmax", "rr", "rch", "epsca",
+                                                  "q2", "q2sat", "th2")], align=True)
+
+
+def noah_lsm_step(dt, par, state, forc):
+    """Noah() over every element (noahlsm_oracle.c): par, state and forc are structured
+    arrays of NOAH_PAR_DT, NOAH_STATE_DT (updated in place) and NOAH_FORC_DT; returns
+    the NOAH_OUT_DT outputs."""
+    L = lib()
+    if not hasattr(L, "_noah_protos"):
+        L.oracle_noah_step.restype = None
+        L.oracle_noah_step.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L._noah_protos = True
+    assert par.dtype == NOAH_PAR_DT and state.dtype == NOAH_STATE_DT and forc.dtype == NOAH_FORC_DT
+    assert state.flags.c_contiguous and par.flags.c_contiguous and forc.flags.c_contiguous
+    out = np.zeros(len(par), NOAH_OUT_DT)
+    L.oracle_noah_step(len(par), float(dt), par.ctypes.data, state.ctypes.data, forc.ctypes.data, out.ctypes.data)
+    return out
+
+
+def noah_fn(name):
+    """A single function of the Noah restatement, for the pins."""
+    L = lib()
+    f = getattr(L, name)
+    d = C.c_double
+    sig = {"oracle_CSnow": (d, [d]), "oracle_SnFrac": (d, [d, d, d]), "oracle_TDfCnd": (d, [d] * 5),
+           "oracle_Pspmu": (d, [d]), "oracle_Psphu": (d, [d]),
+           "oracle_SfcDifOff": (None, [d] * 7 + [C.POINTER(d), C.POINTER(d)]),
+           "oracle_TBnd": (d, [d, d, dp, d, C.c_int, C.c_int]), "oracle_TmpAvg": (d, [d, d, d, dp, C.c_int]),
+           "oracle_FrH2O": (d, [d] * 7),
+           "oracle_SnowPack": (None, [d, d, C.POINTER(d), C.POINTER(d), d, d])}[name]
+    f.restype, f.argtypes = sig
+    return f
