@@ -11,6 +11,7 @@ Prints one JSON line.
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -384,8 +385,8 @@ def noah_rhs_run(seed, iters=200, coupled=True):
                          "achieved_GBps": soil_bytes / (soil_ms * 1e-3) / 1e9,
                          "traffic": NOAH_SOIL_TRAFFIC_BYTES,
                          "note": "once per model step; one thread per element column"},
-           "note": "the per-LSM-step values (edir, ett, per-layer et) come from a host Noah step; "
-                   "the Noah energy balance (SFlx) is not on the device"}
+           "note": "per-LSM-step values (edir, ett, per-layer et) held fixed: the RHS and the soil-water "
+                   "step alone; the coupled run with the Noah step is the flux_pihm object"}
     if not coupled:
         return out
     # one simulated hour of the coupled run (the _NOAH_ RHS in the integrator, the soil-water step at
@@ -415,6 +416,100 @@ def noah_rhs_run(seed, iters=200, coupled=True):
     return out
 
 
+# algorithmic bytes of one Noah energy-balance step per element with N soil layers: reads nsoil, nroot,
+# station (12 B), 22 land-cover/soil parameters, smcmax, smcmin, alpha, beta, 10 carried scalars, q1 and
+# LAI (264 B), and sldpth, smc, sh2o, stc per layer (32N B); writes smc, sh2o, stc per layer (24N B),
+# the 11 carried values, 16 fluxes, pcpdrp/edir/ett, the 11 layers of et and sh2o[0] (336 B)
+def noah_step_bytes_per_elem(nsoil):
+    return 316 + 32 * nsoil + 336 + 24 * nsoil
+
+
+def _flux_inputs(seed, hours):
+    """C2's Flux-PIHM inputs: the 100,000-triangle mesh, 11 soil layers, one station with
+    hourly synthetic forcing, LAI between each element's minimum and maximum."""
+    from pihm_amd.noahlsm import synthetic_noah, synthetic_noah_meteo, synthetic_noah_state
+    m = config_mesh("c2", seed=seed)
+    p = synthetic_noah(m, seed=seed)
+    state = synthetic_noah_state(m, p, seed=seed, t0=283.15)
+    ft, data = synthetic_noah_meteo(1, int(math.ceil(hours)) + 1, seed=seed)
+    rng = np.random.default_rng(seed)
+    lai = p["laimin"] + (p["laimax"] - p["laimin"]) * rng.uniform(0.0, 1.0, m.nelem)
+    return m, p, state, ft, data, lai
+
+
+def _noah_step_ms(m, p, state, ft, data, lai, iters):
+    """HIP-event time of one Noah step at noon (sunlit, every branch of the canopy and
+    soil-heat physics live), `iters` steps back to back after 3 untimed ones."""
+    from pihm_amd.noahlsm import flux_pihm_setup
+    h, soil, lsm = flux_pihm_setup(m, p, state, ft, data, lai)
+    try:
+        lsm.time(12.0 * 3600.0, 900.0, 3)
+        return lsm.time(12.0 * 3600.0, 900.0, iters)
+    finally:
+        lsm.close(); soil.close(); h.close()
+
+
+def flux_pihm_run(seed, hours=24.0, cpu_hours=0.25, nthreads=0, iters=50):
+    """C2, the Flux-PIHM coupled run, on the 100,000-triangle mesh with 11 soil layers: one
+    Noah step timed with HIP events (the stations' interpolation, k_noah, the hand-over),
+    the coupled model over the first `hours` simulated hours (pihm_run_gpu_flux: the Noah
+    step every 15 minutes, the _NOAH_ RHS in the integrator, the soil-water step every
+    model step), and (nthreads > 0) the oracle's host run over the first `cpu_hours`
+    beside the device over the same window."""
+    from pihm_amd.noahlsm import MAXLYR, flux_pihm_setup, run_gpu_flux
+    m, p, state, ft, data, lai = _flux_inputs(seed, max(hours, cpu_hours, 13.0))
+    ne = m.nelem
+    ms = _noah_step_ms(m, p, state, ft, data, lai, iters)
+    step_bytes = ne * noah_step_bytes_per_elem(MAXLYR)
+    ach = step_bytes / (ms * 1e-3) / 1e9
+
+    def device(hrs):
+        h, soil, lsm = flux_pihm_setup(m, p, state, ft, data, lai)
+        t0 = time.perf_counter()
+        rc, y, st = run_gpu_flux(lsm, _cfg(hrs * 3600.0), m.initial_state())
+        wall = time.perf_counter() - t0
+        lsm.close(); soil.close(); h.close()
+        if rc != 0:
+            raise RuntimeError(f"Flux-PIHM device run failed: rc={rc} {st.as_dict()}")
+        return wall, st
+
+    wall, st = device(hours)
+    out = {"workload": f"c2: {ne} triangles, {m.nriver} river segments, {MAXLYR} soil layers, NEQ {m.neq}; "
+                       f"one station, hourly forcing T = 10 + 8 sin(diurnal) degC, storm precipitation",
+           "window": f"first {hours:g} simulated hours from midnight: {int(hours * 4)} Noah steps, "
+                     f"{int(hours * 60)} model steps",
+           "sim_days_per_s": (hours / 24.0) / wall, "wall_s": wall, "nst": st.nst, "nfe": st.nfe,
+           "nli": st.nli, "ncfn": st.ncfn,
+           "noah_step": {"kernel": "one Noah step: k_noah_meteo + k_noah + the hand-over's k_gwtransp",
+                         "ms_per_step": ms, "bound": "FP64 latency (exp/log/pow chains, fixed-point and Newton "
+                                                     "iterations per element)",
+                         "algorithmic_bytes": step_bytes, "achieved_GBps": ach,
+                         "frac_of_hbm_peak": ach / HBM_PEAK_GBS, "per_simulated_hour": 4}}
+    if nthreads > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        gwall, _ = device(cpu_hours)
+        noah = dict(smcmin=p["smcmin"], et=np.zeros((ne, MAXLYR)), nwtbl=np.zeros(ne, np.intc),
+                    nroot=p["nroot"], nsoil=p["nsoil"], sldpth=p["sldpth"], smcmax=p["smcmax"], frzx=p["frzx"],
+                    smc=state["smc"], sh2o=state["sh2o"])
+        run = O.OracleRun(m, StormForcing(4, nelem=ne), nthreads=nthreads, noah=noah)
+        par, ost = O.noah_arrays(m, p, state)
+        run.set_noah_lsm(par, ost, p["station"], ft, data, lai, 900.0)
+        t0 = time.perf_counter()
+        rc_o, _, sto = run.run(m.initial_state(), _cfg(cpu_hours * 3600.0))
+        cwall = time.perf_counter() - t0
+        if rc_o != 0:
+            raise RuntimeError(f"Flux-PIHM host run failed: rc={rc_o}")
+        cpu_rate = (cpu_hours / 24.0) / cwall
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "sim-days/s", "cores": nthreads, "kind": "port",
+                               "sample": f"first {cpu_hours:g} simulated hours of C2 (oracle Noah step, RHS and "
+                                         f"soil-water step, same integrator, host vectors, OpenMP {nthreads} "
+                                         f"threads)",
+                               "wall_s": cwall, "nst": sto.nst, "gpu_same_window": (cpu_hours / 24.0) / gwall,
+                               "gpu_over_cpu_same_window": ((cpu_hours / 24.0) / gwall) / cpu_rate}
+    return out
+
+
 def cpu_run(mesh, forcing, hours, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -430,7 +525,8 @@ def profile_kernels(iters, seed):
     evaluations at C1 on a mid-storm state (then `iters` more timed with HIP events), then
     `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB read and 512 MiB
     written per launch), the calibration kernel for the byte counters.  Between the two,
-    the producers at C1 and the plain and _NOAH_ RHS at C2 (100,000 triangles)."""
+    the producers at C1, the plain and _NOAH_ RHS at C2 (100,000 triangles) and the Noah
+    step at C2."""
     L = _lib.lib()
     mesh = config_mesh("c1", seed=seed)
     desc = mesh.desc()
@@ -457,6 +553,9 @@ def profile_kernels(iters, seed):
     lsm.close()
     h.close()
     noah = noah_rhs_run(seed, iters, coupled=False)
+    from pihm_amd.noahlsm import MAXLYR
+    fm, fp, fs, fft, fdata, flai = _flux_inputs(seed, 13.0)
+    ms_noah = _noah_step_ms(fm, fp, fs, fft, fdata, flai, iters)
     n = 64 << 20
     a = L.N_VNew_Hip(n)
     b = L.N_VNew_Hip(n)
@@ -473,7 +572,9 @@ def profile_kernels(iters, seed):
                       "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem,
                       "noah_rhs_c2_ms_per_eval_events": noah["ms_per_eval"],
                       "plain_rhs_c2_ms_per_eval_events": noah["plain_ms_per_eval"],
-                      "noah_rhs_c2_bytes_algorithmic": noah["roofline"]["algorithmic_bytes"]}))
+                      "noah_rhs_c2_bytes_algorithmic": noah["roofline"]["algorithmic_bytes"],
+                      "noah_step_c2_ms_events": ms_noah,
+                      "noah_step_c2_bytes_algorithmic": fm.nelem * noah_step_bytes_per_elem(MAXLYR)}))
 
 
 def main():
@@ -492,6 +593,9 @@ def main():
     ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-noah", action="store_true")
     ap.add_argument("--closure-days", type=float, default=10.0)
+    ap.add_argument("--no-flux", action="store_true")
+    ap.add_argument("--flux-hours", type=float, default=24.0, help="C2 window (simulated hours)")
+    ap.add_argument("--flux-cpu-hours", type=float, default=0.25, help="C2 CPU baseline sample (simulated hours)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -576,6 +680,10 @@ def main():
     if not args.no_noah:
         _log("RHS with the Noah branches at 100,000 triangles")
         out["noah_rhs"] = noah_rhs_run(args.seed)
+    if not args.no_flux:
+        _log(f"Flux-PIHM: {args.flux_hours} simulated hours at 100,000 triangles")
+        nth = 0 if args.no_cpu else (args.cpu_threads or len(os.sched_getaffinity(0)))
+        out["flux_pihm"] = flux_pihm_run(args.seed, args.flux_hours, args.flux_cpu_hours, nth)
     if not args.no_closure:
         _log(f"closure: {args.closure_days} days at 535 triangles")
         out["closure"] = closure_run(args.closure_days)

@@ -64,6 +64,17 @@ int oracle_meteo_interp(const pihm_meteo *m, double t, double *met)
     return 0;
 }
 
+/* The same for records of nvar variables: met[v * nstation + k]. */
+int oracle_meteo_interp_n(const double *ftime, const double *data, int nstation, int nrec, int nvar, double t,
+                          double *met)
+{
+    int k;
+    for (k = 0; k < nstation; k++)
+        if (interp_series(ftime + (long)k * nrec, data + (long)k * nrec * nvar, nrec, nvar, t, met + k, nstation))
+            return 1;
+    return 0;
+}
+
 /* ApplyElemBc and ApplyRiverBc (src/forcing.c:127-177, 214-244): every series
  * interpolated at t, then each element edge (elem_bc[3*i+j], caller order)
  * and river segment with a nonzero code takes series |code|-1's value.

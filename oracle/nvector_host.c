@@ -8,8 +8,8 @@
  * each function), including the special cases of N_VLinearSum (:421-506)
  * whose rounding differs from the general formula (a == b, a == -b).
  * With nthreads > 1 the loops run under OpenMP like
- * cvode/src/nvec_openmp/nvector_openmp.c (reductions then depend on the
- * thread count, as there).
+ * cvode/src/nvec_openmp/nvector_openmp.c; the sums then depend on the thread
+ * count, as there, but not on the order in which the threads finish.
  */
 #include "../include/pihm_nvector.h"
 
@@ -177,12 +177,44 @@ static void host_addconst(N_Vector x, double b, N_Vector z)
 #pragma omp parallel for num_threads(nt) if (nt > 1)
     for (i = 0; i < N; i++) zd[i] = xd[i] + b;
 }
+/* With nthreads > 1 a sum is split into one contiguous block per thread, each
+ * summed left to right, and the blocks' sums are added in thread order: an OpenMP
+ * reduction clause would add them in the order the threads finish, so two runs
+ * on the same inputs could differ in the last bit and the integrator's step
+ * choices after them.  Up to HOST_MAXT threads take part. */
+#define HOST_MAXT 64
+#define HOST_BLOCK_SUM(N, nt, TERM, out)                                          \
+    do {                                                                           \
+        double part_[HOST_MAXT];                                                   \
+        int used_ = 1, t_;                                                         \
+        const int want_ = (nt) < 1 ? 1 : ((nt) > HOST_MAXT ? HOST_MAXT : (nt));   \
+        _Pragma("omp parallel num_threads(want_) if (want_ > 1)")                  \
+        {                                                                          \
+            int me_ = 0, all_ = 1;                                                 \
+            long int i, lo_, hi_;                                                  \
+            double s_ = 0.0;                                                       \
+            HOST_THREAD_IDS(me_, all_);                                            \
+            lo_ = (long int)((N) * (double)me_ / all_);                            \
+            hi_ = (long int)((N) * (double)(me_ + 1) / all_);                      \
+            if (me_ == all_ - 1) hi_ = (N);                                        \
+            for (i = lo_; i < hi_; i++) s_ += (TERM);                              \
+            part_[me_] = s_;                                                       \
+            if (me_ == 0) used_ = all_;                                            \
+        }                                                                          \
+        (out) = 0.0;                                                               \
+        for (t_ = 0; t_ < used_; t_++) (out) += part_[t_];                         \
+    } while (0)
+#if defined(_OPENMP)
+#define HOST_THREAD_IDS(me, all) ((me) = omp_get_thread_num(), (all) = omp_get_num_threads())
+#else
+#define HOST_THREAD_IDS(me, all) ((me) = 0, (all) = 1)
+#endif
+
 /* N_VDotProd_Serial :637 (left to right) */
 static double host_dot(N_Vector x, N_Vector y)
 {
-    long int i, N = HL(x); double *xd = HD(x), *yd = HD(y), sum = 0.0; int nt = NT(x); (void)nt;
-#pragma omp parallel for reduction(+:sum) num_threads(nt) if (nt > 1)
-    for (i = 0; i < N; i++) sum += xd[i] * yd[i];
+    long int N = HL(x); double *xd = HD(x), *yd = HD(y), sum; int nt = NT(x);
+    HOST_BLOCK_SUM(N, nt, xd[i] * yd[i], sum);
     return sum;
 }
 static double host_maxnorm(N_Vector x)
@@ -195,9 +227,8 @@ static double host_maxnorm(N_Vector x)
 /* N_VWrmsNorm_Serial :673-691 */
 static double host_wrms(N_Vector x, N_Vector w)
 {
-    long int i, N = HL(x); double *xd = HD(x), *wd = HD(w), sum = 0.0; int nt = NT(x); (void)nt;
-#pragma omp parallel for reduction(+:sum) num_threads(nt) if (nt > 1)
-    for (i = 0; i < N; i++) { double p = xd[i] * wd[i]; sum += p * p; }
+    long int N = HL(x); double *xd = HD(x), *wd = HD(w), sum; int nt = NT(x);
+    HOST_BLOCK_SUM(N, nt, (xd[i] * wd[i]) * (xd[i] * wd[i]), sum);
     return sqrt(sum / N);
 }
 static double host_wrmsmask(N_Vector x, N_Vector w, N_Vector id)

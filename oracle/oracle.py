@@ -247,6 +247,25 @@ class OracleRun:
             self.L.oracle_backend_set_noah_soil(self.be, ip(k[6]), _dptr(k[7]), _dptr(k[8]), _dptr(k[9]),
                                                 _dptr(k[10]), _dptr(k[11]))
 
+    def set_noah_lsm(self, par, state, station, ftime, data, lai, etstep):
+        """The Noah step at every LSM step (oracle_backend_set_noah_lsm): par and state are
+        NOAH_PAR_DT / NOAH_STATE_DT arrays (state updated in place as the run goes)."""
+        L = self.L
+        if not hasattr(L, "_noah_lsm_proto"):
+            L.oracle_backend_set_noah_lsm.restype = None
+            L.oracle_backend_set_noah_lsm.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+                                                      C.c_int, C.c_int, dp, dp, dp, C.c_double]
+            L._noah_lsm_proto = True
+        assert par.dtype == NOAH_PAR_DT and state.dtype == NOAH_STATE_DT
+        st = np.ascontiguousarray(np.asarray(station), dtype=np.intc)
+        ft = np.ascontiguousarray(ftime, dtype=np.float64)
+        dd = np.ascontiguousarray(data, dtype=np.float64)
+        la = np.ascontiguousarray(lai, dtype=np.float64)
+        self._lsm_keep = (par, state, st, ft, dd, la)
+        L.oracle_backend_set_noah_lsm(self.be, par.ctypes.data, state.ctypes.data,
+                                      st.ctypes.data_as(C.POINTER(C.c_int)), dd.shape[0], dd.shape[1],
+                                      _dptr(ft), _dptr(dd), _dptr(la), float(etstep))
+
     def noah_soil_state(self):
         ne = self.mesh.nelem
         smc, sh2o = np.zeros((ne, 11)), np.zeros((ne, 11))
@@ -465,4 +484,51 @@ def noah_fn(name):
            "oracle_FrH2O": (d, [d] * 7),
            "oracle_SnowPack": (None, [d, d, C.POINTER(d), C.POINTER(d), d, d])}[name]
     f.restype, f.argtypes = sig
+    return f
+
+
+def noah_arrays(mesh, p, state, lai=None):
+    """The oracle's per-element Noah records from the per-element dicts of
+    pihm_amd.noahlsm (synthetic_noah / synthetic_noah_state) and the mesh's van Genuchten
+    parameters.  Returns (par, st) structured arrays."""
+    ne = mesh.nelem
+    par = np.zeros(ne, NOAH_PAR_DT)
+    par["nsoil"], par["nroot"], par["sldpth"] = p["nsoil"], p["nroot"], p["sldpth"]
+    for k in _NP_SCALARS:
+        if k in ("alpha", "beta"):
+            par[k] = mesh.a[k]
+        else:
+            par[k] = p[k]
+    st = np.zeros(ne, NOAH_STATE_DT)
+    st["smc"], st["sh2o"], st["stc"] = state["smc"], state["sh2o"], state["stc"]
+    st["t1"], st["sneqv"], st["snowh"], st["cmc"] = state["t1"], state["sneqv"], state["snowh"], state["cmc"]
+    st["ch"] = st["cm"] = 1.0e-4
+    st["albedo"], st["emissi"], st["q1"] = 0.18, 0.96, -999.0
+    return par, st
+
+
+def noah_meteo_interp(ftime, data, t):
+    """The stations' records at t (IntrplForc): [7, nstation]."""
+    L = lib()
+    if not hasattr(L, "_interp_n_proto"):
+        L.oracle_meteo_interp_n.restype = C.c_int
+        L.oracle_meteo_interp_n.argtypes = [dp, dp, C.c_int, C.c_int, C.c_int, C.c_double, dp]
+        L._interp_n_proto = True
+    ft = np.ascontiguousarray(ftime, dtype=np.float64)
+    dd = np.ascontiguousarray(data, dtype=np.float64)
+    met = np.zeros((dd.shape[2], dd.shape[0]))
+    if L.oracle_meteo_interp_n(_dptr(ft), _dptr(dd), dd.shape[0], dd.shape[1], dd.shape[2], float(t), _dptr(met)):
+        raise ValueError("t outside the records")
+    return met
+
+
+def noah_forcing(met, station, lai):
+    """ApplyMeteoForc with _NOAH_ (src/forcing.c:340-353) per element."""
+    ne = len(station)
+    f = np.zeros(ne, NOAH_FORC_DT)
+    f["prcp"] = met[0][station] / 1000.0
+    f["sfctmp"], f["rh"], f["sfcspd"] = met[1][station], met[2][station], met[3][station]
+    f["soldn"] = np.maximum(met[4][station], 0.0)
+    f["longwave"], f["sfcprs"] = met[5][station], met[6][station]
+    f["proj_lai"] = lai
     return f

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include "pihm_oracle.h"
+#include "noahlsm_oracle.h"
 #include "../include/pihm_driver.h"
 
 N_Vector oracle_nv_new(long int length, int nthreads);
@@ -33,11 +34,26 @@ typedef struct oracle_noah_soil_host {
     int started;
 } oracle_noah_soil_host;
 
+/* The Noah land-surface step at every LSM step (caller's arrays, which must
+ * outlive the backend): parameters and carried state per element, station
+ * records (7 variables, record-major), the element's station, projected LAI. */
+typedef struct oracle_noah_lsm_host {
+    const oracle_noahlsm_par *par;
+    oracle_noahlsm_state *st;
+    const int *station;
+    int nstation, nrec;
+    const double *ftime, *data, *lai;
+    double etstep;
+    double *met;
+    oracle_noahlsm_out *out;
+} oracle_noah_lsm_host;
+
 typedef struct oracle_backend {
     const pihm_mesh_desc *m;
     oracle_state *s;
     oracle_bj *bj;
     oracle_noah_soil_host *ns;
+    oracle_noah_lsm_host *nl;
     int nstation, nrec;
     double rec_dt;
     double *prcp, *edir, *ett;
@@ -182,6 +198,65 @@ void oracle_backend_noah_soil_state(const oracle_backend *b, double *smc, double
     memcpy(sh2o, b->ns->sh2o, nl * sizeof(double));
 }
 
+int oracle_meteo_interp_n(const double *ftime, const double *data, int nstation, int nrec, int nvar, double t,
+                          double *met);   /* lsm_oracle.c */
+
+/* Noah() every LSM step (src/pihm.c:25-63 with _NOAH_) on the soil columns of
+ * the soil-water step: the stations' records at t, the element's forcing as
+ * ApplyMeteoForc sets it (src/forcing.c:340-353), then what the RHS reads. */
+static int oracle_noah_forcing(void *user, double t)
+{
+    oracle_backend *b = (oracle_backend *)user;
+    oracle_noah_lsm_host *n = b->nl;
+    oracle_noah_soil_host *ns = b->ns;
+    oracle_state *s = b->s;
+    const int ne = b->m->nelem;
+    int i;
+    if (oracle_meteo_interp_n(n->ftime, n->data, n->nstation, n->nrec, 7, t, n->met)) return -1;
+#if defined(_OPENMP)
+# pragma omp parallel for num_threads(b->nthreads)
+#endif
+    for (i = 0; i < ne; i++) {
+        const size_t o = (size_t)i * 11;
+        const int k = n->station[i];
+        oracle_noahlsm_forc f;
+        int j;
+        f.prcp = n->met[0 * n->nstation + k] / 1000.0;
+        f.sfctmp = n->met[1 * n->nstation + k];
+        f.rh = n->met[2 * n->nstation + k];
+        f.sfcspd = n->met[3 * n->nstation + k];
+        f.soldn = n->met[4 * n->nstation + k];
+        f.soldn = (f.soldn > 0.0) ? f.soldn : 0.0;
+        f.longwave = n->met[5 * n->nstation + k];
+        f.sfcprs = n->met[6 * n->nstation + k];
+        f.proj_lai = n->lai[i];
+        memcpy(n->st[i].smc, ns->smc + o, sizeof(n->st[i].smc));
+        memcpy(n->st[i].sh2o, ns->sh2o + o, sizeof(n->st[i].sh2o));
+        oracle_noah_element(&n->par[i], &n->st[i], &f, n->etstep, &n->out[i]);
+        memcpy(ns->smc + o, n->st[i].smc, sizeof(n->st[i].smc));
+        memcpy(ns->sh2o + o, n->st[i].sh2o, sizeof(n->st[i].sh2o));
+        s->pcpdrp[i] = n->out[i].pcpdrp;
+        s->edir[i] = n->out[i].edir;
+        s->ett[i] = n->out[i].ett;
+        for (j = 0; j < 11; j++) s->et[o + j] = n->out[i].et[j];
+        s->sh2o0[i] = n->st[i].sh2o[0];
+    }
+    return 0;
+}
+
+void oracle_backend_set_noah_lsm(oracle_backend *b, const oracle_noahlsm_par *par, oracle_noahlsm_state *st,
+                                 const int *station, int nstation, int nrec, const double *ftime,
+                                 const double *data, const double *lai, double etstep)
+{
+    oracle_noah_lsm_host *n = (oracle_noah_lsm_host *)calloc(1, sizeof(*n));
+    n->par = par; n->st = st; n->station = station; n->nstation = nstation; n->nrec = nrec;
+    n->ftime = ftime; n->data = data; n->lai = lai; n->etstep = etstep;
+    n->met = (double *)calloc((size_t)7 * nstation, sizeof(double));
+    n->out = (oracle_noahlsm_out *)calloc((size_t)b->m->nelem, sizeof(oracle_noahlsm_out));
+    b->nl = n;
+    b->be.apply_forcing = oracle_noah_forcing;
+}
+
 /* the initial Summary (the storage the first balance starts from) */
 int oracle_backend_start(oracle_backend *b, N_Vector y) { return oracle_summary(b, y); }
 
@@ -245,6 +320,10 @@ void oracle_backend_free(oracle_backend *b)
         b->ns = NULL;
     }
     if (!b) return;
+    if (b->nl) {
+        free(b->nl->met); free(b->nl->out); free(b->nl);
+        b->nl = NULL;
+    }
     oracle_bj_free(b->bj);
     free(b->prcp); free(b->edir); free(b->ett); free(b->station);
     free(b);

@@ -212,3 +212,16 @@ static inline long caller_river(const pihm_ctx *c, int r) {
   }
   return -1;
 }
+
+struct pihm_noah_soil;
+namespace pihm {
+// The soil column a soil-water step holds (noahsoil.hip), shared with the Noah
+// land-surface step: device order, layer arrays layer-major [k * ne + p].
+struct NoahSoilView {
+  pihm_ctx *ctx;
+  int ne;
+  int32_t *nsoil;
+  double *sldpth, *smcmax, *smc, *sh2o;
+};
+NoahSoilView noah_soil_view(pihm_noah_soil *s);
+}  // namespace pihm

@@ -11,6 +11,7 @@
 #include "../../include/pihm_lsm.h"
 #include "../../include/pihm_output.h"
 #include "../../include/pihm_noahsoil.h"
+#include "../../include/pihm_noahlsm.h"
 #include "common.hpp"
 #include "ctx.hpp"
 #include "nvector_impl.hpp"
@@ -377,4 +378,55 @@ extern "C" int pihm_run_gpu_lsm(pihm_ctx *c, pihm_lsm *lsm, pihm_bc *bc, const p
   if (!lsm) return PIHM_ERR_ARG;
   const pihm_producers p{lsm, bc, nullptr};
   return pihm_run_gpu_producers(c, &p, cfg, y_inout, st);
+}
+
+// ---------------------------------------------------------------- Flux-PIHM runs
+namespace pihm {
+pihm_noah_soil *noah_lsm_soil(pihm_noah_lsm *n);   // noahlsm.hip
+pihm_ctx *noah_lsm_context(pihm_noah_lsm *n);
+}
+
+namespace {
+
+// The _NOAH_ model-step loop (src/pihm.c:10-187): Noah every LSM step on the
+// soil column the last soil-water step left, Summary with MassBalance and the
+// soil-water step after every model step.
+struct FluxRun {
+  pihm_ctx *ctx;
+  pihm_noah_lsm *lsm;
+  pihm_noah_soil *soil;
+  double etstep;
+  int summaries;
+};
+
+int flux_forcing(void *user, double t) {
+  FluxRun *r = (FluxRun *)user;
+  return pihm_noah_lsm_step(r->lsm, t, r->etstep);
+}
+
+int flux_summary(void *user, N_Vector y) {
+  FluxRun *r = (FluxRun *)user;
+  const double *yd = N_VGetDeviceArrayPointer_Hip(y);
+  const int rc = (r->summaries++ == 0) ? pihm_noah_soil_start(r->soil, yd)
+                                       : pihm_noah_soil_summary(r->soil, r->ctx->dt, yd);
+  if (rc) return rc;
+  return copy_ws0(r->ctx, y);
+}
+
+}  // namespace
+
+extern "C" int pihm_run_gpu_flux(pihm_noah_lsm *n, const pihm_run_config *cfg, double *y_inout,
+                                 pihm_run_stats *st) {
+  if (!n || !cfg || !y_inout) return PIHM_ERR_ARG;
+  pihm_ctx *c = pihm::noah_lsm_context(n);
+  if (!c || !c->noah) return PIHM_ERR_ARG;
+  FluxRun r{c, n, pihm::noah_lsm_soil(n), cfg->etstep, 0};
+  pihm_backend be;
+  std::memset(&be, 0, sizeof(be));
+  be.rhs = gpu_rhs;
+  be.rhs_data = c;
+  be.apply_forcing = flux_forcing;
+  be.summary = flux_summary;
+  be.user = &r;
+  return run_device(c, cfg, y_inout, st, &be);
 }

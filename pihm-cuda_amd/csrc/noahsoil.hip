@@ -391,6 +391,22 @@ struct pihm_noah_soil {
   int started;
 };
 
+namespace pihm {
+// The soil column of a soil-water step, for the Noah land-surface step that
+// shares it (noahlsm.hip): device order, layer arrays layer-major.
+NoahSoilView noah_soil_view(pihm_noah_soil *s) {
+  NoahSoilView v;
+  v.ctx = s->ctx;
+  v.ne = s->ne;
+  v.nsoil = s->d.nsoil;
+  v.sldpth = s->d.sldpth;
+  v.smcmax = s->d.smcmax;
+  v.smc = s->d.smc;
+  v.sh2o = s->d.sh2o;
+  return v;
+}
+}  // namespace pihm
+
 namespace {
 
 // caller-order host values of one field -> device order at dst (layers: ML

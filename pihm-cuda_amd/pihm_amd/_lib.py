@@ -112,6 +112,24 @@ class Meteo(C.Structure):
     _fields_ = [("nstation", C.c_int32), ("nrec", C.c_int32), ("ftime", c_double_p), ("data", c_double_p)]
 
 
+NOAH_LSM_ARRAYS = ["shdfac", "cmcfactr", "laimin", "laimax", "emissmin", "emissmax", "albedomin", "albedomax",
+                   "z0min", "z0max", "snup", "cfactr", "rgl", "rsmin", "rsmax", "topt", "hs", "smcref", "smcwlt",
+                   "smcdry", "quartz", "zlvl_wind"]
+NOAH_LSM_SCALARS = ["csoil", "sbeta", "salp", "fxexp", "czil", "lvcoef", "zbot", "tbot", "snoalb", "zlvl"]
+PIHM_NOAH_NSCALAR = 10
+PIHM_NOAH_NFLUX = 16
+NOAH_FLUX_NAMES = ["edir", "ett", "ec", "pcpdrp", "drip", "esnow", "etp", "snomlt", "ssoil", "sheat", "eta",
+                   "fdown", "sncovr", "beta", "rc", "pc"]
+NOAH_STATE_NAMES = ["t1", "sneqv", "snowh", "sndens", "cmc", "ch", "cm", "snotime1", "albedo", "emissi"]
+
+
+class NoahLsmDesc(C.Structure):
+    """pihm_noah_lsm_desc (include/pihm_noahlsm.h)."""
+
+    _fields_ = [("nroot", c_int32_p), ("station", c_int32_p)] + [(n, c_double_p) for n in NOAH_LSM_ARRAYS] + \
+        [(n, C.c_double) for n in NOAH_LSM_SCALARS]
+
+
 RhsFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p)
 PrecSetupFn = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_double,
                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
@@ -166,6 +184,15 @@ _PROTOS = [
     ("pihm_stream_destroy", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_step_inputs", C.c_int, [C.c_void_p, C.POINTER(StepInputs)]),
     ("pihm_ctx_set_noah", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_noah_lsm_create", C.c_int, [C.c_void_p, C.POINTER(NoahLsmDesc), C.POINTER(Meteo), C.POINTER(C.c_void_p)]),
+    ("pihm_noah_lsm_destroy", None, [C.c_void_p]),
+    ("pihm_noah_lsm_set_lai", C.c_int, [C.c_void_p, c_double_p]),
+    ("pihm_noah_lsm_set_state", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("pihm_noah_lsm_get_state", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    ("pihm_noah_lsm_step", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    ("pihm_noah_lsm_get_fluxes", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    ("pihm_noah_lsm_time", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]),
+    ("pihm_run_gpu_flux", C.c_int, [C.c_void_p, C.POINTER(RunConfig), c_double_p, C.POINTER(RunStats)]),
     ("pihm_noah_soil_create", C.c_int, [C.c_void_p, C.POINTER(NoahSoilDesc), C.POINTER(C.c_void_p)]),
     ("pihm_noah_soil_destroy", None, [C.c_void_p]),
     ("pihm_noah_soil_set_state", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p]),

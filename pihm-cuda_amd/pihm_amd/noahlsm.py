@@ -116,13 +116,108 @@ def synthetic_noah_meteo(nstation, hours, seed=0, p_wet=0.1, mean_mm_h=5.0, t_me
     return ftime, out
 
 
-def interp_meteo(ftime, data, t):
-    """IntrplForc (src/forcing.c:482-530) at t for every station: [nstation, MET_NVAR]."""
-    out = np.zeros((data.shape[0], MET_NVAR))
-    for s in range(data.shape[0]):
-        tt = ftime[s]
-        k = int(np.searchsorted(tt, t, side="right")) - 1
-        k = min(max(k, 0), len(tt) - 2)
-        ta, tb = tt[k], tt[k + 1]
-        out[s] = ((tb - t) * data[s, k] + (t - ta) * data[s, k + 1]) / (tb - ta)
-    return out
+def _dp(a):
+    return a.ctypes.data_as(_lib.c_double_p)
+
+
+class HipNoahLSM:
+    """The Noah step (pihm_noah_lsm_*) on the soil column of a ``HipNoahSoil``, whose
+    context has its Noah branches on.  ``p`` is a dict from ``synthetic_noah`` (or the
+    same fields from the reference's tables); ``ftime``/``data`` the station records."""
+
+    def __init__(self, soil, p, ftime, data):
+        self.L = _lib.lib()
+        self.soil = soil
+        self.ne = soil.ne
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self._keep = {k: f(p[k]) for k in _lib.NOAH_LSM_ARRAYS}
+        self._keep["nroot"], self._keep["station"] = i(p["nroot"]), i(p["station"])
+        d = _lib.NoahLsmDesc()
+        d.nroot = self._keep["nroot"].ctypes.data_as(_lib.c_int32_p)
+        d.station = self._keep["station"].ctypes.data_as(_lib.c_int32_p)
+        for k in _lib.NOAH_LSM_ARRAYS:
+            setattr(d, k, _dp(self._keep[k]))
+        for k in _lib.NOAH_LSM_SCALARS:
+            setattr(d, k, float(p[k]))
+        self._ft, self._data = f(ftime), f(data)
+        m = _lib.Meteo(self._data.shape[0], self._data.shape[1], _dp(self._ft), _dp(self._data))
+        h = C.c_void_p()
+        rc = self.L.pihm_noah_lsm_create(soil.h, C.byref(d), C.byref(m), C.byref(h))
+        if rc:
+            raise RuntimeError(f"pihm_noah_lsm_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pihm_noah_lsm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_lai(self, lai):
+        lai = np.ascontiguousarray(lai, dtype=np.float64)
+        if self.L.pihm_noah_lsm_set_lai(self.h, _dp(lai)):
+            raise RuntimeError("pihm_noah_lsm_set_lai failed")
+
+    def set_state(self, stc=None, t1=None, sneqv=None, snowh=None, cmc=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        arrs = [f(stc), f(t1), f(sneqv), f(snowh), f(cmc)]
+        if self.L.pihm_noah_lsm_set_state(self.h, *[None if a is None else _dp(a) for a in arrs]):
+            raise RuntimeError("pihm_noah_lsm_set_state failed")
+
+    def get_state(self):
+        stc = np.empty((self.ne, MAXLYR))
+        sc = np.empty((self.ne, _lib.PIHM_NOAH_NSCALAR))
+        if self.L.pihm_noah_lsm_get_state(self.h, _dp(stc), _dp(sc)):
+            raise RuntimeError("pihm_noah_lsm_get_state failed")
+        out = {k: sc[:, j].copy() for j, k in enumerate(_lib.NOAH_STATE_NAMES)}
+        out["stc"] = stc
+        return out
+
+    def step(self, t, dt):
+        rc = self.L.pihm_noah_lsm_step(self.h, float(t), float(dt))
+        if rc:
+            raise RuntimeError(f"pihm_noah_lsm_step failed ({rc})")
+
+    def fluxes(self):
+        fx = np.empty((self.ne, _lib.PIHM_NOAH_NFLUX))
+        et = np.empty((self.ne, MAXLYR))
+        if self.L.pihm_noah_lsm_get_fluxes(self.h, _dp(fx), _dp(et)):
+            raise RuntimeError("pihm_noah_lsm_get_fluxes failed")
+        out = {k: fx[:, j].copy() for j, k in enumerate(_lib.NOAH_FLUX_NAMES)}
+        out["et"] = et
+        return out
+
+    def time(self, t, dt, iters=20):
+        ms = C.c_double()
+        if self.L.pihm_noah_lsm_time(self.h, float(t), float(dt), int(iters), C.byref(ms)):
+            raise RuntimeError("pihm_noah_lsm_time failed")
+        return ms.value
+
+
+def flux_pihm_setup(mesh, p, state, ftime, data, lai):
+    """A single-solve context with the _NOAH_ branches, the soil-water step and the Noah
+    step over it, in their initial state.  Returns (rhs, soil, lsm)."""
+    from .noahsoil import HipNoahSoil
+    from .rhs import HipRHS
+    h = HipRHS(mesh, nan_check=False)
+    h.set_noah(p["smcmin"])
+    soil = HipNoahSoil(h, p["nsoil"], p["sldpth"], p["smcmax"], p["frzx"])
+    soil.set_state(state["smc"], state["sh2o"])
+    lsm = HipNoahLSM(soil, p, ftime, data)
+    lsm.set_lai(lai)
+    lsm.set_state(stc=state["stc"], t1=state["t1"], sneqv=state["sneqv"], snowh=state["snowh"], cmc=state["cmc"])
+    return h, soil, lsm
+
+
+def run_gpu_flux(lsm, cfg, y0):
+    """The Flux-PIHM model-step loop on the device (pihm_run_gpu_flux). Returns (rc, y, stats)."""
+    y = np.ascontiguousarray(y0, dtype=np.float64).copy()
+    st = _lib.RunStats()
+    rc = lsm.L.pihm_run_gpu_flux(lsm.h, C.byref(cfg), _dp(y), C.byref(st))
+    return rc, y, st

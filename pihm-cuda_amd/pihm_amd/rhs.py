@@ -70,6 +70,15 @@ class HipRHS:
         if rc:
             raise RuntimeError(f"set_step_inputs failed ({rc})")
 
+    def get_step_inputs(self):
+        """(pcpdrp, edir, ett, ws0_surf, elem_bc [nelem, 3], river_bc) as the context holds
+        them (pihm_ctx_get_step_inputs), caller order."""
+        ne, nr = self.mesh.nelem, self.mesh.nriver
+        out = [np.empty(ne) for _ in range(4)] + [np.empty(3 * ne), np.empty(max(nr, 1))]
+        if self.L.pihm_ctx_get_step_inputs(self.ctx, *[_dp(a) for a in out]):
+            raise RuntimeError("pihm_ctx_get_step_inputs failed")
+        return out[0], out[1], out[2], out[3], out[4].reshape(ne, 3), out[5][:nr]
+
     def set_noah(self, smcmin):
         """Switch on the reference's _NOAH_ branches of EtExtract and Infil with each
         element's smcmin (pihm_ctx_set_noah); None switches them off."""

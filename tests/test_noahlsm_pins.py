@@ -372,3 +372,37 @@ def test_soil_column_heat_budget(cold):
     np.testing.assert_array_less(np.abs(heat / dt - (f_top - f_bot + latent / dt))[sel], 1e-8 * scale[sel])
     if cold:
         assert np.any(latent != 0.0)
+
+
+@pytest.mark.parametrize("warming", [False, True])
+def test_phase_change_follows_the_heat_flow(warming):
+    """The soil's frozen water moves towards equilibrium in the direction the heat flows:
+    a column just below freezing under cold, dark, windy air loses heat at the top and its
+    top layer's liquid water freezes, never thaws; a partly frozen column at equilibrium
+    under warm sunshine gains heat and its top layer's ice thaws, never grows.  (The heat
+    budget closes whatever the direction, since the latent term is taken from the water
+    that changed phase.)"""
+    rng = np.random.default_rng(31 + warming)
+    ne = 200
+    par, st, f = _elements(ne, rng, cold=True)
+    frh2o = O.noah_fn("oracle_FrH2O")
+    t0 = TFREEZ - (0.3 if warming else 0.5)
+    st["stc"] = t0
+    st["t1"] = t0
+    if warming:      # liquid water at the supercooled-water equilibrium of t0
+        for i in range(ne):
+            for k in range(MAXLYR):
+                st["sh2o"][i, k] = frh2o(t0, st["smc"][i, k], st["smc"][i, k], par["smcmax"][i],
+                                         par["smcmin"][i], par["alpha"][i], par["beta"][i])
+        f["sfctmp"], f["soldn"], f["longwave"] = TFREEZ + 20.0, 800.0, 360.0
+    else:
+        f["sfctmp"], f["soldn"], f["longwave"] = TFREEZ - 25.0, 0.0, 180.0
+    f["prcp"] = 0.0
+    f["sfcspd"] = 6.0
+    sh2o0 = st["sh2o"][:, 0].copy()
+    O.noah_lsm_step(900.0, par, st, f)
+    d = st["sh2o"][:, 0] - sh2o0
+    if warming:
+        assert np.all(d >= 0.0) and np.mean(d > 0.0) > 0.9
+    else:
+        assert np.all(d <= 0.0) and np.mean(d < 0.0) > 0.9
