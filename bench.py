@@ -525,6 +525,7 @@ def profile_kernels(iters, seed):
     evaluations at C1 on a mid-storm state (then `iters` more timed with HIP events), then
     `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB read and 512 MiB
     written per launch), the calibration kernel for the byte counters.  Between the two,
+    the first simulated quarter hour at C1 with Krylov iterations replayed from HIP graphs,
     the producers at C1, the plain and _NOAH_ RHS at C2 (100,000 triangles) and the Noah
     step at C2."""
     L = _lib.lib()
@@ -545,6 +546,12 @@ def profile_kernels(iters, seed):
     ms_eval = C.c_double()
     L.pihm_ctx_time_rhs(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), iters,
                         C.byref(ms_eval))
+    # the integrator with its Krylov iterations replayed from HIP graphs: the first quarter hour at C1
+    from pihm_amd.rhs import run_gpu
+    _log("profile: C1 first quarter hour with graph replay")
+    rc_run, _, st_run = run_gpu(mesh, StormForcing(2, seed=11, nelem=mesh.nelem), run_config(900.0, graphs=1),
+                                mesh.initial_state())
+    _log(f"profile: run rc {rc_run}, {st_run.nst} steps")
     h, lsm, bc, _ = _producers(mesh, seed)
     ms_l, ms_b = C.c_double(), C.c_double()
     L.pihm_lsm_time(lsm.h, 3600.0 * 12.25, 900.0, h.y, iters, C.byref(ms_l))
@@ -566,7 +573,8 @@ def profile_kernels(iters, seed):
     for v in (y, yd, a, b):
         L.N_VDestroy_Hip(v)
     L.pihm_ctx_destroy(ctx)
-    print(json.dumps({"profile_kernels": iters, "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
+    print(json.dumps({"profile_kernels": iters, "graph_run": {"rc": rc_run, "nst": st_run.nst, "nli": st_run.nli},
+                      "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
                       "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8,
                       "lsm_ms_per_step_events": ms_l.value, "bc_ms_per_apply_events": ms_b.value,
                       "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem,

@@ -64,6 +64,7 @@ struct Mem {
   // stream and an RHS declared capturable, PihmCVodeSetGraphRhs)
   bool graph_rhs = false, graph_off = false;
   hipGraphExec_t gx[MAXL_CAP] = {};
+  hipStream_t gx_stream = nullptr;   // the stream the executable graphs are launched on
   double *kry = nullptr;        // device: copies of the Krylov sums and gamma
 
   N_Vector zn[6] = {};
@@ -124,11 +125,18 @@ int psolve(Mem *m, N_Vector r, N_Vector z) {
 // Newton's retry with fresh preconditioner data (cvode.c:2760-2830)
 inline int retry_or(const Mem *m, int flag) { return (!m->jcur && setup_nonnull(m)) ? TRY_AGAIN : flag; }
 
+// A replay's last kernel hands its sums to the host before the runtime has
+// retired the launch, so the host can reach this point with launches of these
+// executable graphs still in flight: wait for their stream first.
 void free_graphs(Mem *m) {
+  bool any = false;
+  for (auto g : m->gx) any = any || g;
+  if (any && m->gx_stream) (void)hipStreamSynchronize(m->gx_stream);
   for (auto &g : m->gx) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   }
+  m->gx_stream = nullptr;
 }
 
 void free_vectors(Mem *m) {
@@ -443,6 +451,7 @@ bool build_graph(Mem *m, int l) {
   const hipError_t ei = hipGraphInstantiate(&m->gx[l], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ei != hipSuccess) { m->gx[l] = nullptr; m->graph_off = true; return false; }
+  m->gx_stream = st;
   return true;
 }
 
@@ -463,6 +472,8 @@ int spgmr_fused(Mem *m, double sumsq, double delta, int *krydim_out, int force) 
   bool converged = false;
   int krydim = 0;
   bool graph = m->graph_rhs && !m->graph_off && m->kry;   // gamma was left in kry by pihm_gm_start
+  // graphs captured on another stream (the vectors' stream was changed) are rebuilt
+  if (graph && m->gx_stream && m->gx_stream != (hipStream_t)pihm_nv_stream(s)) free_graphs(m);
   const bool pre = pre_right(m);
   for (int l = 0; l < m->maxl; l++) {
     krydim = l + 1;

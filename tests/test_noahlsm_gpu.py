@@ -88,6 +88,33 @@ def test_noah_step_parity_chained(which, t_mean_c, request):
         h.close()
 
 
+def test_noah_step_parity_c2_mesh():
+    """The 100,000-triangle Flux-PIHM mesh with 11 soil layers: a noon step and a night
+    step, every flux and carried state within 1e-9 of the oracle's."""
+    from pihm_amd.meshgen import config_mesh
+    m = config_mesh("c2", seed=1)
+    print(f"[noah lsm] c2 ne={m.nelem}", flush=True)
+    p, state, ft, data, lai = _setup(m, seed=3, t_mean_c=6.0)
+    h, soil, lsm = flux_pihm_setup(m, p, state, ft, data, lai)
+    par, st = O.noah_arrays(m, p, state)
+    try:
+        for k, t in enumerate((12 * 3600.0, 23.5 * 3600.0)):
+            lsm.step(t, 900.0)
+            dev, dst, dsoil = lsm.fluxes(), lsm.get_state(), soil.get_state()
+            out = O.noah_lsm_step(900.0, par, st, O.noah_forcing(O.noah_meteo_interp(ft, data, t), p["station"], lai))
+            for key in FLUXES + ["et"]:
+                _close(dev[key], out[key], f"c2 step {k} {key}")
+            for key in ("t1", "cmc", "ch", "cm", "albedo", "emissi"):
+                _close(dst[key], st[key], f"c2 step {k} {key}")
+            _close(dst["stc"][:, :11], st["stc"], f"c2 step {k} stc")
+            _close(dsoil["smc"], st["smc"], f"c2 step {k} smc")
+            _close(dsoil["sh2o"], st["sh2o"], f"c2 step {k} sh2o")
+    finally:
+        lsm.close()
+        soil.close()
+        h.close()
+
+
 def test_noah_step_drives_the_rhs(c0):
     """After a Noah step the RHS (with the _NOAH_ branches) reads the step's pcpdrp, edir,
     ett, layer transpiration and top-layer water: equal to the oracle's RHS given the
