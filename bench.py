@@ -30,11 +30,15 @@ GRAPHS = 1               # replay Krylov iterations as HIP graphs (off with --no
 BYTES_PER_ELEM = 344     # algorithmic bytes of one RHS per element
 BYTES_PER_SEG = 200      # ... per river segment
 # HBM bytes one RHS evaluation at the 200,000-triangle workload moves, from the rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (summed over
-# k_prep and k_flux, KiB per dispatch), each scaled by what the counter reads for the
-# calibration copy in the same run: FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288
-# KiB copied), WRITE_SIZE x 1.000.  (44,752.5 x 2 + 17,737.5) KiB.
-RHS_TRAFFIC_BYTES = (44752.5 * 2.0 + 17737.5) * 1024
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (KiB per dispatch),
+# each scaled by what the counter reads for the calibration copy in the same run:
+# FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288 KiB copied), WRITE_SIZE x 1.000.
+# One launch (k_rhs_tile): 37,698.5 x 2 + 4,825.7 KiB.  Two kernels (k_prep + k_flux):
+# (14,710.5 + 28,244.1) x 2 + (12,919.4 + 4,818.1) KiB.
+RHS_TRAFFIC_BYTES = {"one launch": (37698.5 * 2.0 + 4825.7) * 1024,
+                     "two kernels": ((14710.5 + 28244.1) * 2.0 + (12919.4 + 4818.1)) * 1024}
+RHS_KERNELS = {"one launch": "one RHS evaluation (k_rhs_tile: one launch, tile records in LDS)",
+               "two kernels": "one RHS evaluation (k_prep + k_flux, back to back)"}
 # Land-surface step (k_intcp_snow_et), algorithmic bytes per element: 15 FP64 parameters
 # (albedo, shdfac, cmcfactr, cfactr, rzd, rgl, rsmin, rsmax, topt, depth, porosity, smcmin,
 # smcwlt, smcref, surface pressure: 120) + station index (4) + lai, the aerodynamic
@@ -85,6 +89,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     span, span_ms = C.c_int(), np.zeros(len(_lib.PIHM_TUNE_SPANS))
     assert L.pihm_ctx_autotune(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), 20,
                                C.byref(span), span_ms.ctypes.data_as(_lib.c_double_p)) == 0
+    rhs_path = "one launch" if L.pihm_ctx_get_rhs_path(ctx) == _lib.PIHM_RHS_ONE_LAUNCH else "two kernels"
     marks = {}
     want = {int(round(h * 60)) for h in mark_hours}     # in model steps of 60 s
 
@@ -121,6 +126,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t2],
                 ms_eval=ms_eval.value, yend=yend,
                 tuning={"tile_span": span.value,
+                        "rhs_path": rhs_path,
                         "rhs_ms_by_span": dict(zip(_lib.PIHM_TUNE_SPANS, span_ms.round(6).tolist()))})
 
 
@@ -522,7 +528,8 @@ def cpu_run(mesh, forcing, hours, nthreads):
 
 def profile_kernels(iters, seed):
     """Short fixed workload for kernel traces and hardware-counter passes: `iters` RHS
-    evaluations at C1 on a mid-storm state (then `iters` more timed with HIP events), then
+    evaluations at C1 on a mid-storm state on the one-launch path and `iters` on the
+    two-kernel path (each followed by `iters` more timed with HIP events), then
     `iters` copies of a 64 Mi-entry vector (N_VScale(1, x, z): 512 MiB read and 512 MiB
     written per launch), the calibration kernel for the byte counters.  Between the two,
     the first simulated quarter hour at C1 with Krylov iterations replayed from HIP graphs,
@@ -540,12 +547,19 @@ def profile_kernels(iters, seed):
     y = L.N_VNew_Hip(mesh.neq)
     yd = L.N_VNew_Hip(mesh.neq)
     assert L.pihm_ctx_state_to_device(ctx, y0.ctypes.data_as(_lib.c_double_p), y) == 0
-    for _ in range(iters):
-        L.PIHM_ODE_Hip(0.0, y, yd, ctx)
-    L.pihm_ctx_sync(ctx)
-    ms_eval = C.c_double()
-    L.pihm_ctx_time_rhs(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), iters,
-                        C.byref(ms_eval))
+    # the one-launch RHS (the default), then the two-kernel RHS, each iters times and timed
+    ms_paths = {}
+    for path in (_lib.PIHM_RHS_ONE_LAUNCH, _lib.PIHM_RHS_TWO_KERNELS):
+        assert L.pihm_ctx_set_rhs_path(ctx, path) == 0
+        for _ in range(iters):
+            L.PIHM_ODE_Hip(0.0, y, yd, ctx)
+        L.pihm_ctx_sync(ctx)
+        ms = C.c_double()
+        L.pihm_ctx_time_rhs(ctx, L.N_VGetDeviceArrayPointer_Hip(y), L.N_VGetDeviceArrayPointer_Hip(yd), iters,
+                            C.byref(ms))
+        ms_paths[path] = ms.value
+    assert L.pihm_ctx_set_rhs_path(ctx, _lib.PIHM_RHS_ONE_LAUNCH) == 0
+    ms_eval = C.c_double(ms_paths[_lib.PIHM_RHS_ONE_LAUNCH])
     # the integrator with its Krylov iterations replayed from HIP graphs: the first quarter hour at C1
     from pihm_amd.rhs import run_gpu
     _log("profile: C1 first quarter hour with graph replay")
@@ -575,7 +589,9 @@ def profile_kernels(iters, seed):
     L.pihm_ctx_destroy(ctx)
     print(json.dumps({"profile_kernels": iters, "graph_run": {"rc": rc_run, "nst": st_run.nst, "nli": st_run.nli},
                       "rhs_bytes_algorithmic": BYTES_PER_ELEM * mesh.nelem + BYTES_PER_SEG * mesh.nriver,
-                      "rhs_ms_per_eval_events": ms_eval.value, "copy_bytes": 2 * n * 8,
+                      "rhs_ms_per_eval_events": ms_eval.value,
+                      "rhs_two_kernel_ms_per_eval_events": ms_paths[_lib.PIHM_RHS_TWO_KERNELS],
+                      "copy_bytes": 2 * n * 8,
                       "lsm_ms_per_step_events": ms_l.value, "bc_ms_per_apply_events": ms_b.value,
                       "lsm_bytes_algorithmic": LSM_BYTES_PER_ELEM * mesh.nelem,
                       "noah_rhs_c2_ms_per_eval_events": noah["ms_per_eval"],
@@ -664,10 +680,10 @@ def main():
                    "nelem": mesh.nelem, "nriver": mesh.nriver, "neq": mesh.neq},
         "solver": {k: st[k] for k in ("nst", "nfe", "nfes", "nni", "nli", "ncfn", "netf")},
         "rhs": {"ms_per_eval": rhs_ms, "evals_per_s": 1e3 / rhs_ms,
-                "ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}, "tuning": g["tuning"]},
-        "roofline": {"kernel": "one RHS evaluation (k_prep + k_flux, back to back)",
+                "two_kernel_ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}, "tuning": g["tuning"]},
+        "roofline": {"kernel": RHS_KERNELS[g["tuning"]["rhs_path"]],
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": RHS_TRAFFIC_BYTES,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": RHS_TRAFFIC_BYTES[g["tuning"]["rhs_path"]],
                      "algorithmic_bytes": alg_bytes},
     }
     if not args.no_cpu:

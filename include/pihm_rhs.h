@@ -200,27 +200,43 @@ int  pihm_ctx_set_closure(pihm_ctx *ctx, int on);
 /* Wait for all work queued on the context's stream. */
 int  pihm_ctx_sync(pihm_ctx *ctx);
 
-/* Measurement: average device time [ms] of each of the two RHS kernels
- * (neighbour records + slopes; all fluxes and the derivative) over `iters`
- * evaluations at state y_dev, each between its own pair of HIP events on the
- * context's stream. */
+/* How a single solve evaluates the RHS.  PIHM_RHS_ONE_LAUNCH (the default for
+ * single-solve contexts): one kernel, each block staging the records of a
+ * tile of elements and its halo in LDS.  PIHM_RHS_TWO_KERNELS: the
+ * neighbour-record kernel, then the flux kernel (batches always use it).  Both
+ * give the same bits.  pihm_ctx_set_rhs_path returns PIHM_ERR_ARG for a batch
+ * context, or when the one-launch tables could not be built (a halo larger
+ * than the LDS budget).  No counterpart in the reference. */
+#define PIHM_RHS_ONE_LAUNCH 0
+#define PIHM_RHS_TWO_KERNELS 1
+int  pihm_ctx_set_rhs_path(pihm_ctx *ctx, int path);
+int  pihm_ctx_get_rhs_path(const pihm_ctx *ctx);
+
+/* Measurement: average device time [ms] of each of the two kernels of the
+ * two-kernel RHS (neighbour records + slopes; all fluxes and the derivative)
+ * over `iters` evaluations at state y_dev, each between its own pair of HIP
+ * events on the context's stream, whichever path the context uses. */
 int  pihm_ctx_time_kernels(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters,
                            double *ms_prep, double *ms_flux);
 
-/* Measurement: average device time [ms] of one whole RHS evaluation (both
- * kernels issued back to back, as an integrator issues them) over `iters`
- * evaluations at state y_dev: one HIP event before the first launch and one
- * after the last, on the context's stream. */
+/* Measurement: average device time [ms] of one whole RHS evaluation on the
+ * context's path (its launches issued back to back, as an integrator issues
+ * them) over `iters` evaluations at state y_dev: one HIP event before the
+ * first launch and one after the last, on the context's stream. */
 int  pihm_ctx_time_rhs(pihm_ctx *ctx, const double *y_dev, double *ydot_dev, int iters, double *ms_per_eval);
 
 /* Placement of the element tiles over the XCDs: 0 gives each XCD one
  * contiguous range of Hilbert-ordered tiles; s >= 1 deals groups of s
  * consecutive tiles to the XCDs in turn.  Results do not depend on it.
  * pihm_ctx_autotune (single-solve contexts) times reps back-to-back RHS
- * evaluations at y_dev for each candidate of PIHM_TUNE_SPANS, keeps the
- * fastest and returns it (and, when ms is not NULL, the ms per evaluation of
- * each candidate).  The carried river-edge flux is restored afterwards, so the
- * next call reads what it would have read.  No counterpart in the reference. */
+ * evaluations at y_dev for each placement of PIHM_TUNE_SPANS on each RHS path
+ * the context has, three interleaved rounds, each candidate's best round
+ * counting.  It keeps the default (the one-launch path where available,
+ * placement 0) unless another candidate is more than 2% faster, sets the
+ * chosen path and placement on the context and returns the placement (and,
+ * when ms is not NULL, each placement's best ms per evaluation over the
+ * paths).  The carried river-edge flux is restored afterwards, so the next
+ * call reads what it would have read.  No counterpart in the reference. */
 #define PIHM_TUNE_NSPAN 8
 #define PIHM_TUNE_SPANS {0, 1, 2, 4, 8, 16, 32, 64}
 int  pihm_ctx_set_tile_span(pihm_ctx *ctx, int span);

@@ -71,6 +71,21 @@ struct RiverDev {
   double *bank_sub;     // [2*nr] this call's river-edge subsurface flux, by slot
 };
 
+// Element tiles of the one-launch RHS of a single solve (k_rhs_tile): tile t is
+// the kTile consecutive device elements from t * kTile; its halo lists the
+// outside neighbours of the tile's elements (first ring), then their outside
+// neighbours not listed yet, each part in ascending device index.  In LDS,
+// slot s < kTile is tile element s and the halo follows in list order.
+struct TileArgs {
+  const int32_t *hbeg;   // [ntile + 1] each tile's range of halo entries
+  const int32_t *hn1;    // [ntile] first-ring entries at the front of the range
+  const int32_t *hidx;   // device element of each halo entry
+  const int32_t *hnb;    // [3 * entries] neighbour codes of the entries, elements as LDS slots
+                         // (first ring only; the second ring's are not read)
+  const int32_t *lnb;    // [3 * ne] (j * ne + p) the same for the tile elements
+  int s1cap, scap;       // LDS records (tile + first ring) and heads (tile + halo) per block
+};
+
 struct RecordDev {      // optional flux recording (device order)
   double *ovl, *sub;    // [3*ne]
   double *infil, *rechg;
@@ -197,6 +212,11 @@ struct pihm_ctx {
   double *stage_host;            // scratch for host-side permutations
   int noah;                      // RHS with the reference's _NOAH_ branches
   void *noah_arena;              // the Noah fields of ElemDev
+  int rhs_path;                  // PIHM_RHS_ONE_LAUNCH or PIHM_RHS_TWO_KERNELS (single solves)
+  int one_launch_ok;             // tile tables built (single solve, halos fit the LDS budget)
+  pihm::TileArgs tiles;
+  size_t tile_lds;               // dynamic LDS bytes per element block
+  void *tile_arena;
 };
 
 // Caller-order index of device element p / device river r (the loaded slots

@@ -201,146 +201,329 @@ __global__ __launch_bounds__(kBlock) void k_prep(RhsArgs a, int nbe) {
   rp[2] = make_double2(effkh, sfmag);
 }
 
-// N: the reference built with _NOAH_ (src/hydrol.c:43-56,79-84;
-// src/vert_flow.c:16-19,94-98), single solves only.
-template <bool B, bool N = false>
-__device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
-  const int ne = a.ne;
-  const ElemDev &e = a.e;
-  const Ix ix = ix_elem<B>(a, p);
-  if (!ix.active) return;
-  const int lp = p - ix.eoff;
-  if (B && lp >= ix.ne) return;     // unused capacity of a slot
-  const double *y = ix.y;
-  double *ydot = ix.ydot;
+// What a river segment reads of one bank element: its record fields, evaluated
+// from the element's state with the same expressions as the record's.
+struct BankView {
+  double surfh, gw, gwz, effkh, zmax, zmin;
+};
 
-  // Loads first, in two groups as in k_prep: the element's own entries and
-  // edge codes, then the neighbours' records, boundary values and carried
-  // river-edge fluxes the codes point at.
-  const NbrRec me = e.rec[p];
-  const double yu = y[ix.ne + lp];
-  const double area = e.area[p], zmax = e.zmax[p], zmin = e.zmin[p];
-  const double depth = e.depth[p], dinf = e.dinf[p], rzd = e.rzd[p];
-  const double edir = e.edir[p], ett = e.ett[p], pcpdrp = e.pcpdrp[p], ws0 = e.ws0s[p];
-  const double alpha = e.alpha[p], beta = e.beta[p], kinfv = e.kinfv[p], kmacv = e.kmacv[p];
-  const double areafh = e.areafh[p], dmac = e.dmac[p], ksatv = e.ksatv[p], porosity = e.porosity[p];
+// RiverToElem for one bank (src/river_flow.c:113-183): overland (rivflow 2/3),
+// channel exchange (4/5) and aquifer exchange (7/8) between segment r and the
+// bank element bv on `side` (0 left, 1 right); effk_riv is the mean EffKh of
+// the two banks.  Evaluated by the segment and by the bank element, from the
+// same operands, so both get the same bits.
+struct BankFlux {
+  double ovl, chan, sub;
+};
+__device__ __forceinline__ BankFlux bank_flux(const RiverDev &R, int r, int side, double stage, double rgw,
+                                              const BankView &bv, double effk_riv) {
+  const double zbed = R.zbed[r], length = R.length[r];
+  const double dist = side ? R.dist_r[r] : R.dist_l[r];
+  BankFlux f;
+  f.ovl = ovl_elem_to_river(R.zmax[r], bv.zmax, bv.surfh, zbed, stage, R.cwr[r], length);
+  f.chan = chan_elem_to_river(stage, zbed, length, R.ksath[r], bv.gw, bv.gwz, bv.zmin, bv.effkh, dist);
+  f.sub = sub_elem_to_river(rgw, R.zmin[r], zbed, length, bv.gw, bv.gwz, bv.zmin, bv.effkh, effk_riv, dist);
+  return f;
+}
+
+template <bool B>
+__device__ __forceinline__ double effkh_of(const RhsArgs &a, const Ix &ix, int q) {
+  const ElemDev &e = a.e;
+  const double g = clamp0(ix.y[2 * ix.ne + (q - ix.eoff)]);
+  return eff_kh(e.depth[q], e.dmac[q], e.kmach[q], e.areafv[q], e.ksath[q], g);
+}
+
+// Where an element's per-call neighbour values come from.
+//   GlobalSrc: the records k_prep wrote (codes: device element indices) and
+//              the bank fluxes its river blocks left.
+//   TileSrc:   the tile's records in LDS (codes: LDS slots) and the bank
+//              fluxes evaluated here, from the element's own record.
+struct GlobalSrc {
+  const RhsArgs &a;
+  int p;
+  __device__ __forceinline__ NbrRec self() const { return a.e.rec[p]; }
+  __device__ __forceinline__ int code(int j) const { return a.e.nb[j * a.ne + p]; }
+  __device__ __forceinline__ NbrRec nbr(int c) const { return a.e.rec[c]; }
+  __device__ __forceinline__ void bank(const Ix &, int slot, const NbrRec &, double, double, double &ovl,
+                                       double &sub) const {
+    ovl = a.r.hid_ovl[2 * a.nr + slot];
+    sub = a.r.bank_sub[slot];
+  }
+};
+
+// The tile's records in LDS, one array of cap doubles per NbrRec field (slots
+// 0 .. tile size - 1: the tile's elements; then the halo), so that lanes
+// touching consecutive slots touch consecutive banks.
+struct LdsRecs {
+  double *f;
+  int cap;
+  __device__ __forceinline__ NbrRec get(int s) const {
+    NbrRec r;
+    r.surfh = f[s]; r.head = f[cap + s]; r.gw = f[2 * cap + s]; r.gwz = f[3 * cap + s];
+    r.effkh = f[4 * cap + s]; r.sfmag = f[5 * cap + s]; r.rough = f[6 * cap + s]; r.zmax = f[7 * cap + s];
+    return r;
+  }
+  __device__ __forceinline__ void put(int s, const NbrRec &r) const {
+    f[s] = r.surfh; f[cap + s] = r.head; f[2 * cap + s] = r.gw; f[3 * cap + s] = r.gwz;
+    f[4 * cap + s] = r.effkh; f[5 * cap + s] = r.sfmag; f[6 * cap + s] = r.rough; f[7 * cap + s] = r.zmax;
+  }
+  __device__ __forceinline__ void put_sfmag(int s, double v) const { f[5 * cap + s] = v; }
+};
+
+struct TileSrc {
+  const RhsArgs &a;
+  LdsRecs srec;         // LDS records of the tile and its halo
+  const int32_t *lnb;   // neighbour slots of the tile's elements
+  int p, i;
+  __device__ __forceinline__ NbrRec self() const { return srec.get(i); }
+  __device__ __forceinline__ int code(int j) const { return lnb[j * a.ne + p]; }
+  __device__ __forceinline__ NbrRec nbr(int c) const { return srec.get(c); }
+  __device__ __forceinline__ void bank(const Ix &ix, int slot, const NbrRec &me, double zmax, double zmin,
+                                       double &ovl, double &sub) const {
+    const RiverDev &R = a.r;
+    const int r = slot >> 1, side = slot & 1;
+    const double *ys = ix.y + 3L * ix.ne - ix.roff;
+    const double stage = clamp0(ys[r]);
+    const double rgw = clamp0(ys[ix.nr + r]);
+    const double other = effkh_of<false>(a, ix, side ? R.left[r] : R.right[r]);
+    const BankView bv{me.surfh, me.gw, me.gwz, me.effkh, zmax, zmin};
+    const double effk_riv = side ? 0.5 * (other + me.effkh) : 0.5 * (me.effkh + other);
+    const BankFlux f = bank_flux(R, r, side, stage, rgw, bv, effk_riv);
+    ovl = -f.ovl;
+    sub = -(f.chan + f.sub);
+  }
+};
+
+// An element's own inputs to elem_core: its unsaturated storage, statics and
+// per-step inputs, its edges (codes from the Src, distances, lengths) and,
+// by edge kind, the boundary value or the carried river-edge overland flux.
+struct ElemIn {
+  double yu, area, zmax, zmin, depth, dinf, rzd, edir, ett, pcpdrp, ws0;
+  double alpha, beta, kinfv, kmacv, areafh, dmac, ksatv, porosity;
+  double gwet, sh2o0, smcmin, fcr;   // the _NOAH_ branches only
   int c[3];
   double dist[3], edge[3];
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    c[j] = e.nb[j * ne + p];
-    dist[j] = e.nabrdist[j * ne + p];
-    edge[j] = e.edge[j * ne + p];
-  }
-  NbrRec nrec[3];           // each read only in its own case, as in k_prep
   double bcv[3], hov[3];
+};
+
+// Loads in two groups: the element's own entries and edge codes, then what the
+// codes point at (a wavefront waits for memory twice, not once per edge).
+template <bool N, class Src>
+__device__ __forceinline__ ElemIn load_elem_in(const RhsArgs &a, int p, const Ix &ix, const Src &src) {
+  const int ne = a.ne;
+  const ElemDev &e = a.e;
+  ElemIn in;
+  in.yu = ix.y[ix.ne + (p - ix.eoff)];
+  in.area = e.area[p]; in.zmax = e.zmax[p]; in.zmin = e.zmin[p];
+  in.depth = e.depth[p]; in.dinf = e.dinf[p]; in.rzd = e.rzd[p];
+  in.edir = e.edir[p]; in.ett = e.ett[p]; in.pcpdrp = e.pcpdrp[p]; in.ws0 = e.ws0s[p];
+  in.alpha = e.alpha[p]; in.beta = e.beta[p]; in.kinfv = e.kinfv[p]; in.kmacv = e.kmacv[p];
+  in.areafh = e.areafh[p]; in.dmac = e.dmac[p]; in.ksatv = e.ksatv[p]; in.porosity = e.porosity[p];
+  if (N) {
+    in.gwet = e.gwet[p]; in.sh2o0 = e.sh2o0[p]; in.smcmin = e.smcmin[p]; in.fcr = e.fcr[p];
+  }
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    if (c[j] >= 0) {
-      nrec[j] = e.rec[c[j]];
-    } else if (nb_is_bound(c[j])) {
-      bcv[j] = e.ebc[j * ne + p];
+    in.c[j] = src.code(j);
+    in.dist[j] = e.nabrdist[j * ne + p];
+    in.edge[j] = e.edge[j * ne + p];
+  }
+  // each read only in its own case (a register shared by two cases, or a
+  // default written into it, would make a load wait for another lane's)
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    if (in.c[j] >= 0) {
+    } else if (nb_is_bound(in.c[j])) {
+      in.bcv[j] = e.ebc[j * ne + p];
     } else {
-      hov[j] = a.r.hid_ovl[-c[j] - 1];     // the carried overland flux
+      in.hov[j] = a.r.hid_ovl[-in.c[j] - 1];     // the carried overland flux
     }
   }
+  return in;
+}
 
-  const double u = clamp0(yu);
+// An element's fluxes come in three parts, which the two-kernel RHS runs one
+// after the other in one thread and the one-launch RHS in two wavefronts:
+//   elem_lateral   LateralFlow over the three edges (src/lat_flow.c:29-63),
+//                  with the overland flux carried from the previous call on
+//                  river edges, the sum Infil's application rate starts from,
+//                  then this call's bank fluxes on river edges;
+//   elem_vert_pre  EtExtract's split and the unsaturated zone's saturation,
+//                  pressure head and conductivity;
+//   elem_vert_post Infil, Recharge and the element ODEs.
+// N: the reference built with _NOAH_ (src/hydrol.c:43-56,79-84;
+// src/vert_flow.c:16-19,94-98), single solves only.
+struct LatOut {
+  double ovl[3], sub[3];   // this call's edge fluxes
+  double asum;             // sum of -ovl / area with the carried fluxes on river edges
+  double out;              // ovl + sub over the domain-boundary edges (closure)
+};
+
+template <bool B, class Src>
+__device__ __forceinline__ LatOut elem_lateral(const RhsArgs &a, int p, const Ix &ix, const Src &src,
+                                               const ElemIn &in, const NbrRec &me) {
+  const int ne = a.ne;
+  NbrRec nrec[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if (in.c[j] >= 0) nrec[j] = src.nbr(in.c[j]);
+  const double area = in.area, zmax = in.zmax, zmin = in.zmin;
   const double g = me.gw;
   const double surfh = me.surfh;
-
-  // EtExtract: src/hydrol.c:33-97
-  double edir_s, edir_u, edir_g, ett_u, ett_g;
-  if (N) {
-    // no direct evaporation from ponded water; the saturated zone's share of
-    // transpiration, GwTransp (src/noah/lsm_func.c:163-185), is fixed per step
-    edir_s = 0.0;
-    if (g > depth - dinf) { edir_u = 0.0; edir_g = edir; }
-    else { edir_u = edir; edir_g = 0.0; }
-    const double gwet = e.gwet[p];
-    ett_u = (1.0 - gwet) * ett;
-    ett_g = gwet * ett;
-  } else {
-    if (surfh >= DEPRSTG) { edir_s = edir; edir_u = 0.0; edir_g = 0.0; }
-    else if (g > depth - dinf) { edir_s = 0.0; edir_u = 0.0; edir_g = edir; }
-    else { edir_s = 0.0; edir_u = edir; edir_g = 0.0; }
-    if (g > depth - rzd) { ett_u = 0.0; ett_g = ett; }
-    else { ett_u = ett; ett_g = 0.0; }
-  }
-
-  // LateralFlow: src/lat_flow.c:29-63
-  double ovl[3], sub[3];
+  LatOut o;
+  double *ovl = o.ovl, *sub = o.sub;
   bool bank = false;
   bool bnd[3] = {false, false, false};
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    bnd[j] = nb_is_bound(c[j]);
-    if (c[j] >= 0) {
+    const int c = in.c[j];
+    bnd[j] = nb_is_bound(c);
+    if (c >= 0) {
       const NbrRec &nr = nrec[j];
       // SubFlowElemToElem: src/lat_flow.c:285-309
       {
         const double diff_h = me.gwz - nr.gwz;
         const double avg_h = avg_h_sub(diff_h, g, nr.gw);
-        const double grad_h = diff_h / dist[j];
+        const double grad_h = diff_h / in.dist[j];
         const double avg_ksat = 0.5 * (me.effkh + nr.effkh);
-        sub[j] = avg_ksat * grad_h * avg_h * edge[j];
+        sub[j] = avg_ksat * grad_h * avg_h * in.edge[j];
       }
       // OvlFlowElemToElem: src/lat_flow.c:311-338
       {
         double avg_sf = 0.5 * (me.sfmag + nr.sfmag);
         const double diff_h = (a.surf_mode == KINEMATIC) ? zmax - nr.zmax : me.head - nr.head;
         const double avg_h = avg_hsurf(diff_h, surfh, nr.surfh);
-        const double grad_h = diff_h / dist[j];
+        const double grad_h = diff_h / in.dist[j];
         if (a.surf_mode == KINEMATIC) avg_sf = (grad_h > 0.0) ? grad_h : GRADMIN;
         else avg_sf = (avg_sf > GRADMIN) ? avg_sf : GRADMIN;
         const double avg_rough = 0.5 * (me.rough + nr.rough);
-        const double crossa = avg_h * edge[j];
+        const double crossa = avg_h * in.edge[j];
         ovl[j] = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
       }
-    } else if (nb_is_bound(c[j])) {
+    } else if (nb_is_bound(c)) {
       // BoundFluxElem: src/lat_flow.c:340-382
       ovl[j] = 0.0;
-      const int bt = nb_bc_sign(c[j]);
+      const int bt = nb_bc_sign(c);
       if (bt == 0) {
         sub[j] = 0.0;
       } else if (bt > 0) {
-        const double diff_h = me.gwz - bcv[j];
-        const double avg_h = avg_h_sub(diff_h, g, bcv[j] - zmin);
-        const double grad_h = diff_h / dist[j];
-        sub[j] = me.effkh * grad_h * avg_h * edge[j];
+        const double diff_h = me.gwz - in.bcv[j];
+        const double avg_h = avg_h_sub(diff_h, g, in.bcv[j] - zmin);
+        const double grad_h = diff_h / in.dist[j];
+        sub[j] = me.effkh * grad_h * avg_h * in.edge[j];
       } else {
-        sub[j] = -bcv[j];
+        sub[j] = -in.bcv[j];
       }
     } else {
       // river edge: the overland flux carried from the previous call is what
-      // Infil reads (src/vert_flow.c:59-62); both fluxes are replaced later.
+      // Infil reads (src/vert_flow.c:59-62); both fluxes are replaced below.
       bank = true;
-      ovl[j] = hov[j];
+      ovl[j] = in.hov[j];
       sub[j] = 0.0;
     }
   }
+  // Infil's application rate starts from the lateral inflow (src/vert_flow.c:56-66)
+  o.asum = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; j++) o.asum += -ovl[j] / area;
+  // River edges: this call's bank fluxes (src/river_flow.c:113-183) replace
+  // the carried overland flux that Infil uses, and the new overland flux is
+  // carried to the next call (src/river_flow.c:159-182).
+  if (bank) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (nb_is_river(in.c[j])) {
+        const int slot = -in.c[j] - 1;
+        src.bank(ix, slot, me, zmax, zmin, ovl[j], sub[j]);
+        a.r.hid_ovl[slot] = ovl[j];
+      }
+    }
+  }
+  if (a.record) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      a.rec.ovl[j * ne + p] = ovl[j];
+      a.rec.sub[j * ne + p] = sub[j];
+    }
+  }
+  // volume rates crossing the element's outer boundary
+  o.out = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if (bnd[j]) o.out += ovl[j] + sub[j];
+  return o;
+}
 
+struct VertPre {
+  double edir_s, edir_u, edir_g, ett_u, ett_g;
+  double deficit, satn, psi_raw, kr;
+  bool wet_bottom;
+};
+
+template <bool N>
+__device__ __forceinline__ VertPre elem_vert_pre(const ElemIn &in, const NbrRec &me) {
+  const double depth = in.depth, dinf = in.dinf, rzd = in.rzd;
+  const double edir = in.edir, ett = in.ett;
+  const double u = clamp0(in.yu);
+  const double g = me.gw;
+  const double surfh = me.surfh;
+  VertPre v;
+  // EtExtract: src/hydrol.c:33-97
+  if (N) {
+    // no direct evaporation from ponded water; the saturated zone's share of
+    // transpiration, GwTransp (src/noah/lsm_func.c:163-185), is fixed per step
+    v.edir_s = 0.0;
+    if (g > depth - dinf) { v.edir_u = 0.0; v.edir_g = edir; }
+    else { v.edir_u = edir; v.edir_g = 0.0; }
+    const double gwet = in.gwet;
+    v.ett_u = (1.0 - gwet) * ett;
+    v.ett_g = gwet * ett;
+  } else {
+    if (surfh >= DEPRSTG) { v.edir_s = edir; v.edir_u = 0.0; v.edir_g = 0.0; }
+    else if (g > depth - dinf) { v.edir_s = 0.0; v.edir_u = 0.0; v.edir_g = edir; }
+    else { v.edir_s = 0.0; v.edir_u = edir; v.edir_g = 0.0; }
+    if (g > depth - rzd) { v.ett_u = 0.0; v.ett_g = ett; }
+    else { v.ett_u = ett; v.ett_g = 0.0; }
+  }
   // Unsaturated-zone terms shared by Infil and Recharge (same deficit and
   // saturation in both, src/vert_flow.c:93-102 and :149-156).
-  const bool wet_bottom = g > depth - dinf;
-  double deficit = 0.0, satn = 0.0, psi_raw = 0.0, kr = 0.0;
-  if (!wet_bottom) {
-    deficit = depth - g;
-    satn = u / deficit;
+  v.wet_bottom = g > depth - dinf;
+  v.deficit = 0.0; v.satn = 0.0; v.psi_raw = 0.0; v.kr = 0.0;
+  if (!v.wet_bottom) {
+    v.deficit = depth - g;
+    double satn = u / v.deficit;
     satn = (satn > 1.0) ? 1.0 : satn;
     satn = (satn < SATMIN) ? SATMIN : satn;
-    psi_raw = psi_vg(satn, alpha, beta);
-    kr = kr_func(beta, satn);
+    v.satn = satn;
+    v.psi_raw = psi_vg(satn, in.alpha, in.beta);
+    v.kr = kr_func(in.beta, satn);
   }
+  return v;
+}
+
+template <bool B, bool N>
+__device__ __forceinline__ void elem_vert_post(const RhsArgs &a, int p, const Ix &ix, const ElemIn &in,
+                                               const NbrRec &me, const VertPre &v, const LatOut &lat) {
+  const int ne = a.ne;
+  const int lp = p - ix.eoff;
+  double *ydot = ix.ydot;
+  const double area = in.area, zmax = in.zmax, zmin = in.zmin;
+  const double depth = in.depth, dinf = in.dinf;
+  const double pcpdrp = in.pcpdrp, ws0 = in.ws0;
+  const double beta = in.beta, kinfv = in.kinfv, kmacv = in.kmacv;
+  const double areafh = in.areafh, dmac = in.dmac, ksatv = in.ksatv, porosity = in.porosity;
+  const double u = clamp0(in.yu);
+  const double g = me.gw;
+  const double surfh = me.surfh;
+  const bool wet_bottom = v.wet_bottom;
+  const double deficit = v.deficit, satn = v.satn, psi_raw = v.psi_raw, kr = v.kr;
 
   // Infil: src/vert_flow.c:33-130
   double infil;
   if (u + g > depth) {
     infil = 0.0;
   } else {
-    double applrate = 0.0;
-#pragma unroll
-    for (int j = 0; j < 3; j++) applrate += -ovl[j] / area;
+    double applrate = lat.asum;
     applrate = (applrate > 0.0) ? applrate : 0.0;
     applrate += pcpdrp;
     double wetfrac = surfh / DEPRSTG;
@@ -359,10 +542,10 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
       if (N) {
         // saturation of the top soil layer; smcmax - smcmin is the porosity
         // (src/init_soil.c:34-36)
-        isatn = (e.sh2o0[p] - e.smcmin[p]) / porosity;
+        isatn = (in.sh2o0 - in.smcmin) / porosity;
         isatn = (isatn > 1.0) ? 1.0 : isatn;
         isatn = (isatn < SATMIN) ? SATMIN : isatn;
-        ipsi = psi_vg(isatn, alpha, beta);
+        ipsi = psi_vg(isatn, in.alpha, beta);
         ikr = kr_func(beta, isatn);
       }
       const double psi_u = (ipsi > PSIMIN) ? ipsi : PSIMIN;
@@ -377,7 +560,7 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     infil = (infil > infil_max) ? infil_max : infil;
     infil *= wetfrac;
   }
-  if (N) infil *= e.fcr[p];   // frozen top soil (src/vert_flow.c:16-19), before Recharge reads it
+  if (N) infil *= in.fcr;   // frozen top soil (src/vert_flow.c:16-19), before Recharge reads it
 
   // Recharge: src/vert_flow.c:132-170
   double rechg;
@@ -390,43 +573,21 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
     rechg = (rechg > 0.0 && u <= 0.0) ? 0.0 : rechg;
     rechg = (rechg < 0.0 && g <= 0.0) ? 0.0 : rechg;
   }
-
-  // River edges: this call's bank fluxes (src/river_flow.c:113-183, from the
-  // river blocks of k_prep) replace the carried overland flux that Infil has
-  // used, and the new overland flux is carried to the next call
-  // (src/river_flow.c:159-182).
-  if (bank) {
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      if (nb_is_river(c[j])) {
-        const int slot = -c[j] - 1;
-        ovl[j] = a.r.hid_ovl[2 * a.nr + slot];
-        sub[j] = a.r.bank_sub[slot];
-        a.r.hid_ovl[slot] = ovl[j];
-      }
-    }
-  }
-
   if (a.record) {
     a.rec.infil[p] = infil;
     a.rec.rechg[p] = rechg;
-#pragma unroll
-    for (int j = 0; j < 3; j++) {
-      a.rec.ovl[j * ne + p] = ovl[j];
-      a.rec.sub[j * ne + p] = sub[j];
-    }
   }
 
   // element ODEs: src/ode.c:121-149
   double dys = 0.0, dyu = 0.0, dyg = 0.0;
-  dys += pcpdrp - infil - edir_s;
-  dyu += infil - rechg - edir_u - ett_u;
-  dyg += rechg - edir_g - ett_g;
+  dys += pcpdrp - infil - v.edir_s;
+  dyu += infil - rechg - v.edir_u - v.ett_u;
+  dyg += rechg - v.edir_g - v.ett_g;
   dyu /= porosity;
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    dys -= ovl[j] / area;
-    dyg -= sub[j] / area;
+    dys -= lat.ovl[j] / area;
+    dyg -= lat.sub[j] / area;
   }
   dyg /= porosity;
   ydot[lp] = dys;
@@ -434,23 +595,28 @@ __device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
   ydot[2 * ix.ne + lp] = dyg;
   if (isnan(dys) || isnan(dyu) || isnan(dyg)) flag_nan<B>(a, ix);
   if (!B && a.closure) {
-    // volume rates crossing the element's outer boundary
-    double out = 0.0;
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-      if (bnd[j]) out += ovl[j] + sub[j];
     a.cl_e[p] = pcpdrp * area;
-    a.cl_e[ne + p] = ((((edir_s + edir_u) + edir_g) + ett_u) + ett_g) * area;
-    a.cl_e[2L * ne + p] = out;
+    a.cl_e[ne + p] = ((((v.edir_s + v.edir_u) + v.edir_g) + v.ett_u) + v.ett_g) * area;
+    a.cl_e[2L * ne + p] = lat.out;
   }
 }
 
-// What a river segment reads of one bank element: the fields k_prep stores in
-// the element's record, evaluated from the same operands with the same
-// expressions (the river fluxes run in k_prep, beside the element records).
-struct BankView {
-  double surfh, gw, gwz, effkh, zmax, zmin;
-};
+template <bool B, bool N, class Src>
+__device__ __forceinline__ void elem_core(const RhsArgs &a, int p, const Ix &ix, const Src &src, const ElemIn &in) {
+  const NbrRec me = src.self();
+  const LatOut lat = elem_lateral<B>(a, p, ix, src, in, me);
+  const VertPre v = elem_vert_pre<N>(in, me);
+  elem_vert_post<B, N>(a, p, ix, in, me, v, lat);
+}
+
+template <bool B, bool N = false>
+__device__ __forceinline__ void elem_flux(const RhsArgs &a, int p) {
+  const Ix ix = ix_elem<B>(a, p);
+  if (!ix.active) return;
+  if (B && p - ix.eoff >= ix.ne) return;     // unused capacity of a slot
+  const GlobalSrc src{a, p};
+  elem_core<B, N>(a, p, ix, src, load_elem_in<N>(a, p, ix, src));
+}
 
 template <bool B>
 __device__ __forceinline__ BankView bank_view(const RhsArgs &a, const Ix &ix, int q) {
@@ -469,13 +635,12 @@ __device__ __forceinline__ BankView bank_view(const RhsArgs &a, const Ix &ix, in
 }
 
 // SubFlowRiverToRiver (_ARITH_): src/river_flow.c:302-331, aquifer flow from
-// segment u into its downstream segment d, from the bank elements' records
-// (yg: river groundwater of the solve, by river index).  Evaluated in k_flux
-// by d for each upstream u and by u for itself, so both get the same bits.
-__device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, int u, int d) {
-  const RiverDev &R = a.r;
-  const double effk = 0.5 * (a.e.rec[R.left[u]].effkh + a.e.rec[R.right[u]].effkh);
-  const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
+// segment u into its downstream segment d (yg: river groundwater of the solve,
+// by river index; effk, effk_nabr: the mean EffKh of u's and of d's banks).
+// Evaluated by d for each upstream u and by u for itself, from the same
+// operands, so both get the same bits.
+__device__ __forceinline__ double sub_r2r_k(const RiverDev &R, const double *yg, int u, int d, double effk,
+                                            double effk_nabr) {
   const double rgw = clamp0(yg[u]);
   const double dgw = clamp0(yg[d]);
   const double total_h = rgw + R.zmin[u];
@@ -489,11 +654,108 @@ __device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, in
   return avg_ksat * grad_h * avg_h * avg_wid;
 }
 
-template <bool B>
-__device__ __forceinline__ double effkh_of(const RhsArgs &a, const Ix &ix, int q) {
-  const ElemDev &e = a.e;
-  const double g = clamp0(ix.y[2 * ix.ne + (q - ix.eoff)]);
-  return eff_kh(e.depth[q], e.dmac[q], e.kmach[q], e.areafv[q], e.ksath[q], g);
+// The same with the banks' EffKh from the records k_prep wrote.
+__device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, int u, int d) {
+  const RiverDev &R = a.r;
+  const double effk = 0.5 * (a.e.rec[R.left[u]].effkh + a.e.rec[R.right[u]].effkh);
+  const double effk_nabr = 0.5 * (a.e.rec[R.left[d]].effkh + a.e.rec[R.right[d]].effkh);
+  return sub_r2r_k(R, yg, u, d, effk, effk_nabr);
+}
+
+// Segment r's channel outflow (rivflow 1: to the downstream segment or through
+// the outlet), the boundary flux of its upstream node (upinit) and its channel
+// leakage (rivflow 6): src/river_flow.c:10-87, 257-300, 333-427, 500-518.
+// ys, yg: stage and river groundwater of the solve, by river index.
+__device__ __forceinline__ void river_out(const RhsArgs &a, const double *ys, const double *yg, int r,
+                                          double &upinit_out, double &rf1_out, double &rf6_out) {
+  const RiverDev &R = a.r;
+  const double stage = clamp0(ys[r]);
+  const double rgw = clamp0(yg[r]);
+  const double zbed = R.zbed[r], length = R.length[r];
+  const double rzmin = R.zmin[r];
+  const int ord = R.ord[r];
+  const double coeff = R.coeff[r];
+  const double bcv = R.bc[r];
+  const int dcode = R.down[r];
+  double upinit = 0.0, rf1;
+  if (dcode >= 0) {
+    const int d = dcode;
+    if (R.bctype[r] != 0) {
+      // BoundFluxRiver: src/river_flow.c:392-427
+      double flux = 0.0;
+      if (R.bctype[r] > 0) {
+        const double total_h = stage + zbed;
+        const double distance = 0.5 * length;
+        const double grad_h = (total_h - bcv) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+        flux = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+      } else {
+        flux = -bcv;
+      }
+      upinit += flux;
+    }
+    // ChanFlowRiverToRiver: src/river_flow.c:257-300
+    const double dstage = clamp0(ys[d]);
+    const double total_h = stage + zbed;
+    const double perim = riv_perim(ord, stage, coeff);
+    const double total_h_down = dstage + R.zbed[d];
+    const double perim_down = riv_perim(R.ord[d], dstage, R.coeff[d]);
+    const double avg_perim = (perim + perim_down) / 2.0;
+    const double avg_rough = (R.rough[r] + R.rough[d]) / 2.0;
+    const double distance = 0.5 * (length + R.length[d]);
+    const double diff_h = (a.riv_mode == KINEMATIC) ? (zbed - R.zbed[d]) : (total_h - total_h_down);
+    const double grad_h = diff_h / distance;
+    const double avg_sf = (grad_h > 0.0) ? grad_h : RIVGRADMIN;
+    const double crossa = riv_cs_area(ord, stage, coeff);
+    const double crossa_down = riv_cs_area(R.ord[d], dstage, R.coeff[d]);
+    const double avg_crossa = 0.5 * (crossa + crossa_down);
+    const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
+    rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
+  } else {
+    // OutletFlux: src/river_flow.c:333-390
+    double q = 0.0;
+    switch (dcode) {
+      case DIRICHLET: {
+        const double total_h = stage + zbed;
+        const double distance = 0.5 * length;
+        const double grad_h = (total_h - bcv) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
+        q = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
+        break;
+      }
+      case NEUMANN: q = -bcv; break;
+      case ZERO_DPTH_GRAD: {
+        const double distance = 0.5 * length;
+        const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
+        const double perim = riv_perim(ord, stage, coeff);
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        q = sqrt(grad_h) * crossa * ((perim > 0.0) ? xpow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
+        break;
+      }
+      default: {  // CRIT_DPTH (other codes rejected at creation)
+        const double crossa = riv_cs_area(ord, stage, coeff);
+        q = crossa * sqrt(GRAV * stage);
+        break;
+      }
+    }
+    rf1 = q;
+  }
+  // ChanLeak: src/river_flow.c:500-518
+  double rf6;
+  {
+    double diff_h;
+    if (zbed - (rgw + rzmin) > 0.0) diff_h = stage;
+    else diff_h = stage + zbed - (rgw + rzmin);
+    const double grad_h = diff_h / R.bedthick[r];
+    rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
+  }
+  upinit_out = upinit;
+  rf1_out = rf1;
+  rf6_out = rf6;
 }
 
 // River segment fluxes (src/river_flow.c:10-87, 113-183, 257-300, 333-518),
@@ -516,93 +778,11 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r, int part) {
   if (B && r - ix.roff >= ix.nr) return;            // unused capacity of a slot
   const double *ys = ix.y + 3L * ix.ne - ix.roff;   // ys[r']: stage of river r'
   const double *yg = ys + ix.nr;                    // yg[r']: river groundwater
-  const double stage = clamp0(ys[r]);
-  const double rgw = clamp0(yg[r]);
-  const double zbed = R.zbed[r], length = R.length[r];
-  const double rzmin = R.zmin[r];
 
   if (part == 0) {
-    const int ord = R.ord[r];
-    const double coeff = R.coeff[r];
-    const double bcv = R.bc[r];
-    const int dcode = R.down[r];
-    double upinit = 0.0, rf1;
-    if (dcode >= 0) {
-      const int d = dcode;
-      if (R.bctype[r] != 0) {
-        // BoundFluxRiver: src/river_flow.c:392-427
-        double flux = 0.0;
-        if (R.bctype[r] > 0) {
-          const double total_h = stage + zbed;
-          const double distance = 0.5 * length;
-          const double grad_h = (total_h - bcv) / distance;
-          const double perim = riv_perim(ord, stage, coeff);
-          const double crossa = riv_cs_area(ord, stage, coeff);
-          const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
-          flux = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
-        } else {
-          flux = -bcv;
-        }
-        upinit += flux;
-      }
-      // ChanFlowRiverToRiver: src/river_flow.c:257-300
-      const double dstage = clamp0(ys[d]);
-      const double total_h = stage + zbed;
-      const double perim = riv_perim(ord, stage, coeff);
-      const double total_h_down = dstage + R.zbed[d];
-      const double perim_down = riv_perim(R.ord[d], dstage, R.coeff[d]);
-      const double avg_perim = (perim + perim_down) / 2.0;
-      const double avg_rough = (R.rough[r] + R.rough[d]) / 2.0;
-      const double distance = 0.5 * (length + R.length[d]);
-      const double diff_h = (a.riv_mode == KINEMATIC) ? (zbed - R.zbed[d]) : (total_h - total_h_down);
-      const double grad_h = diff_h / distance;
-      const double avg_sf = (grad_h > 0.0) ? grad_h : RIVGRADMIN;
-      const double crossa = riv_cs_area(ord, stage, coeff);
-      const double crossa_down = riv_cs_area(R.ord[d], dstage, R.coeff[d]);
-      const double avg_crossa = 0.5 * (crossa + crossa_down);
-      const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
-      rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
-    } else {
-      // OutletFlux: src/river_flow.c:333-390
-      double q = 0.0;
-      switch (dcode) {
-        case DIRICHLET: {
-          const double total_h = stage + zbed;
-          const double distance = 0.5 * length;
-          const double grad_h = (total_h - bcv) / distance;
-          const double perim = riv_perim(ord, stage, coeff);
-          const double crossa = riv_cs_area(ord, stage, coeff);
-          const double avg_h = (perim == 0.0) ? 0.0 : (crossa / perim);
-          q = overland_flow(avg_h, grad_h, grad_h, crossa, R.rough[r]);
-          break;
-        }
-        case NEUMANN: q = -bcv; break;
-        case ZERO_DPTH_GRAD: {
-          const double distance = 0.5 * length;
-          const double grad_h = (zbed - (R.node_zmax[r] - R.depth[r])) / distance;
-          const double perim = riv_perim(ord, stage, coeff);
-          const double crossa = riv_cs_area(ord, stage, coeff);
-          q = sqrt(grad_h) * crossa * ((perim > 0.0) ? xpow(crossa / perim, 2.0 / 3.0) : 0.0) / R.rough[r];
-          break;
-        }
-        default: {  // CRIT_DPTH (other codes rejected at creation)
-          const double crossa = riv_cs_area(ord, stage, coeff);
-          q = crossa * sqrt(GRAV * stage);
-          break;
-        }
-      }
-      rf1 = q;
-    }
-    // ChanLeak: src/river_flow.c:500-518
-    double rf6;
-    {
-      double diff_h;
-      if (zbed - (rgw + rzmin) > 0.0) diff_h = stage;
-      else diff_h = stage + zbed - (rgw + rzmin);
-      const double grad_h = diff_h / R.bedthick[r];
-      rf6 = R.ksatv[r] * R.width[r] * length * grad_h;
-    }
-    if (!B && a.closure) a.cl_r[r] = upinit + ((dcode < 0) ? rf1 : 0.0);   // leaves through river boundaries
+    double upinit, rf1, rf6;
+    river_out(a, ys, yg, r, upinit, rf1, rf6);
+    if (!B && a.closure) a.cl_r[r] = upinit + ((R.down[r] < 0) ? rf1 : 0.0);   // leaves through river boundaries
     R.rf[1 * nr + r] = rf1;
     R.rf[6 * nr + r] = rf6;
     R.upinit[r] = upinit;
@@ -611,21 +791,19 @@ __device__ __forceinline__ void river_flux(const RhsArgs &a, int r, int part) {
 
   // RiverToElem for one bank: src/river_flow.c:113-183
   const int side = part - 1;
+  const double stage = clamp0(ys[r]);
+  const double rgw = clamp0(yg[r]);
   const int q = side ? R.right[r] : R.left[r];
   const BankView bv = bank_view<B>(a, ix, q);
   const double other = effkh_of<B>(a, ix, side ? R.left[r] : R.right[r]);
-  const double dist = side ? R.dist_r[r] : R.dist_l[r];
-  const double rf_ovl = ovl_elem_to_river(R.zmax[r], bv.zmax, bv.surfh, zbed, stage, R.cwr[r], length);
-  const double rf_chan = chan_elem_to_river(stage, zbed, length, R.ksath[r], bv.gw, bv.gwz, bv.zmin, bv.effkh, dist);
   const double effk_riv = side ? 0.5 * (other + bv.effkh) : 0.5 * (bv.effkh + other);
-  const double rf_sub =
-      sub_elem_to_river(rgw, rzmin, zbed, length, bv.gw, bv.gwz, bv.zmin, bv.effkh, effk_riv, dist);
-  R.rf[(2 + side) * nr + r] = rf_ovl;
-  R.rf[(4 + side) * nr + r] = rf_chan;
-  R.rf[(7 + side) * nr + r] = rf_sub;
+  const BankFlux f = bank_flux(R, r, side, stage, rgw, bv, effk_riv);
+  R.rf[(2 + side) * nr + r] = f.ovl;
+  R.rf[(4 + side) * nr + r] = f.chan;
+  R.rf[(7 + side) * nr + r] = f.sub;
   // bank-edge replacement values (src/river_flow.c:159-182)
-  R.hid_ovl[2 * nr + 2 * r + side] = -rf_ovl;
-  R.bank_sub[2 * r + side] = -(rf_chan + rf_sub);
+  R.hid_ovl[2 * nr + 2 * r + side] = -f.ovl;
+  R.bank_sub[2 * r + side] = -(f.chan + f.sub);
 }
 
 // River ODEs (src/ode.c:238-251), run by the river blocks of k_flux: the
@@ -685,10 +863,289 @@ __global__ __launch_bounds__(kBlock, 3) void k_flux(RhsArgs a, int nbe) {
   }
 }
 
+// River segments of a single solve in the one-launch RHS: a block takes
+// kRivSeg consecutive segments, and each quarter of it one part of every
+// segment's work, so that no thread carries the whole chain of gathers:
+//   part 0: river_out (channel outflow, upstream-node boundary flux, leakage);
+//   part 1: the left bank's fluxes; part 2: the right bank's;
+//   part 3: the aquifer flow to the downstream segment (rivflow 9) and the
+//           upstream inflow: from the boundary flux of the upstream node,
+//           each upstream segment's channel outflow subtracted in ascending
+//           caller index (src/river_flow.c:96-110), and each one's aquifer
+//           outflow, all evaluated here from the same operands that segment
+//           uses for itself.
+// The parts meet in LDS (xr: 9 doubles per segment); part 0 then writes the
+// river ODEs (src/ode.c:238-251) in river_ode's order.
+constexpr int kTile = 64;             // elements per tile = threads per block of the one-launch RHS
+constexpr int kRivSeg = kTile / 4;
+
+__device__ __forceinline__ void river_parts(const RhsArgs &a, int rb, double *xr) {
+  const int nr = a.nr;
+  const RiverDev &R = a.r;
+  const int part = (int)threadIdx.x / kRivSeg, k = (int)threadIdx.x % kRivSeg;
+  const int r = rb * kRivSeg + k;
+  const Ix ix = ix_river<false>(a, 0);
+  const double *ys = ix.y + 3L * ix.ne;
+  const double *yg = ys + nr;
+  double *x = xr + 9 * k;
+  double upinit = 0.0, rf1 = 0.0, rf6 = 0.0;
+  if (r < nr) {
+    if (part == 0) {
+      river_out(a, ys, yg, r, upinit, rf1, rf6);
+    } else if (part <= 2) {
+      const int side = part - 1;
+      const double stage = clamp0(ys[r]);
+      const double rgw = clamp0(yg[r]);
+      const BankView bv = bank_view<false>(a, ix, side ? R.right[r] : R.left[r]);
+      const double other = effkh_of<false>(a, ix, side ? R.left[r] : R.right[r]);
+      const double effk_riv = side ? 0.5 * (other + bv.effkh) : 0.5 * (bv.effkh + other);
+      const BankFlux f = bank_flux(R, r, side, stage, rgw, bv, effk_riv);
+      x[3 * side + 0] = f.ovl;
+      x[3 * side + 1] = f.chan;
+      x[3 * side + 2] = f.sub;
+    } else {
+      double ui, u1, u6;
+      river_out(a, ys, yg, r, ui, u1, u6);
+      const double effk_r = 0.5 * (effkh_of<false>(a, ix, R.left[r]) + effkh_of<false>(a, ix, R.right[r]));
+      const int dcode = R.down[r];
+      double rf9 = 0.0;
+      if (dcode >= 0) {
+        const double effk_d =
+            0.5 * (effkh_of<false>(a, ix, R.left[dcode]) + effkh_of<false>(a, ix, R.right[dcode]));
+        rf9 = sub_r2r_k(R, yg, r, dcode, effk_r, effk_d);
+      }
+      double up = ui, upa = 0.0;
+      for (int kk = R.up_beg[r]; kk < R.up_end[r]; kk++) {
+        const int u = R.up_idx[kk];
+        double vi, v1, v6;
+        river_out(a, ys, yg, u, vi, v1, v6);
+        up -= v1;
+        const double effk_u = 0.5 * (effkh_of<false>(a, ix, R.left[u]) + effkh_of<false>(a, ix, R.right[u]));
+        upa -= sub_r2r_k(R, yg, u, r, effk_u, effk_r);
+      }
+      x[6] = up;
+      x[7] = upa;
+      x[8] = rf9;
+    }
+  }
+  __syncthreads();
+  if (part != 0 || r >= nr) return;
+  const double flo = x[0], fro = x[3], flc = x[1], frc = x[4], fls = x[2], frs = x[5];
+  const double up = x[6], upa = x[7], rf9 = x[8];
+  const double area = R.area[r];
+  const double rfs[7] = {0.0, rf1, flo, fro, flc, frc, rf6};
+  double ds = 0.0;
+  ds -= up / area;
+#pragma unroll
+  for (int j = 1; j <= 6; j++) ds -= rfs[j] / area;
+  double dg = 0.0;
+  dg += -fls - frs - rf9 - upa + rf6;
+  dg /= R.porosity[r] * area;
+  ix.ydot[3L * ix.ne + r] = ds;
+  ix.ydot[3L * ix.ne + nr + r] = dg;
+  if (a.closure) a.cl_r[r] = upinit + ((R.down[r] < 0) ? rf1 : 0.0);
+  if (a.record) {
+    R.rf[0 * nr + r] = up;
+#pragma unroll
+    for (int j = 1; j <= 6; j++) R.rf[j * nr + r] = rfs[j];
+    R.rf[7 * nr + r] = fls;
+    R.rf[8 * nr + r] = frs;
+    R.rf[9 * nr + r] = rf9;
+    R.rf[10 * nr + r] = upa;
+  }
+  if (isnan(ds) || isnan(dg)) flag_nan<false>(a, ix);
+}
+
+// A halo or tile entry's record fields (all but the FrictSlope gradient) and
+// its head, as k_prep evaluates them.
+__device__ __forceinline__ NbrRec tile_rec(double ys, double yg, double zmax, double zmin, double depth,
+                                           double dmac, double kmach, double areafv, double ksath, double rough) {
+  const double surfh = surf_h(clamp0(ys));
+  const double g = clamp0(yg);
+  NbrRec r;
+  r.surfh = surfh;
+  r.head = zmax + surfh;
+  r.gw = g;
+  r.gwz = g + zmin;
+  r.effkh = eff_kh(depth, dmac, kmach, areafv, ksath, g);
+  r.sfmag = 0.0;
+  r.rough = rough;
+  r.zmax = zmax;
+  return r;
+}
+
+// FrictSlope (src/lat_flow.c:130-185) of the LDS entry at slot s (device
+// element q): neighbour heads from LDS, or the boundary value, or the stage
+// of the river edge.
+__device__ __forceinline__ double tile_sfmag(const RhsArgs &a, int s, long q, const int c[3], const double nx[3],
+                                             const double ny[3], const double *shead) {
+  const long ne = a.ne;
+  double h[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    if (c[j] >= 0) {
+      h[j] = shead[c[j]];
+    } else if (nb_is_bound(c[j])) {
+      h[j] = (c[j] == NB_BOUND) ? shead[s] : a.e.ebc[j * ne + q];
+    } else {
+      const int r = (-c[j] - 1) >> 1;
+      const double st = clamp0(a.y[3 * ne + r]);
+      h[j] = (st > a.r.depth[r]) ? a.r.zbed[r] + st : a.r.zmax[r];
+    }
+  }
+  const double dhdx = dh_by_dl(ny[0], ny[1], ny[2], nx[0], nx[1], nx[2], h[0], h[1], h[2]);
+  const double dhdy = dh_by_dl(nx[0], nx[1], nx[2], ny[0], ny[1], ny[2], h[0], h[1], h[2]);
+  return sqrt(dhdx * dhdx + dhdy * dhdy);
+}
+
+// One RHS call of a single solve in one launch.  Grid: river blocks
+// (river_parts, kRivSeg segments each), then one block per tile of kTile
+// consecutive elements (Hilbert order), each block one wavefront, so that a
+// block's barriers cost little and blocks run independently, hiding each
+// other's memory latency.  Each element block stages in LDS what its
+// elements' fluxes read: the heads of the tile and of its halo (the first
+// ring of outside neighbours and their outside neighbours), and the records
+// of the tile and the first ring, whose FrictSlope gradients need those heads;
+// then it runs the element fluxes from them (elem_core with TileSrc).  Every
+// global load a thread makes is issued at the start (its element's inputs, and
+// the inputs of the halo entry it stages), so a block waits for memory about
+// twice, and the barriers wait only for LDS work.  The halo's records are
+// evaluated again by every tile that borders them, from the same operands:
+// every output keeps one writer, and the arithmetic is that of k_prep +
+// k_flux, bit for bit.
+template <bool N>
+__global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, int nbe) {
+  extern __shared__ double tile_lds[];
+  const LdsRecs srec{tile_lds, t.s1cap};
+  double *shead = tile_lds + 8 * t.s1cap;
+  const int nrb = (int)gridDim.x - nbe;
+  if ((int)blockIdx.x < nrb) {
+    river_parts(a, (int)blockIdx.x, tile_lds);
+    return;
+  }
+  const int tile = xcd_tile((int)blockIdx.x - nrb, nbe, a.tile_span);
+  const long ne = a.ne;
+  const int lo = tile * kTile, i = (int)threadIdx.x, p = lo + i;
+  const bool diff = a.surf_mode == DIFF_WAVE;
+  const int hb = t.hbeg[tile], nh = t.hbeg[tile + 1] - hb, n1 = t.hn1[tile];
+  const ElemDev &e = a.e;
+  const double *y = a.y;
+  const Ix ix = ix_elem<false>(a, 0);
+  const TileSrc src{a, srec, t.lnb, p, i};
+
+  // own element: everything it needs in all three phases
+  const bool own = p < ne;
+  ElemIn in;
+  double ys = 0.0, yg = 0.0, kmach = 0.0, areafv = 0.0, ksath = 0.0, rough = 0.0;
+  double nx[3] = {0.0, 0.0, 0.0}, ny[3] = {0.0, 0.0, 0.0};
+  if (own) {
+    in = load_elem_in<N>(a, p, ix, src);
+    ys = y[p];
+    yg = y[2 * ne + p];
+    kmach = e.kmach[p]; areafv = e.areafv[p]; ksath = e.ksath[p]; rough = e.rough[p];
+    if (diff) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        nx[j] = e.nabr_x[j * ne + p];
+        ny[j] = e.nabr_y[j * ne + p];
+      }
+    }
+  }
+  // the halo entry this thread stages (further entries, in halos longer than a
+  // block, are staged in the loops below)
+  const bool hv = i < nh, h1 = i < n1;
+  long hq = 0;
+  double hys = 0.0, hyg = 0.0, hzmax = 0.0, hzmin = 0.0, hdepth = 0.0, hdmac = 0.0;
+  double hkmach = 0.0, hareafv = 0.0, hksath = 0.0, hrough = 0.0;
+  int hc[3] = {0, 0, 0};
+  double hnx[3] = {0.0, 0.0, 0.0}, hny[3] = {0.0, 0.0, 0.0};
+  if (hv) {
+    hq = t.hidx[hb + i];
+    hys = y[hq];
+    hzmax = e.zmax[hq];
+    if (h1) {
+      hyg = y[2 * ne + hq];
+      hzmin = e.zmin[hq]; hdepth = e.depth[hq]; hdmac = e.dmac[hq];
+      hkmach = e.kmach[hq]; hareafv = e.areafv[hq]; hksath = e.ksath[hq]; hrough = e.rough[hq];
+      if (diff) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          hc[j] = t.hnb[3 * (hb + i) + j];
+          hnx[j] = e.nabr_x[j * ne + hq];
+          hny[j] = e.nabr_y[j * ne + hq];
+        }
+      }
+    }
+  }
+
+  // records and heads
+  if (own) {
+    const NbrRec r = tile_rec(ys, yg, in.zmax, in.zmin, in.depth, in.dmac, kmach, areafv, ksath, rough);
+    srec.put(i, r);
+    shead[i] = r.head;
+  }
+  if (hv) {
+    if (h1) {
+      const NbrRec r = tile_rec(hys, hyg, hzmax, hzmin, hdepth, hdmac, hkmach, hareafv, hksath, hrough);
+      srec.put(kTile + i, r);
+      shead[kTile + i] = r.head;
+    } else {
+      shead[kTile + i] = hzmax + surf_h(clamp0(hys));
+    }
+  }
+  for (int h = i + kTile; h < nh; h += kTile) {
+    const long q = t.hidx[hb + h];
+    if (h < n1) {
+      const NbrRec r = tile_rec(y[q], y[2 * ne + q], e.zmax[q], e.zmin[q], e.depth[q], e.dmac[q], e.kmach[q],
+                                e.areafv[q], e.ksath[q], e.rough[q]);
+      srec.put(kTile + h, r);
+      shead[kTile + h] = r.head;
+    } else {
+      shead[kTile + h] = e.zmax[q] + surf_h(clamp0(y[q]));
+    }
+  }
+  __syncthreads();
+  if (diff) {
+    if (own) srec.put_sfmag(i, tile_sfmag(a, i, p, in.c, nx, ny, shead));
+    if (h1) srec.put_sfmag(kTile + i, tile_sfmag(a, kTile + i, hq, hc, hnx, hny, shead));
+    for (int h = i + kTile; h < n1; h += kTile) {
+      const long q = t.hidx[hb + h];
+      int c[3];
+      double qx[3], qy[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        c[j] = t.hnb[3 * (hb + h) + j];
+        qx[j] = e.nabr_x[j * ne + q];
+        qy[j] = e.nabr_y[j * ne + q];
+      }
+      srec.put_sfmag(kTile + h, tile_sfmag(a, kTile + h, q, c, qx, qy, shead));
+    }
+    __syncthreads();
+  }
+  if (!own) return;
+  elem_core<false, N>(a, p, ix, src, in);
+}
+
 // k_flux of a single solve, with the _NOAH_ branches when the context has them.
 static void launch_flux_single(const pihm_ctx *c, const RhsArgs &a, int nbe, int nbr) {
   if (c->noah) hipLaunchKernelGGL((k_flux<false, true>), dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   else hipLaunchKernelGGL((k_flux<false, false>), dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
+}
+
+// One RHS evaluation of a single solve on the context's path.
+static void launch_single(const pihm_ctx *c, const RhsArgs &a) {
+  const int nbe = div_up(c->ne, kBlock);
+  const int nbr = div_up(c->nr, kBlock);
+  if (c->rhs_path == PIHM_RHS_ONE_LAUNCH) {
+    const int nt = div_up(c->ne, kTile), nrb = div_up(c->nr, kRivSeg);
+    if (c->noah)
+      hipLaunchKernelGGL((k_rhs_tile<true>), dim3(nt + nrb), dim3(kTile), c->tile_lds, c->stream, a, c->tiles, nt);
+    else
+      hipLaunchKernelGGL((k_rhs_tile<false>), dim3(nt + nrb), dim3(kTile), c->tile_lds, c->stream, a, c->tiles, nt);
+    return;
+  }
+  hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
+  launch_flux_single(c, a, nbe, nbr);
 }
 
 // Water-balance accumulators: the four volume rates summed over the solve in
@@ -828,6 +1285,8 @@ namespace {
 // upstream-list and state ranges of the slots are concatenated at their
 // capacities; slots start empty (no elements, no rivers) until a mesh is
 // loaded into them (ctx_load).
+int build_tiles(pihm_ctx *c);
+
 int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pihm_ctx **out) {
   *out = nullptr;
   const int nmem = (int)caps.size();
@@ -862,6 +1321,11 @@ int ctx_alloc(const std::vector<SlotCap> &caps, int surf_mode, int riv_mode, pih
   c->stream = nullptr; c->nan_check = 1; c->record = 0; c->ncalls = 0;
   c->arena = nullptr;
   c->nanflag_host = nullptr;
+  c->rhs_path = PIHM_RHS_TWO_KERNELS;
+  c->one_launch_ok = 0;
+  c->tiles = TileArgs{};
+  c->tile_lds = 0;
+  c->tile_arena = nullptr;
   PIHM_HIP_TRY(hipGetDevice(&c->device));
   c->perm.assign(ne, -1);
   c->iperm.assign(ne, -1);
@@ -1152,7 +1616,91 @@ int build_ctx(const std::vector<const pihm_mesh_desc *> &ms, pihm_ctx **out) {
   for (int k = 0; k < (int)ms.size(); k++) {
     if ((rc = ctx_load(c, k, ms[k]))) { pihm_ctx_destroy(c); return rc; }
   }
+  if ((rc = build_tiles(c))) { pihm_ctx_destroy(c); return rc; }
   *out = c;
+  return PIHM_OK;
+}
+
+// Tile tables of the one-launch RHS (TileArgs) for a single-solve context.
+// Each element block holds s1cap records and scap heads in LDS; a mesh whose
+// halos would need more than kTileLdsMax bytes keeps the two-kernel RHS.
+constexpr size_t kTileLdsMax = 64 * 1024;
+
+int build_tiles(pihm_ctx *c) {
+  const long ne = c->ne;
+  if (c->nmem != 1 || ne < 1) return PIHM_OK;
+  const int ntile = div_up((int)ne, kTile);
+  std::vector<int32_t> hbeg(ntile + 1, 0), hn1(ntile, 0), hidx, hnb, lnb(3 * ne);
+  std::vector<int32_t> stamp(ne, -1), slot(ne, 0), r1, r2;
+  const auto code = [&](int j, long q) { return c->nb_host[(long)j * ne + q]; };
+  int n1max = 0, nhmax = 0;
+  for (int t = 0; t < ntile; t++) {
+    const long lo = (long)t * kTile, hi = std::min(ne, lo + kTile);
+    for (long q = lo; q < hi; q++) { stamp[q] = t; slot[q] = (int)(q - lo); }
+    r1.clear();
+    r2.clear();
+    for (long q = lo; q < hi; q++)
+      for (int j = 0; j < 3; j++) {
+        const int cc = code(j, q);
+        if (cc >= 0 && stamp[cc] != t) { stamp[cc] = t; r1.push_back(cc); }
+      }
+    std::sort(r1.begin(), r1.end());
+    for (size_t i = 0; i < r1.size(); i++) slot[r1[i]] = kTile + (int)i;
+    for (int q : r1)
+      for (int j = 0; j < 3; j++) {
+        const int cc = code(j, q);
+        if (cc >= 0 && stamp[cc] != t) { stamp[cc] = t; r2.push_back(cc); }
+      }
+    std::sort(r2.begin(), r2.end());
+    for (size_t i = 0; i < r2.size(); i++) slot[r2[i]] = kTile + (int)(r1.size() + i);
+    // every element neighbour of the tile and of its first ring now has a slot
+    for (long q = lo; q < hi; q++)
+      for (int j = 0; j < 3; j++) {
+        const int cc = code(j, q);
+        lnb[(long)j * ne + q] = (cc >= 0) ? slot[cc] : cc;
+      }
+    for (int q : r1)
+      for (int j = 0; j < 3; j++) {
+        const int cc = code(j, q);
+        hnb.push_back((cc >= 0) ? slot[cc] : cc);
+      }
+    for (size_t i = 0; i < 3 * r2.size(); i++) hnb.push_back(NB_BOUND);
+    hidx.insert(hidx.end(), r1.begin(), r1.end());
+    hidx.insert(hidx.end(), r2.begin(), r2.end());
+    hbeg[t + 1] = (int32_t)hidx.size();
+    hn1[t] = (int32_t)r1.size();
+    n1max = std::max(n1max, (int)r1.size());
+    nhmax = std::max(nhmax, (int)(r1.size() + r2.size()));
+  }
+  const int s1cap = kTile + n1max, scap = kTile + nhmax;
+  const size_t lds = (8 * (size_t)s1cap + (size_t)scap) * sizeof(double);   // 8 record fields, heads
+  if (lds > kTileLdsMax) return PIHM_OK;       // the two-kernel RHS stays
+  const size_t nh = hidx.size();
+  const size_t words = (ntile + 1) + ntile + nh + 3 * nh + 3 * ne;
+  int32_t *d = nullptr;
+  PIHM_HIP_TRY(hipMalloc((void **)&d, std::max<size_t>(words, 1) * sizeof(int32_t)));
+  c->tile_arena = d;
+  int32_t *w = d;
+  const auto put = [&](const std::vector<int32_t> &v) -> const int32_t * {
+    if (!v.empty() && hipMemcpy(w, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return nullptr;
+    const int32_t *at = w;
+    w += v.size();
+    return at;
+  };
+  TileArgs T{};
+  T.hbeg = put(hbeg);
+  T.hn1 = put(hn1);
+  T.hidx = put(hidx);
+  T.hnb = put(hnb);
+  T.lnb = put(lnb);
+  if (!T.hbeg || !T.hn1 || (nh && (!T.hidx || !T.hnb)) || !T.lnb) return PIHM_ERR_HIP;
+  T.s1cap = s1cap;
+  T.scap = scap;
+  c->tiles = T;
+  c->tile_lds = lds;
+  c->one_launch_ok = 1;
+  c->rhs_path = PIHM_RHS_ONE_LAUNCH;
   return PIHM_OK;
 }
 
@@ -1203,9 +1751,20 @@ extern "C" void pihm_ctx_destroy(pihm_ctx *c) {
   if (!c) return;
   if (c->arena) (void)hipFree(c->arena);
   if (c->noah_arena) (void)hipFree(c->noah_arena);
+  if (c->tile_arena) (void)hipFree(c->tile_arena);
   if (c->nanflag_host) (void)hipHostFree(c->nanflag_host);
   delete c;
 }
+
+extern "C" int pihm_ctx_set_rhs_path(pihm_ctx *c, int path) {
+  if (!c || c->nmem != 1) return PIHM_ERR_ARG;
+  if (path == PIHM_RHS_ONE_LAUNCH && !c->one_launch_ok) return PIHM_ERR_ARG;
+  if (path != PIHM_RHS_ONE_LAUNCH && path != PIHM_RHS_TWO_KERNELS) return PIHM_ERR_ARG;
+  c->rhs_path = path;
+  return PIHM_OK;
+}
+
+extern "C" int pihm_ctx_get_rhs_path(const pihm_ctx *c) { return c ? c->rhs_path : PIHM_ERR_ARG; }
 
 extern "C" int pihm_ctx_set_stream(pihm_ctx *c, void *s) {
   if (!c) return PIHM_ERR_ARG;
@@ -1516,8 +2075,7 @@ extern "C" int pihm_rhs_launch(pihm_ctx *c, const double *y, double *ydot) {
     hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
     hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
   } else {
-    hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-    launch_flux_single(c, a, nbe, nbr);
+    launch_single(c, a);
     if (a.closure) {
       const int nb = std::max(1, std::min(kClosureBlocks, div_up(std::max(c->ne, c->nr), kBlock)));
       hipLaunchKernelGGL(k_closure_part, dim3(nb), dim3(kBlock), 0, c->stream, a);
@@ -1757,9 +2315,7 @@ int pihm_ctx_eval_fluxes(pihm_ctx *c, const double *y, double *ydot, double *car
   a.ydot = ydot;
   a.record = 1;
   a.closure = 0;
-  const int nbe = div_up(c->ne, kBlock), nbr = div_up(c->nr, kBlock);
-  hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-  launch_flux_single(c, a, nbe, nbr);
+  launch_single(c, a);
   PIHM_HIP_TRY(hipGetLastError());
   return pihm_ctx_carried_copy(c, carried_save, 1);
 }
@@ -1784,24 +2340,42 @@ int pihm_ctx_carried_copy(pihm_ctx *c, double *buf, int dir) {
 extern "C" int pihm_ctx_autotune(pihm_ctx *c, const double *y, double *ydot, int reps, int *best_span, double *ms) {
   if (!c || !y || !ydot || reps <= 0 || c->nmem != 1) return PIHM_ERR_ARG;
   static const int spans[PIHM_TUNE_NSPAN] = PIHM_TUNE_SPANS;
+  // candidates: every tile placement on each RHS path the context has
+  const int npath = c->one_launch_ok ? 2 : 1;
+  const int paths[2] = {c->one_launch_ok ? PIHM_RHS_ONE_LAUNCH : PIHM_RHS_TWO_KERNELS, PIHM_RHS_TWO_KERNELS};
+  const int ncand = npath * PIHM_TUNE_NSPAN;
+  std::vector<double> tmin(ncand, HUGE_VAL);
   double *save = nullptr;
   if (c->nr > 0) PIHM_HIP_TRY(hipMalloc(&save, 2L * c->nr * sizeof(double)));
   int rc = pihm_ctx_carried_copy(c, save, 0);
-  int best = c->args.tile_span;
-  double tbest = HUGE_VAL;
-  for (int k = 0; k < PIHM_TUNE_NSPAN && !rc; k++) {
-    c->args.tile_span = spans[k];
-    double t = 0.0, w = 0.0;
-    rc = pihm_ctx_time_rhs(c, y, ydot, 2, &w);
-    if (!rc) rc = pihm_ctx_time_rhs(c, y, ydot, reps, &t);
-    if (ms) ms[k] = t;
-    if (!rc && t < tbest) { tbest = t; best = spans[k]; }
+  // three interleaved rounds, the best of each candidate's rounds
+  for (int round = 0; round < 3 && !rc; round++) {
+    for (int k = 0; k < ncand && !rc; k++) {
+      c->rhs_path = paths[k / PIHM_TUNE_NSPAN];
+      c->args.tile_span = spans[k % PIHM_TUNE_NSPAN];
+      double t = 0.0, w = 0.0;
+      rc = pihm_ctx_time_rhs(c, y, ydot, 2, &w);
+      if (!rc) rc = pihm_ctx_time_rhs(c, y, ydot, reps, &t);
+      if (!rc) tmin[k] = std::min(tmin[k], t);
+    }
   }
-  c->args.tile_span = best;
+  // the default (candidate 0) stays unless another is more than 2% faster:
+  // differences below that are within the spread of repeated timings
+  int best = 0;
+  for (int k = 1; k < ncand; k++)
+    if (tmin[k] < tmin[best]) best = k;
+  if (!(tmin[best] < 0.98 * tmin[0])) best = 0;
+  c->rhs_path = paths[best / PIHM_TUNE_NSPAN];
+  c->args.tile_span = spans[best % PIHM_TUNE_NSPAN];
+  if (ms)
+    for (int k = 0; k < PIHM_TUNE_NSPAN; k++) {
+      ms[k] = tmin[k];
+      for (int q = 1; q < npath; q++) ms[k] = std::min(ms[k], tmin[q * PIHM_TUNE_NSPAN + k]);
+    }
   if (!rc) rc = pihm_ctx_carried_copy(c, save, 1);
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PIHM_ERR_HIP;
   if (save) (void)hipFree(save);
-  if (best_span) *best_span = best;
+  if (best_span) *best_span = c->args.tile_span;
   return rc;
 }
 
@@ -1822,8 +2396,7 @@ extern "C" int pihm_ctx_time_rhs(pihm_ctx *c, const double *y, double *ydot, int
       hipLaunchKernelGGL(k_prep<true>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
       hipLaunchKernelGGL(k_flux<true>, dim3(nbe + nbr), dim3(kBlock), 0, c->stream, a, nbe);
     } else {
-      hipLaunchKernelGGL(k_prep<false>, dim3(nbe + 3 * nbr), dim3(kBlock), 0, c->stream, a, nbe);
-      launch_flux_single(c, a, nbe, nbr);
+      launch_single(c, a);
     }
   }
   PIHM_HIP_TRY(hipGetLastError());

@@ -139,6 +139,7 @@ PIHM_PREC_NONE = 0
 PIHM_PREC_RIGHT = 2
 PIHM_REC_ITERS = 8
 PIHM_NOAH_MAXLYR = 11
+PIHM_RHS_ONE_LAUNCH, PIHM_RHS_TWO_KERNELS = 0, 1     # pihm_ctx_set_rhs_path
 PIHM_TUNE_SPANS = (0, 1, 2, 4, 8, 16, 32, 64)
 PIHM_ENS_TUNE_G = (1, 2, 4, 8)
 
@@ -327,6 +328,8 @@ _PROTOS = [
     ("pihm_run_gpu_producers", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(RunConfig), c_double_p,
                                          C.POINTER(RunStats)]),
     ("pihm_ens_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("pihm_ctx_set_rhs_path", C.c_int, [C.c_void_p, C.c_int]),
+    ("pihm_ctx_get_rhs_path", C.c_int, [C.c_void_p]),
     ("pihm_ctx_set_tile_span", C.c_int, [C.c_void_p, C.c_int]),
     ("pihm_ctx_get_tile_span", C.c_int, [C.c_void_p]),
     ("pihm_ctx_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), c_double_p]),

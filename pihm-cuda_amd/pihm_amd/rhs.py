@@ -18,7 +18,7 @@ def _dp(a):
 
 
 class HipRHS:
-    def __init__(self, mesh, nan_check=True, record=False, closure=False):
+    def __init__(self, mesh, nan_check=True, record=False, closure=False, rhs_path=None):
         self.L = _lib.lib()
         self.mesh = mesh
         self.desc = mesh.desc()
@@ -37,6 +37,16 @@ class HipRHS:
         self.L.pihm_ctx_set_nan_check(ctx, 1 if nan_check else 0)
         self.L.pihm_ctx_set_record(ctx, 1 if record else 0)
         self.record = record
+        if rhs_path is not None:
+            self.set_rhs_path(rhs_path)
+
+    def set_rhs_path(self, path):
+        """_lib.PIHM_RHS_ONE_LAUNCH (the default) or _lib.PIHM_RHS_TWO_KERNELS."""
+        if self.L.pihm_ctx_set_rhs_path(self.ctx, int(path)):
+            raise RuntimeError(f"pihm_ctx_set_rhs_path({path}) failed")
+
+    def rhs_path(self):
+        return self.L.pihm_ctx_get_rhs_path(self.ctx)
 
     def close(self):
         if getattr(self, "ctx", None):

@@ -48,6 +48,7 @@ def test_tile_placement_is_neutral_and_tuner_restores_state(tree_small):
     print("[tune] ms by span", dict(zip(_lib.PIHM_TUNE_SPANS, ms.round(5))), "best", best.value, flush=True)
     assert best.value in _lib.PIHM_TUNE_SPANS and np.all(ms > 0)
     assert h.L.pihm_ctx_get_tile_span(h.ctx) == best.value
+    assert h.rhs_path() in (_lib.PIHM_RHS_ONE_LAUNCH, _lib.PIHM_RHS_TWO_KERNELS)
     assert np.array_equal(h.get_river_edge_ovlflow(), carried)
     rc, d = h.ode(y)
     assert np.array_equal(d, ref)
