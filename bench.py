@@ -59,9 +59,10 @@ def _cfg(t_end):
     return run_config(t_end, graphs=GRAPHS)
 
 
-def gpu_run(mesh, forcing, hours_total, mark_hours):
+def gpu_run(mesh, forcing, hours_total, mark_hours, snap_hour=None):
     """Run the device path for hours_total simulated hours; return wall-clock marks at the
-    requested hour boundaries (device synchronised at each), stats and kernel timings."""
+    requested hour boundaries (device synchronised at each), stats and kernel timings, and
+    (snap_hour) the state at that hour, copied out before that hour's mark is taken."""
     L = _lib.lib()
     desc = mesh.desc()
     ctx = C.c_void_p()
@@ -92,9 +93,14 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     rhs_path = "one launch" if L.pihm_ctx_get_rhs_path(ctx) == _lib.PIHM_RHS_ONE_LAUNCH else "two kernels"
     marks = {}
     want = {int(round(h * 60)) for h in mark_hours}     # in model steps of 60 s
+    snap_k = None if snap_hour is None else int(round(snap_hour * 60))
+    snap = np.empty(mesh.neq)
 
     def step_end(user, t, yv):
         k = int(round(t / 60.0))
+        if k == snap_k:
+            L.pihm_ctx_sync(ctx)
+            L.pihm_ctx_state_from_device(ctx, y, snap.ctypes.data_as(_lib.c_double_p))
         if k in want:
             L.pihm_ctx_sync(ctx)
             marks[k / 60.0] = time.perf_counter()
@@ -124,7 +130,7 @@ def gpu_run(mesh, forcing, hours_total, mark_hours):
     L.pihm_ctx_destroy(ctx)
     L.pihm_stream_destroy(stream)
     return dict(rc=rc, nan=nan, marks=marks, stats=st.as_dict(), kernel_ms=[x.value for x in t2],
-                ms_eval=ms_eval.value, yend=yend,
+                ms_eval=ms_eval.value, yend=yend, snap=snap if snap_k is not None else None,
                 tuning={"tile_span": span.value,
                         "rhs_path": rhs_path,
                         "rhs_ms_by_span": dict(zip(_lib.PIHM_TUNE_SPANS, span_ms.round(6).tolist()))})
@@ -516,14 +522,53 @@ def flux_pihm_run(seed, hours=24.0, cpu_hours=0.25, nthreads=0, iters=50):
     return out
 
 
-def cpu_run(mesh, forcing, hours, nthreads):
+def fp64_valu(path, rhs_ms):
+    """Share of the SIMDs' issue cycles the RHS's FP64 VALU instructions occupy during one
+    evaluation (instruction counts from the profile, time from the live measurement)."""
+    cycles = sum(insts * waves * 4.0 for insts, waves in RHS_VALU[path]) / SIMDS
+    return {"issue_cycles_per_simd": cycles, "eval_cycles": rhs_ms * 1e3 * CLOCK_MHZ,
+            "issue_frac": cycles / (rhs_ms * 1e3 * CLOCK_MHZ),
+            "insts_per_wave": [x for x, _ in RHS_VALU[path]], "waves": [w for _, w in RHS_VALU[path]]}
+
+
+def cpu_run(mesh, forcing, hours, nthreads, y0=None, t0=0.0):
+    """The oracle's host run (same integrator and driver, host vectors, OpenMP nthreads)
+    from state y0 at time t0 over `hours`; wall = the driver's own clock."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     run = O.OracleRun(mesh, forcing, nthreads=nthreads)
-    cfg = _cfg(hours * 3600.0)
-    t0 = time.perf_counter()
-    rc, y, st = run.run(mesh.initial_state(), cfg)
-    return dict(rc=rc, wall=time.perf_counter() - t0, stats=st.as_dict(), yend=y)
+    cfg = run_config(t0 + hours * 3600.0, t_start=t0, graphs=GRAPHS)
+    rc, y, st = run.run(mesh.initial_state() if y0 is None else y0, cfg)
+    return dict(rc=rc, wall=st.wall_s, stats=st.as_dict(), yend=y)
+
+
+def gpu_window(mesh, forcing, y0, t0, hours):
+    """The device run (pihm_run_gpu) from state y0 at time t0 over `hours`: the driver's clock."""
+    from pihm_amd.rhs import run_gpu
+    rc, _, st = run_gpu(mesh, forcing, run_config(t0 + hours * 3600.0, t_start=t0, graphs=GRAPHS), y0)
+    if rc != 0:
+        raise RuntimeError(f"device window run failed: rc={rc}")
+    return st.wall_s
+
+
+def physical_cores():
+    """Distinct (package, core) pairs among the CPUs this process may run on."""
+    seen = set()
+    for cpu in os.sched_getaffinity(0):
+        base = f"/sys/devices/system/cpu/cpu{cpu}/topology/"
+        try:
+            seen.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            return None
+    return len(seen)
+
+
+# FP64 VALU issue: instructions per wavefront and wavefronts per RHS evaluation at the
+# 200,000-triangle workload (SQ_INSTS_VALU / SQ_WAVES, the same profile passes as the traffic).
+# A wavefront's FP64 instruction occupies a SIMD for 4 cycles (16 lanes per cycle); the chip
+# has 1,024 SIMDs at 2.4 GHz.
+RHS_VALU = {"one launch": [(1911.3, 3381)], "two kernels": [(268.1, 3320), (1665.4, 3192)]}
+SIMDS, CLOCK_MHZ = 1024, 2400.0
 
 
 def profile_kernels(iters, seed):
@@ -606,7 +651,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12, help="timed simulated hours")
     ap.add_argument("--warmup", type=int, default=1, help="untimed simulated hours")
-    ap.add_argument("--cpu-hours", type=float, default=0.25, help="CPU baseline sample (simulated hours)")
+    ap.add_argument("--cpu-hours", type=float, default=0.25, help="CPU baseline sample, all cores (simulated hours)")
+    ap.add_argument("--cpu1-hours", type=float, default=0.05, help="CPU baseline sample, one thread (simulated hours)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
@@ -650,7 +696,7 @@ def main():
     global GRAPHS
     GRAPHS = 0 if args.no_graphs else 1
     _log(f"device run: {W} + {K} simulated hours at {mesh.nelem} triangles")
-    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K, S])
+    g = gpu_run(mesh, forcing, W + K, mark_hours=[W, W + K], snap_hour=W)
     _log("device run done")
     if g["rc"] != 0 or g["nan"] != 0:
         raise RuntimeError(f"device run failed: rc={g['rc']} nan={g['nan']} {g['stats']}")
@@ -680,25 +726,44 @@ def main():
                    "nelem": mesh.nelem, "nriver": mesh.nriver, "neq": mesh.neq},
         "solver": {k: st[k] for k in ("nst", "nfe", "nfes", "nni", "nli", "ncfn", "netf")},
         "rhs": {"ms_per_eval": rhs_ms, "evals_per_s": 1e3 / rhs_ms,
+                "fp64_valu": fp64_valu(g["tuning"]["rhs_path"], rhs_ms),
                 "two_kernel_ms_each_timed_alone": {"k_prep": ms_prep, "k_flux": ms_flux}, "tuning": g["tuning"]},
         "roofline": {"kernel": RHS_KERNELS[g["tuning"]["rhs_path"]],
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": RHS_TRAFFIC_BYTES[g["tuning"]["rhs_path"]],
                      "algorithmic_bytes": alg_bytes},
     }
+    ms_ls, ms_dot = C.c_double(), C.c_double()
+    assert _lib.lib().pihm_nv_time_ops(mesh.neq, 200, C.byref(ms_ls), C.byref(ms_dot)) == 0
+    out["nvector"] = {"n": mesh.neq, "linear_sum_us": ms_ls.value * 1e3,
+                      "linear_sum_GBps": 24.0 * mesh.neq / (ms_ls.value * 1e-3) / 1e9,
+                      "linear_sum_frac_of_hbm_peak": 24.0 * mesh.neq / (ms_ls.value * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "dot_prod_us_per_call": ms_dot.value * 1e3,
+                      "note": "N_VLinearSum_Hip (24 B per entry) and N_VDotProd_Hip (both passes and the "
+                              "host's wait) on C1-length vectors, HIP events over 200 calls each"}
     if not args.no_cpu:
         nthreads = args.cpu_threads or len(os.sched_getaffinity(0))
-        _log(f"CPU baseline: {S} simulated hours, {nthreads} threads")
-        c = cpu_run(mesh, forcing, S, nthreads)
-        cpu_rate = (S / 24.0) / c["wall"]
-        out["cpu_baseline"] = {"value": cpu_rate, "unit": "sim-days/s", "cores": nthreads, "kind": "port",
-                               "sample": f"first {S} simulated hours of the same workload (oracle RHS, "
-                                         f"same integrator, host vectors, OpenMP {nthreads} threads)",
-                               "wall_s": c["wall"]}
-        if S in g["marks"]:
-            gs = g["marks"][S] - g["marks"][0.0]
-            out["cpu_baseline"]["gpu_same_window"] = (S / 24.0) / gs
-            out["cpu_baseline"]["gpu_over_cpu_same_window"] = ((S / 24.0) / gs) / cpu_rate
+        t0 = W * 3600.0
+        legs = {}
+        for nth, hrs in ((nthreads, S), (1, args.cpu1_hours)):
+            _log(f"CPU baseline: hours {W} to {W} + {hrs}, {nth} threads, from the device state at hour {W}")
+            c = cpu_run(mesh, forcing, hrs, nth, y0=g["snap"], t0=t0)
+            if c["rc"] != 0:
+                raise RuntimeError(f"CPU baseline run failed: rc={c['rc']}")
+            gw = gpu_window(mesh, forcing, g["snap"], t0, hrs)
+            rate, grate = (hrs / 24.0) / c["wall"], (hrs / 24.0) / gw
+            legs[nth] = {"value": rate, "unit": "sim-days/s", "threads": nth,
+                         "sample": f"simulated hours {W} to {W + hrs:g}, host and device both restarted "
+                                   f"from the device's state at hour {W} (the start of the timed window)",
+                         "wall_s": c["wall"], "gpu_same_window": grate, "gpu_over_cpu_same_window": grate / rate}
+        allc = legs[nthreads]
+        out["cpu_baseline"] = {"value": allc["value"], "unit": "sim-days/s", "cores": nthreads,
+                               "physical_cores": physical_cores(), "kind": "port",
+                               "sample": allc["sample"] + f"; oracle RHS, same integrator and driver, host "
+                                                          f"vectors, OpenMP {nthreads} threads",
+                               "wall_s": allc["wall_s"], "gpu_same_window": allc["gpu_same_window"],
+                               "gpu_over_cpu_same_window": allc["gpu_over_cpu_same_window"],
+                               "one_thread": legs[1]}
     _log("producers")
     out["producers"] = producers_run(mesh, args.seed)
     if not args.no_noah:

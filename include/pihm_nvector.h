@@ -150,6 +150,13 @@ int      N_VDotProdMulti_Hip(int nvec, N_Vector x, N_Vector *Y, double *d);
 double   N_VLinearSumWrms_Hip(double a, N_Vector x, double b, N_Vector y,
                               N_Vector z, N_Vector w);
 
+/* Measurement, no counterpart in the reference: average device time [ms] of
+ * N_VLinearSum_Hip (z = 1.5 x + 0.5 y: 24 bytes per entry) and of
+ * N_VDotProd_Hip (its two passes and the host's wait for the result) on new
+ * vectors of length n, each over `iters` calls between HIP events on the
+ * vectors' stream.  Returns 0, -1 for bad arguments, -2 on a runtime error. */
+int      pihm_nv_time_ops(long int n, int iters, double *ms_linearsum, double *ms_dotprod);
+
 /* No counterpart in the reference or in SUNDIALS.  For callers that cannot
  * allocate device or pinned memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc): N_VDotProd_Hip on raw

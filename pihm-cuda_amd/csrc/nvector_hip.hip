@@ -683,6 +683,42 @@ int pihm_nv_dotprod_noalloc(long int n, const double *x_dev, const double *y_dev
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int pihm_nv_time_ops(long int n, int iters, double *ms_linearsum, double *ms_dotprod) {
+  if (n <= 0 || iters <= 0 || !ms_linearsum || !ms_dotprod) return -1;
+  N_Vector x = N_VNew_Hip(n), y = N_VNew_Hip(n), z = N_VNew_Hip(n);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = (x && y && z) ? 0 : -2;
+  for (auto &e : ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = -2;
+  if (!rc) {
+    const hipStream_t st = (hipStream_t)pihm_nv_stream(x);
+    N_VConst_Hip(1.0, x);
+    N_VConst_Hip(2.0, y);
+    N_VLinearSum_Hip(1.5, x, 0.5, y, z);
+    double acc = N_VDotProd_Hip(x, y);
+    float ms = 0.0f;
+    if (hipEventRecord(ev[0], st) != hipSuccess) rc = -2;
+    for (int k = 0; k < iters && !rc; k++) N_VLinearSum_Hip(1.5, x, 0.5, y, z);
+    if (!rc && (hipEventRecord(ev[1], st) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+      rc = -2;
+    if (!rc) *ms_linearsum = ms / iters;
+    if (!rc && hipEventRecord(ev[0], st) != hipSuccess) rc = -2;
+    for (int k = 0; k < iters && !rc; k++) acc += N_VDotProd_Hip(x, y);
+    if (!rc && (hipEventRecord(ev[1], st) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+      rc = -2;
+    if (!rc) *ms_dotprod = ms / iters;
+    if (!rc && acc != 2.0 * (double)n * (iters + 1)) rc = -2;   // every sum is exact here
+  }
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  N_VDestroy_Hip(x);
+  N_VDestroy_Hip(y);
+  N_VDestroy_Hip(z);
+  return rc;
+}
+
 }  // extern "C"
 
 namespace {

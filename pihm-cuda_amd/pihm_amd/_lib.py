@@ -249,6 +249,7 @@ _PROTOS = [
     ("N_VInv_Hip", None, [C.c_void_p, C.c_void_p]),
     ("N_VAddConst_Hip", None, [C.c_void_p, C.c_double, C.c_void_p]),
     ("N_VDotProd_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
+    ("pihm_nv_time_ops", C.c_int, [C.c_long, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("pihm_nv_dotprod_noalloc", C.c_int, [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("N_VMaxNorm_Hip", C.c_double, [C.c_void_p]),
     ("N_VWrmsNorm_Hip", C.c_double, [C.c_void_p, C.c_void_p]),
