@@ -151,11 +151,13 @@ def ensemble_run(nmem, hours, sample, seed):
     for k, m in enumerate(meshes):
         e.set_state(k, m.initial_state())
     tuning = e.autotune(10)          # launch shapes, before the timed run
+    ops0 = e.op_stats()
     t0 = time.perf_counter()
     nfail = e.run(hours * 3600.0)
     wall = time.perf_counter() - t0
     st = [e.stats(k) for k in range(nmem)]
     cnt = e.counters()
+    ops1 = e.op_stats()
     e.close()
     member_days = nmem * hours / 24.0
     # one at a time: every (nmem/sample)-th member in size order
@@ -190,7 +192,19 @@ def ensemble_run(nmem, hours, sample, seed):
                                  f"{nmem}/{len(pick)}",
                        "sample_wall_s": float(np.sum(single_wall))},
         speedup_vs_one_at_a_time=(member_days / wall) / (member_days / est_single_total),
-        tuning=tuning)
+        tuning=tuning, ops_kernel=_ops_roofline(ops0, ops1))
+
+
+def _ops_roofline(a, b):
+    """k_ens_ops over the timed run: launches timed with HIP events, algorithmic bytes per
+    launch, device time per launch and the achieved bandwidth against the HBM peak."""
+    n, by, ms = b["launches"] - a["launches"], b["bytes"] - a["bytes"], b["ms"] - a["ms"]
+    if n <= 0 or ms <= 0.0:
+        return None
+    gbps = by / (ms * 1e-3) / 1e9
+    return {"kernel": "k_ens_ops (every ready member's queued vector operations, one launch)",
+            "launches_timed": n, "bytes_per_launch": by / n, "us_per_launch": ms * 1e3 / n,
+            "achieved": gbps, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbps / HBM_PEAK_GBS}
 
 
 SERVE_CLASSES = [(8192, 48), (32768, 32), (131072, 12)]   # slot capacity (elements) x count

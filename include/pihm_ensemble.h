@@ -68,6 +68,13 @@ int  pihm_ens_member_stats(pihm_ensemble *e, int k, pihm_run_stats *st);
  * right-hand-side evaluations since creation. */
 int  pihm_ens_counters(pihm_ensemble *e, long *ticks, long *op_launches, long *rhs_launches);
 
+/* Measurement, no counterpart in the reference: the vector-operation kernel's
+ * launches timed with HIP events since creation (up to 8 per tick, recorded on
+ * the ensemble's stream around each launch and read after the tick's wait),
+ * their total device time [ms] and their algorithmic bytes (8 bytes per entry
+ * of every vector each queued operation reads or writes). */
+int  pihm_ens_op_stats(pihm_ensemble *e, long *launches, double *bytes, double *ms);
+
 /* Autotuner of the launch shapes, choices the results do not depend on:
  * the placement of the batched RHS's element tiles over the XCDs (the
  * candidates of PIHM_TUNE_SPANS, pihm_ctx_set_tile_span) and the number of

@@ -104,7 +104,9 @@ struct Round {
 // The round's operations (nblocks = chunks of all participants), then the
 // per-member totals of its reductions (a second launch, one workgroup per
 // member with a reduction).
-void launch_ops(const Dev &d, const Round &r, hipStream_t st);
+// after_ops, when not null, is recorded between the operation kernel and the
+// reduction kernel (timing of the operation kernel alone).
+void launch_ops(const Dev &d, const Round &r, hipStream_t st, hipEvent_t after_ops = nullptr);
 constexpr int kTuneNG = 4;
 constexpr int kTuneG[kTuneNG] = {1, 2, 4, 8};
 // One-lane kernel writing ++(*ctr) to *flag (host memory) after the work queued before it.

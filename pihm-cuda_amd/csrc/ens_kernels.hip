@@ -489,9 +489,10 @@ void launch_signal(unsigned *ctr, volatile unsigned *flag, hipStream_t st) {
   hipLaunchKernelGGL(k_ens_signal, dim3(1), dim3(64), 0, st, ctr, flag);
 }
 
-void launch_ops(const Dev &d, const Round &r, hipStream_t st) {
+void launch_ops(const Dev &d, const Round &r, hipStream_t st, hipEvent_t after_ops) {
   const int g = r.g > 0 ? r.g : 1;
   hipLaunchKernelGGL(k_ens_ops, dim3((r.nchunks + g - 1) / g), dim3(kChunkThreads), 0, st, d, r);
+  if (after_ops) (void)hipEventRecord(after_ops, st);
   if (r.nred > 0) hipLaunchKernelGGL(k_ens_final, dim3(r.nred), dim3(kChunkThreads), 0, st, d, r);
 }
 

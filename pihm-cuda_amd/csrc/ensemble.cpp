@@ -716,6 +716,12 @@ struct pihm_ensemble {
   std::vector<long> k_end;     // model steps each member runs to
   std::vector<Task<int>> tops; // running model-step loops
   long ticks = 0, op_launches = 0, rhs_launches = 0;
+  // the operation kernel's launches timed with HIP events (the first kEvPairs
+  // of each tick) and their algorithmic bytes
+  static constexpr int kEvPairs = 8;
+  hipEvent_t ev_ops[2 * kEvPairs] = {};
+  long ops_timed = 0;
+  double ops_bytes = 0.0, ops_ms = 0.0;
   unsigned *sig_ctr = nullptr;           // device tick count
   volatile unsigned *sig_flag = nullptr; // pinned host copy
   unsigned sig_expect = 0;
@@ -753,7 +759,33 @@ struct RoundPlan {
   size_t off_part, off_cpre, off_obeg, off_ops, off_io, off_rmem, off_rinfo, off_bm, off_bmm;
   int np, nblocks, nred;
   int nact, nbe, nbr;
+  double bytes;      // algorithmic bytes of the round's operations
 };
+
+// Algorithmic bytes of operation o over member M: the vectors it reads and
+// writes, 8 bytes per entry each (what k_ens_ops loads and stores).
+double op_bytes(const Op &o, const MemberInfo &M) {
+  const double n = 8.0 * M.neq, ne = (double)M.ne;
+  switch (o.kind) {
+    case OP_LIN: return 3 * n;
+    case OP_SCALE: return 2 * n;
+    case OP_CONST: return n;
+    case OP_PREDICT: return n * (2 * o.a0 + 1 + ((o.a1 & 1) ? 1 : 0) + ((o.a1 & 2) ? 2 : 0));
+    case OP_RESTORE: return n * (2 * o.a0 + 1);
+    case OP_RESCALE: return n * (2 * o.a0);
+    case OP_CORRECT: return n * (1 + 2 * (o.a0 + 1));
+    case OP_GM_START: return 6 * n;
+    case OP_GM_PREP: return n * (o.a0 + 5);
+    case OP_GM_ARNOLDI: return n * (o.a0 + 5);
+    case OP_GM_ORTH: return n * (o.a0 + 3);
+    case OP_GM_FINISH: return n * ((o.a0 == 0 ? o.a1 + 3 : (o.a0 == 1 ? 4 : 3)) + 2);
+    case OP_ORDSEL: return n * (1 + ((o.a1 & 1) ? 1 : 0) + ((o.a1 & 2) ? 3 : 0));
+    case OP_DKY: return n * (o.a0 + 2);
+    case OP_FORCING: return 28.0 * ne;   // station index, three step inputs written
+    case OP_WS0: return 16.0 * ne;
+    default: return 0.0;
+  }
+}
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
@@ -771,6 +803,7 @@ int pihm_ensemble::flush() {
     // vector operations of every member whose next item is one
     std::vector<int> part, cpre{0}, obeg{0}, rmem, rinfo;
     std::vector<Op> ops;
+    double round_bytes = 0.0;
     for (int m = 0; m < nmem; m++) {
       auto &Q = mem[m].q_items;
       if (pos[m] >= Q.size() || Q[pos[m]].rhs) continue;
@@ -787,6 +820,7 @@ int pihm_ensemble::flush() {
           red = true;
         }
         ops.push_back(Q[pos[m]].op);
+        round_bytes += op_bytes(Q[pos[m]].op, info[m]);
         pos[m]++;
       }
       obeg.push_back((int)ops.size());
@@ -796,6 +830,7 @@ int pihm_ensemble::flush() {
     if (!part.empty()) {
       RoundPlan r{};
       r.rhs = false;
+      r.bytes = round_bytes;
       r.np = (int)part.size();
       r.nblocks = cpre.back();
       r.off_part = reserve(part.size() * sizeof(int));
@@ -858,6 +893,8 @@ int pihm_ensemble::flush() {
   if (rc) return rc;
   std::memcpy(stage_h, buf.data(), buf.size());
   PIHM_HIP_TRY(hipMemcpyAsync(stage_d, stage_h, buf.size(), hipMemcpyHostToDevice, st));
+  int ntimed = 0;
+  double timed_bytes[kEvPairs];
   for (const auto &r : plans) {
     if (r.rhs) {
       rc = pihm_rhs_launch_compact(ctx, (const MemberIO *)(stage_d + r.off_io), mnan,
@@ -877,8 +914,11 @@ int pihm_ensemble::flush() {
       rd.nred = r.nred;
       rd.nchunks = r.nblocks;
       rd.g = ops_g;
-      launch_ops(dev, rd, st);
+      const bool timed = ntimed < kEvPairs && ev_ops[2 * ntimed];
+      if (timed) PIHM_HIP_TRY(hipEventRecord(ev_ops[2 * ntimed], st));
+      launch_ops(dev, rd, st, timed ? ev_ops[2 * ntimed + 1] : nullptr);
       PIHM_HIP_TRY(hipGetLastError());
+      if (timed) timed_bytes[ntimed++] = r.bytes;
       op_launches++;
     }
   }
@@ -897,6 +937,14 @@ int pihm_ensemble::flush() {
     __builtin_ia32_pause();
   }
   std::atomic_thread_fence(std::memory_order_acquire);
+  for (int k = 0; k < ntimed; k++) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev_ops[2 * k], ev_ops[2 * k + 1]) == hipSuccess) {
+      ops_ms += ms;
+      ops_bytes += timed_bytes[k];
+      ops_timed++;
+    }
+  }
   ticks++;
   return PIHM_OK;
 }
@@ -1008,6 +1056,8 @@ int ens_alloc(pihm_ctx *c, const int *nrec_cap, const int *nst_cap, const pihm_r
   std::memset(e->result, 0, (size_t)nmem * kRes * sizeof(double));
   if (hipHostMalloc((void **)&e->mnan, nmem * sizeof(int), hipHostMallocCoherent) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMalloc((void **)&e->sig_ctr, sizeof(unsigned)) != hipSuccess) return fail(PIHM_ERR_HIP);
+  for (auto &ev : e->ev_ops)
+    if (hipEventCreate(&ev) != hipSuccess) return fail(PIHM_ERR_HIP);
   if (hipMemset(e->sig_ctr, 0, sizeof(unsigned)) != hipSuccess) return fail(PIHM_ERR_HIP);
   {
     unsigned *f = nullptr;
@@ -1191,6 +1241,8 @@ void pihm_ens_destroy(pihm_ensemble *e) {
   if (e->mnan) (void)hipHostFree(e->mnan);
   if (e->sig_ctr) (void)hipFree(e->sig_ctr);
   if (e->sig_flag) (void)hipHostFree((void *)e->sig_flag);
+  for (auto &ev : e->ev_ops)
+    if (ev) (void)hipEventDestroy(ev);
   (void)hipFree(e->info_dev);
   (void)hipFree(e->station);
   (void)hipFree(e->prcp);
@@ -1428,6 +1480,14 @@ int pihm_ens_member_stats(pihm_ensemble *e, int k, pihm_run_stats *s) {
   std::memset(s, 0, sizeof(*s));
   s->nst = m.nst; s->nfe = m.nfe; s->nfes = m.nfes; s->nni = m.nni; s->nli = m.nli;
   s->ncfn = m.ncfn; s->netf = m.netf; s->model_steps = m.model_steps; s->flag = m.flag;
+  return PIHM_OK;
+}
+
+int pihm_ens_op_stats(pihm_ensemble *e, long *launches, double *bytes, double *ms) {
+  if (!e) return PIHM_ERR_ARG;
+  if (launches) *launches = e->ops_timed;
+  if (bytes) *bytes = e->ops_bytes;
+  if (ms) *ms = e->ops_ms;
   return PIHM_OK;
 }
 

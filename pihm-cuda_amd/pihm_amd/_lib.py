@@ -313,6 +313,7 @@ _PROTOS = [
     ("pihm_ens_run", C.c_int, [C.c_void_p, C.c_double]),
     ("pihm_ens_member_stats", C.c_int, [C.c_void_p, C.c_int, C.POINTER(RunStats)]),
     ("pihm_ens_counters", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("pihm_ens_op_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("pihm_ens_create_slots", C.c_int, [C.c_void_p, c_int32_p, c_int32_p, C.POINTER(RunConfig), C.POINTER(C.c_void_p)]),
     ("pihm_ens_load_member", C.c_int, [C.c_void_p, C.c_int, C.POINTER(MeshDesc), C.POINTER(Forcing), c_double_p,
                                        C.c_double]),

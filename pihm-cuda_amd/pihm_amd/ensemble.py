@@ -84,6 +84,13 @@ class Ensemble:
         self.L.pihm_ens_counters(self.ens, C.byref(a), C.byref(b), C.byref(c))
         return dict(ticks=a.value, op_launches=b.value, rhs_launches=c.value)
 
+    def op_stats(self):
+        """The vector-operation kernel's timed launches so far: count, algorithmic bytes,
+        device ms (pihm_ens_op_stats)."""
+        n, b, ms = C.c_long(), C.c_double(), C.c_double()
+        self.L.pihm_ens_op_stats(self.ens, C.byref(n), C.byref(b), C.byref(ms))
+        return dict(launches=n.value, bytes=b.value, ms=ms.value)
+
     def close(self):
         if getattr(self, "ens", None):
             self.L.pihm_ens_destroy(self.ens)
