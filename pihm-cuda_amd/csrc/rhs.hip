@@ -239,6 +239,7 @@ __device__ __forceinline__ double effkh_of(const RhsArgs &a, const Ix &ix, int q
 //   TileSrc:   the tile's records in LDS (codes: LDS slots) and the bank
 //              fluxes evaluated here, from the element's own record.
 struct GlobalSrc {
+  static constexpr bool kGatherFirst = true;   // neighbour records from HBM: all three loads at once
   const RhsArgs &a;
   int p;
   __device__ __forceinline__ NbrRec self() const { return a.e.rec[p]; }
@@ -271,6 +272,7 @@ struct LdsRecs {
 };
 
 struct TileSrc {
+  static constexpr bool kGatherFirst = false;  // from LDS: each edge's record when its turn comes
   const RhsArgs &a;
   LdsRecs srec;         // LDS records of the tile and its halo
   const int32_t *lnb;   // neighbour slots of the tile's elements
@@ -364,9 +366,11 @@ __device__ __forceinline__ LatOut elem_lateral(const RhsArgs &a, int p, const Ix
                                                const ElemIn &in, const NbrRec &me) {
   const int ne = a.ne;
   NbrRec nrec[3];
+  if constexpr (Src::kGatherFirst) {
 #pragma unroll
-  for (int j = 0; j < 3; j++)
-    if (in.c[j] >= 0) nrec[j] = src.nbr(in.c[j]);
+    for (int j = 0; j < 3; j++)
+      if (in.c[j] >= 0) nrec[j] = src.nbr(in.c[j]);
+  }
   const double area = in.area, zmax = in.zmax, zmin = in.zmin;
   const double g = me.gw;
   const double surfh = me.surfh;
@@ -379,6 +383,7 @@ __device__ __forceinline__ LatOut elem_lateral(const RhsArgs &a, int p, const Ix
     const int c = in.c[j];
     bnd[j] = nb_is_bound(c);
     if (c >= 0) {
+      if constexpr (!Src::kGatherFirst) nrec[j] = src.nbr(c);
       const NbrRec &nr = nrec[j];
       // SubFlowElemToElem: src/lat_flow.c:285-309
       {
@@ -662,6 +667,37 @@ __device__ __forceinline__ double sub_r2r(const RhsArgs &a, const double *yg, in
   return sub_r2r_k(R, yg, u, d, effk, effk_nabr);
 }
 
+// One segment's side of ChanFlowRiverToRiver (src/river_flow.c:257-300).
+struct ChanSide {
+  double stage, zbed, total_h, perim, crossa, rough, length;
+};
+
+__device__ __forceinline__ ChanSide chan_side(const RiverDev &R, const double *ys, int r) {
+  ChanSide c;
+  c.stage = clamp0(ys[r]);
+  c.zbed = R.zbed[r];
+  c.total_h = c.stage + c.zbed;
+  c.perim = riv_perim(R.ord[r], c.stage, R.coeff[r]);
+  c.crossa = riv_cs_area(R.ord[r], c.stage, R.coeff[r]);
+  c.rough = R.rough[r];
+  c.length = R.length[r];
+  return c;
+}
+
+// ChanFlowRiverToRiver: channel flow from segment u into its downstream
+// segment d, from their two sides.
+__device__ __forceinline__ double chan_r2r(int riv_mode, const ChanSide &u, const ChanSide &d) {
+  const double avg_perim = (u.perim + d.perim) / 2.0;
+  const double avg_rough = (u.rough + d.rough) / 2.0;
+  const double distance = 0.5 * (u.length + d.length);
+  const double diff_h = (riv_mode == KINEMATIC) ? (u.zbed - d.zbed) : (u.total_h - d.total_h);
+  const double grad_h = diff_h / distance;
+  const double avg_sf = (grad_h > 0.0) ? grad_h : RIVGRADMIN;
+  const double avg_crossa = 0.5 * (u.crossa + d.crossa);
+  const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
+  return overland_flow(avg_h, grad_h, avg_sf, u.crossa, avg_rough);
+}
+
 // Segment r's channel outflow (rivflow 1: to the downstream segment or through
 // the outlet), the boundary flux of its upstream node (upinit) and its channel
 // leakage (rivflow 6): src/river_flow.c:10-87, 257-300, 333-427, 500-518.
@@ -696,23 +732,7 @@ __device__ __forceinline__ void river_out(const RhsArgs &a, const double *ys, co
       }
       upinit += flux;
     }
-    // ChanFlowRiverToRiver: src/river_flow.c:257-300
-    const double dstage = clamp0(ys[d]);
-    const double total_h = stage + zbed;
-    const double perim = riv_perim(ord, stage, coeff);
-    const double total_h_down = dstage + R.zbed[d];
-    const double perim_down = riv_perim(R.ord[d], dstage, R.coeff[d]);
-    const double avg_perim = (perim + perim_down) / 2.0;
-    const double avg_rough = (R.rough[r] + R.rough[d]) / 2.0;
-    const double distance = 0.5 * (length + R.length[d]);
-    const double diff_h = (a.riv_mode == KINEMATIC) ? (zbed - R.zbed[d]) : (total_h - total_h_down);
-    const double grad_h = diff_h / distance;
-    const double avg_sf = (grad_h > 0.0) ? grad_h : RIVGRADMIN;
-    const double crossa = riv_cs_area(ord, stage, coeff);
-    const double crossa_down = riv_cs_area(R.ord[d], dstage, R.coeff[d]);
-    const double avg_crossa = 0.5 * (crossa + crossa_down);
-    const double avg_h = (avg_perim == 0.0) ? 0.0 : (avg_crossa / avg_perim);
-    rf1 = overland_flow(avg_h, grad_h, avg_sf, crossa, avg_rough);
+    rf1 = chan_r2r(a.riv_mode, chan_side(R, ys, r), chan_side(R, ys, d));
   } else {
     // OutletFlux: src/river_flow.c:333-390
     double q = 0.0;
@@ -915,11 +935,10 @@ __device__ __forceinline__ void river_parts(const RhsArgs &a, int rb, double *xr
         rf9 = sub_r2r_k(R, yg, r, dcode, effk_r, effk_d);
       }
       double up = ui, upa = 0.0;
+      const ChanSide me = chan_side(R, ys, r);
       for (int kk = R.up_beg[r]; kk < R.up_end[r]; kk++) {
         const int u = R.up_idx[kk];
-        double vi, v1, v6;
-        river_out(a, ys, yg, u, vi, v1, v6);
-        up -= v1;
+        up -= chan_r2r(a.riv_mode, chan_side(R, ys, u), me);   // u's channel outflow: its downstream segment is r
         const double effk_u = 0.5 * (effkh_of<false>(a, ix, R.left[u]) + effkh_of<false>(a, ix, R.right[u]));
         upa -= sub_r2r_k(R, yg, u, r, effk_u, effk_r);
       }
@@ -1006,15 +1025,18 @@ __device__ __forceinline__ double tile_sfmag(const RhsArgs &a, int s, long q, co
 // elements' fluxes read: the heads of the tile and of its halo (the first
 // ring of outside neighbours and their outside neighbours), and the records
 // of the tile and the first ring, whose FrictSlope gradients need those heads;
-// then it runs the element fluxes from them (elem_core with TileSrc).  Every
-// global load a thread makes is issued at the start (its element's inputs, and
-// the inputs of the halo entry it stages), so a block waits for memory about
-// twice, and the barriers wait only for LDS work.  The halo's records are
-// evaluated again by every tile that borders them, from the same operands:
-// every output keeps one writer, and the arithmetic is that of k_prep +
-// k_flux, bit for bit.
+// then it runs the element fluxes from them (elem_core with TileSrc).  The
+// loads for the staging (its element's and its halo entry's record inputs)
+// are issued at the start, the element's flux inputs after the barriers, and
+// each edge reads its neighbour's record from LDS when its turn comes: the
+// kernel stays within 128 VGPRs, four wavefronts per SIMD, so that a
+// 200,000-triangle RHS (3,125 tiles and 256 river blocks) runs in one round
+// of 4,096 wavefront slots instead of spilling 309 blocks into a second round
+// of 3,072.  The halo's records are evaluated again by every tile that borders
+// them, from the same operands: every output keeps one writer, and the
+// arithmetic is that of k_prep + k_flux, bit for bit.
 template <bool N>
-__global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, int nbe) {
+__global__ __launch_bounds__(kTile, 4) void k_rhs_tile(RhsArgs a, TileArgs t, int nbe) {
   extern __shared__ double tile_lds[];
   const LdsRecs srec{tile_lds, t.s1cap};
   double *shead = tile_lds + 8 * t.s1cap;
@@ -1033,19 +1055,23 @@ __global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, in
   const Ix ix = ix_elem<false>(a, 0);
   const TileSrc src{a, srec, t.lnb, p, i};
 
-  // own element: everything it needs in all three phases
+  // own element: what its record and FrictSlope gradient need (the flux
+  // inputs are loaded after the barriers, so that they do not hold registers
+  // through the staging)
   const bool own = p < ne;
-  ElemIn in;
-  double ys = 0.0, yg = 0.0, kmach = 0.0, areafv = 0.0, ksath = 0.0, rough = 0.0;
+  double ys = 0.0, yg = 0.0, zmax = 0.0, zmin = 0.0, depth = 0.0, dmac = 0.0;
+  double kmach = 0.0, areafv = 0.0, ksath = 0.0, rough = 0.0;
+  int oc[3] = {0, 0, 0};
   double nx[3] = {0.0, 0.0, 0.0}, ny[3] = {0.0, 0.0, 0.0};
   if (own) {
-    in = load_elem_in<N>(a, p, ix, src);
+    zmax = e.zmax[p]; zmin = e.zmin[p]; depth = e.depth[p]; dmac = e.dmac[p];
     ys = y[p];
     yg = y[2 * ne + p];
     kmach = e.kmach[p]; areafv = e.areafv[p]; ksath = e.ksath[p]; rough = e.rough[p];
     if (diff) {
 #pragma unroll
       for (int j = 0; j < 3; j++) {
+        oc[j] = src.code(j);
         nx[j] = e.nabr_x[j * ne + p];
         ny[j] = e.nabr_y[j * ne + p];
       }
@@ -1080,7 +1106,7 @@ __global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, in
 
   // records and heads
   if (own) {
-    const NbrRec r = tile_rec(ys, yg, in.zmax, in.zmin, in.depth, in.dmac, kmach, areafv, ksath, rough);
+    const NbrRec r = tile_rec(ys, yg, zmax, zmin, depth, dmac, kmach, areafv, ksath, rough);
     srec.put(i, r);
     shead[i] = r.head;
   }
@@ -1106,7 +1132,7 @@ __global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, in
   }
   __syncthreads();
   if (diff) {
-    if (own) srec.put_sfmag(i, tile_sfmag(a, i, p, in.c, nx, ny, shead));
+    if (own) srec.put_sfmag(i, tile_sfmag(a, i, p, oc, nx, ny, shead));
     if (h1) srec.put_sfmag(kTile + i, tile_sfmag(a, kTile + i, hq, hc, hnx, hny, shead));
     for (int h = i + kTile; h < n1; h += kTile) {
       const long q = t.hidx[hb + h];
@@ -1123,7 +1149,7 @@ __global__ __launch_bounds__(kTile, 3) void k_rhs_tile(RhsArgs a, TileArgs t, in
     __syncthreads();
   }
   if (!own) return;
-  elem_core<false, N>(a, p, ix, src, in);
+  elem_core<false, N>(a, p, ix, src, load_elem_in<N>(a, p, ix, src));
 }
 
 // k_flux of a single solve, with the _NOAH_ branches when the context has them.
