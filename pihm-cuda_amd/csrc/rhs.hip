@@ -1032,11 +1032,12 @@ __device__ __forceinline__ double tile_sfmag(const RhsArgs &a, int s, long q, co
 // kernel stays within 128 VGPRs, four wavefronts per SIMD, so that a
 // 200,000-triangle RHS (3,125 tiles and 256 river blocks) runs in one round
 // of 4,096 wavefront slots instead of spilling 309 blocks into a second round
-// of 3,072.  The halo's records are evaluated again by every tile that borders
+// of 3,072.  With the _NOAH_ branches the cap is three wavefronts (140 VGPRs):
+// four would spill in the element path.  The halo's records are evaluated again by every tile that borders
 // them, from the same operands: every output keeps one writer, and the
 // arithmetic is that of k_prep + k_flux, bit for bit.
 template <bool N>
-__global__ __launch_bounds__(kTile, 4) void k_rhs_tile(RhsArgs a, TileArgs t, int nbe) {
+__global__ __launch_bounds__(kTile, N ? 3 : 4) void k_rhs_tile(RhsArgs a, TileArgs t, int nbe) {
   extern __shared__ double tile_lds[];
   const LdsRecs srec{tile_lds, t.s1cap};
   double *shead = tile_lds + 8 * t.s1cap;
