@@ -33,9 +33,9 @@ BYTES_PER_SEG = 200      # ... per river segment
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (KiB per dispatch),
 # each scaled by what the counter reads for the calibration copy in the same run:
 # FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288 KiB copied), WRITE_SIZE x 1.000.
-# One launch (k_rhs_tile): 37,698.5 x 2 + 4,825.7 KiB.  Two kernels (k_prep + k_flux):
+# One launch (k_rhs_tile): 40,823.6 x 2 + 5,040.0 KiB.  Two kernels (k_prep + k_flux):
 # (14,710.5 + 28,244.1) x 2 + (12,919.4 + 4,818.1) KiB.
-RHS_TRAFFIC_BYTES = {"one launch": (37698.5 * 2.0 + 4825.7) * 1024,
+RHS_TRAFFIC_BYTES = {"one launch": (40823.6 * 2.0 + 5040.0) * 1024,
                      "two kernels": ((14710.5 + 28244.1) * 2.0 + (12919.4 + 4818.1)) * 1024}
 RHS_KERNELS = {"one launch": "one RHS evaluation (k_rhs_tile: one launch, tile records in LDS)",
                "two kernels": "one RHS evaluation (k_prep + k_flux, back to back)"}
@@ -344,8 +344,9 @@ def producers_run(mesh, seed, iters=200):
 
 NOAH_EXTRA_BYTES_PER_ELEM = 32   # smcmin, sh2o[0], fcr, gwet read by the _NOAH_ branches
 # HBM bytes per launch at the 100,000-triangle mesh from the FETCH_SIZE and WRITE_SIZE passes
-# (FETCH_SIZE x 2, calibrated on the copy kernel): one Noah RHS, one soil-water step
-NOAH_RHS_TRAFFIC_BYTES = 56153190.0
+# (FETCH_SIZE x 2, calibrated on the copy kernel): one Noah RHS on the one-launch path
+# (k_rhs_tile with the _NOAH_ branches: 19,893.9 x 2 + 2,413.1 KiB), one soil-water step
+NOAH_RHS_TRAFFIC_BYTES = (19893.9 * 2.0 + 2413.1) * 1024
 NOAH_SOIL_TRAFFIC_BYTES = 122959155.0
 # algorithmic bytes of the soil-water step per element: reads 4 layer fields (sldpth, smc, sh2o, et;
 # 352 B), 12 FP64 scalars and 2 int32 (104 B); writes 5 layer fields (satdpth, runoff2_lyr, smflxv,
@@ -402,7 +403,7 @@ def noah_rhs_run(seed, iters=200, coupled=True):
     ach = bytes_noah / (ms["noah"] * 1e-3) / 1e9
     out = {"workload": f"c2 mesh: {ne} triangles, {m.nriver} river segments, NEQ {m.neq}",
            "ms_per_eval": ms["noah"], "plain_ms_per_eval": ms["plain"],
-           "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_prep + k_flux<noah>)",
+           "roofline": {"kernel": "one RHS evaluation with the _NOAH_ branches (k_rhs_tile<noah>: one launch)",
                         "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": bytes_noah,
                         "traffic": NOAH_RHS_TRAFFIC_BYTES},
