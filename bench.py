@@ -98,6 +98,9 @@ def gpu_run(mesh, forcing, hours_total, mark_hours, snap_hour=None):
 
     def step_end(user, t, yv):
         k = int(round(t / 60.0))
+        if hours_total >= 24 and k % 1440 == 0:       # a line per simulated day of a long run
+            _log(f"{mesh.nelem}-triangle run: day {k // 1440} of {hours_total / 24:g} at "
+                 f"{time.perf_counter() - marks[0.0]:.1f} s")
         if k == snap_k:
             L.pihm_ctx_sync(ctx)
             L.pihm_ctx_state_from_device(ctx, y, snap.ctypes.data_as(_lib.c_double_p))
