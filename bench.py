@@ -585,7 +585,7 @@ def physical_cores():
 # 200,000-triangle workload (SQ_INSTS_VALU / SQ_WAVES, the same profile passes as the traffic).
 # A wavefront's FP64 instruction occupies a SIMD for 4 cycles (16 lanes per cycle); the chip
 # has 1,024 SIMDs at 2.4 GHz.
-RHS_VALU = {"one launch": [(1911.3, 3381)], "two kernels": [(268.1, 3320), (1665.4, 3192)]}
+RHS_VALU = {"one launch": [(1929.2, 3381)], "two kernels": [(268.2, 3320), (1669.2, 3192)]}
 SIMDS, CLOCK_MHZ = 1024, 2400.0
 
 
