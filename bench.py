@@ -156,7 +156,16 @@ def ensemble_run(nmem, hours, sample, seed):
     tuning = e.autotune(10)          # launch shapes, before the timed run
     ops0 = e.op_stats()
     t0 = time.perf_counter()
-    nfail = e.run(hours * 3600.0)
+    if hours <= 2.0:
+        nfail = e.run(hours * 3600.0)
+    else:
+        # simulated hour by simulated hour, a progress line after each (the members'
+        # model-step loops pause at each boundary and resume; results do not change)
+        nfail, h = 0, 0.0
+        while h < hours:
+            h = min(h + 1.0, hours)
+            nfail += e.run(h * 3600.0)
+            _log(f"ensemble: hour {h:g} of {hours:g} at {time.perf_counter() - t0:.1f} s, {nfail} failed")
     wall = time.perf_counter() - t0
     st = [e.stats(k) for k in range(nmem)]
     cnt = e.counters()
@@ -167,8 +176,9 @@ def ensemble_run(nmem, hours, sample, seed):
     order = np.argsort([m.nelem for m in meshes])
     pick = order[np.linspace(0, nmem - 1, min(sample, nmem)).round().astype(int)]
     single_wall = []
-    for k in pick:
+    for n, k in enumerate(pick):
         m = meshes[k]
+        _log(f"ensemble: one-at-a-time member {n + 1} of {len(pick)} ({m.nelem} triangles)")
         desc = m.desc()
         ctx = C.c_void_p()
         assert L.pihm_ctx_create(C.byref(desc), C.byref(ctx)) == 0

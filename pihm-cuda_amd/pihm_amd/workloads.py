@@ -59,6 +59,13 @@ def start_workers():
     """Start the server process that worker processes are forked from.  Called before the
     first HIP call, it keeps every worker free of the GPU runtime's state and threads."""
     from multiprocessing import forkserver
+    # The server is a new interpreter that does not inherit this process's sys.path;
+    # PYTHONPATH lets it import the module that resets foreign signal handlers.
+    pkg_root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    paths = [p for p in os.environ.get("PYTHONPATH", "").split(os.pathsep) if p]
+    if pkg_root not in paths:
+        os.environ["PYTHONPATH"] = os.pathsep.join([pkg_root] + paths)
+    forkserver.set_forkserver_preload(["pihm_amd._fs_init"])
     forkserver.ensure_running()
 
 
