@@ -161,3 +161,29 @@ def test_ensemble_resumes_across_runs():
     _, yb, _, _ = _run_ens(ms[:2], fs[:2], y0[:2], hours=2)
     for k in range(2):
         assert np.array_equal(ya[k], yb[k])
+
+
+def test_ensemble_op_stats():
+    """The operation kernel's launches are timed with events and priced at their
+    algorithmic bytes (pihm_ens_op_stats): counts grow with each run, and every timed
+    launch moves at least one vector of the smallest member and at most every member's
+    whole queue (ops of at most 14 vectors, eight launches a tick)."""
+    ms, fs, y0 = _members()
+    cfg = run_config(HOURS * 3600.0)
+    e = Ensemble(ms, fs, cfg)
+    try:
+        for k, y in enumerate(y0):
+            e.set_state(k, y)
+        s0 = e.op_stats()
+        assert e.run(3600.0) == 0
+        s1 = e.op_stats()
+        assert e.run(HOURS * 3600.0) == 0
+        s2 = e.op_stats()
+        cnt = e.counters()
+    finally:
+        e.close()
+    assert s0["launches"] < s1["launches"] < s2["launches"] <= cnt["op_launches"]
+    assert s0["ms"] < s1["ms"] < s2["ms"] and s0["bytes"] < s1["bytes"] < s2["bytes"]
+    per = s2["bytes"] / s2["launches"]
+    neq = [m.neq for m in ms]
+    assert 8.0 * min(neq) <= per <= 8.0 * sum(neq) * 14 * 64
