@@ -101,3 +101,33 @@ def test_host_replays_device_sequence(c0):
     assert np.all(np.abs(S_c - S_g)[live] <= 1e-8 * np.abs(S_g)[live])
     o_g, o_c = _outlet(m, y_g), _outlet(m, y_c)
     assert np.all(np.abs(o_c - o_g) <= 1e-8 * np.abs(o_g))
+
+
+def test_device_divergence_over_an_hour_matches_the_reduction_order_spread(c0):
+    """Over one hour the device replay of the host record leaves 1e-8 behind, as two
+    host runs that differ only in their reduction order do (tests/test_replay.py): the
+    device's storage and outlet divergence stays within 30 times the host pair's."""
+    m = c0
+    f = StormForcing(24, seed=5, nelem=m.nelem, p_wet=0.5)
+    y0 = m.initial_state()
+    rec = StepRecord(500000)
+    rc_c, y_c, _ = O.OracleRun(m, f).run(y0, rec.attach(run_config(3600.0)))
+    assert rc_c == 0
+    h, q, it = rec.steps()
+    rc_8, y_8, _ = O.OracleRun(m, f, nthreads=8).run(y0, set_replay(StepRecord(500000).attach(run_config(3600.0)),
+                                                                    h, q, it))
+    assert rc_8 == 0
+    print(f"[replay] 535-element storm, 1 h: {len(h)} host steps, device replay", flush=True)
+    rc_g, y_g, st_g = run_gpu(m, f, set_replay(StepRecord(500000).attach(run_config(3600.0)), h, q, it), y0)
+    assert rc_g == 0, st_g.as_dict()
+
+    def div(y):
+        S, Sc = _stores(m, y), _stores(m, y_c)
+        live = np.abs(Sc) > 0
+        o, oc = _outlet(m, y), _outlet(m, y_c)
+        return np.max(np.abs(S - Sc)[live] / np.abs(Sc)[live]), np.max(np.abs(o - oc) / np.abs(oc))
+
+    d_host, d_dev = div(y_8), div(y_g)
+    print(f"  host pair {d_host}  device {d_dev}", flush=True)
+    assert d_dev[0] <= 30.0 * max(d_host[0], 1e-8)
+    assert d_dev[1] <= 30.0 * max(d_host[1], 1e-8)
