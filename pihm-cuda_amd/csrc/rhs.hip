@@ -245,7 +245,7 @@ struct GlobalSrc {
   __device__ __forceinline__ NbrRec self() const { return a.e.rec[p]; }
   __device__ __forceinline__ int code(int j) const { return a.e.nb[j * a.ne + p]; }
   __device__ __forceinline__ NbrRec nbr(int c) const { return a.e.rec[c]; }
-  __device__ __forceinline__ void bank(const Ix &, int slot, const NbrRec &, double, double, double &ovl,
+  __device__ __forceinline__ void bank(const Ix &, int, int slot, const NbrRec &, double, double, double &ovl,
                                        double &sub) const {
     ovl = a.r.hid_ovl[2 * a.nr + slot];
     sub = a.r.bank_sub[slot];
@@ -271,17 +271,32 @@ struct LdsRecs {
   __device__ __forceinline__ void put_sfmag(int s, double v) const { f[5 * cap + s] = v; }
 };
 
+constexpr int kTile = 64;             // elements per tile = threads per block of the one-launch RHS
+
 struct TileSrc {
   static constexpr bool kGatherFirst = false;  // from LDS: each edge's record when its turn comes
   const RhsArgs &a;
   LdsRecs srec;         // LDS records of the tile and its halo
   const int32_t *lnb;   // neighbour slots of the tile's elements
   int p, i;
+  const double *bk;     // LDS: the bank fluxes of each tile element's first river edge (ovl, then sub)
+  const int *bkj;       // LDS: that edge (-1: none)
   __device__ __forceinline__ NbrRec self() const { return srec.get(i); }
   __device__ __forceinline__ int code(int j) const { return lnb[j * a.ne + p]; }
   __device__ __forceinline__ NbrRec nbr(int c) const { return srec.get(c); }
-  __device__ __forceinline__ void bank(const Ix &ix, int slot, const NbrRec &me, double zmax, double zmin,
-                                       double &ovl, double &sub) const {
+  // edge j's bank fluxes: the first river edge's were evaluated during the
+  // staging, from the same operands; a further river edge's are evaluated here
+  __device__ __forceinline__ void bank(const Ix &ix, int j, int slot, const NbrRec &me, double zmax,
+                                       double zmin, double &ovl, double &sub) const {
+    if (bkj[i] == j) {
+      ovl = bk[i];
+      sub = bk[kTile + i];
+      return;
+    }
+    bank_eval(ix, slot, me, zmax, zmin, ovl, sub);
+  }
+  __device__ __forceinline__ void bank_eval(const Ix &ix, int slot, const NbrRec &me, double zmax, double zmin,
+                                            double &ovl, double &sub) const {
     const RiverDev &R = a.r;
     const int r = slot >> 1, side = slot & 1;
     const double *ys = ix.y + 3L * ix.ne - ix.roff;
@@ -439,7 +454,7 @@ __device__ __forceinline__ LatOut elem_lateral(const RhsArgs &a, int p, const Ix
     for (int j = 0; j < 3; j++) {
       if (nb_is_river(in.c[j])) {
         const int slot = -in.c[j] - 1;
-        src.bank(ix, slot, me, zmax, zmin, ovl[j], sub[j]);
+        src.bank(ix, j, slot, me, zmax, zmin, ovl[j], sub[j]);
         a.r.hid_ovl[slot] = ovl[j];
       }
     }
@@ -896,7 +911,6 @@ __global__ __launch_bounds__(kBlock, 3) void k_flux(RhsArgs a, int nbe) {
 //           uses for itself.
 // The parts meet in LDS (xr: 9 doubles per segment); part 0 then writes the
 // river ODEs (src/ode.c:238-251) in river_ode's order.
-constexpr int kTile = 64;             // elements per tile = threads per block of the one-launch RHS
 constexpr int kRivSeg = kTile / 4;
 
 __device__ __forceinline__ void river_parts(const RhsArgs &a, int rb, double *xr) {
@@ -1041,6 +1055,8 @@ __global__ __launch_bounds__(kTile, N ? 3 : 4) void k_rhs_tile(RhsArgs a, TileAr
   extern __shared__ double tile_lds[];
   const LdsRecs srec{tile_lds, t.s1cap};
   double *shead = tile_lds + 8 * t.s1cap;
+  double *sbank = shead + t.scap;                         // [2 * kTile]: ovl, then sub
+  int *sbankj = reinterpret_cast<int *>(sbank + 2 * kTile);   // [kTile]
   const int nrb = (int)gridDim.x - nbe;
   if ((int)blockIdx.x < nrb) {
     river_parts(a, (int)blockIdx.x, tile_lds);
@@ -1054,7 +1070,7 @@ __global__ __launch_bounds__(kTile, N ? 3 : 4) void k_rhs_tile(RhsArgs a, TileAr
   const ElemDev &e = a.e;
   const double *y = a.y;
   const Ix ix = ix_elem<false>(a, 0);
-  const TileSrc src{a, srec, t.lnb, p, i};
+  const TileSrc src{a, srec, t.lnb, p, i, sbank, sbankj};
 
   // own element: what its record and FrictSlope gradient need (the flux
   // inputs are loaded after the barriers, so that they do not hold registers
@@ -1069,10 +1085,11 @@ __global__ __launch_bounds__(kTile, N ? 3 : 4) void k_rhs_tile(RhsArgs a, TileAr
     ys = y[p];
     yg = y[2 * ne + p];
     kmach = e.kmach[p]; areafv = e.areafv[p]; ksath = e.ksath[p]; rough = e.rough[p];
+#pragma unroll
+    for (int j = 0; j < 3; j++) oc[j] = src.code(j);
     if (diff) {
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        oc[j] = src.code(j);
         nx[j] = e.nabr_x[j * ne + p];
         ny[j] = e.nabr_y[j * ne + p];
       }
@@ -1105,11 +1122,21 @@ __global__ __launch_bounds__(kTile, N ? 3 : 4) void k_rhs_tile(RhsArgs a, TileAr
     }
   }
 
-  // records and heads
+  // records and heads; the bank fluxes of the element's first river edge,
+  // whose gathers (segment, then the other bank) then overlap the halo's
   if (own) {
     const NbrRec r = tile_rec(ys, yg, zmax, zmin, depth, dmac, kmach, areafv, ksath, rough);
     srec.put(i, r);
     shead[i] = r.head;
+    int bj = -1;
+#pragma unroll
+    for (int j = 2; j >= 0; j--)
+      if (nb_is_river(oc[j])) bj = j;
+    double bo = 0.0, bs = 0.0;
+    if (bj >= 0) src.bank_eval(ix, -oc[bj] - 1, r, zmax, zmin, bo, bs);
+    sbank[i] = bo;
+    sbank[kTile + i] = bs;
+    sbankj[i] = bj;
   }
   if (hv) {
     if (h1) {
@@ -1700,7 +1727,8 @@ int build_tiles(pihm_ctx *c) {
     nhmax = std::max(nhmax, (int)(r1.size() + r2.size()));
   }
   const int s1cap = kTile + n1max, scap = kTile + nhmax;
-  const size_t lds = (8 * (size_t)s1cap + (size_t)scap) * sizeof(double);   // 8 record fields, heads
+  // 8 record fields, heads, bank fluxes of the first river edge (two fields), that edge
+  const size_t lds = (8 * (size_t)s1cap + (size_t)scap + 2 * (size_t)kTile) * sizeof(double) + kTile * sizeof(int);
   if (lds > kTileLdsMax) return PIHM_OK;       // the two-kernel RHS stays
   const size_t nh = hidx.size();
   const size_t words = (ntile + 1) + ntile + nh + 3 * nh + 3 * ne;
