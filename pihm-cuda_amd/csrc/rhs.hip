@@ -271,7 +271,7 @@ struct LdsRecs {
   __device__ __forceinline__ void put_sfmag(int s, double v) const { f[5 * cap + s] = v; }
 };
 
-constexpr int kTile = 64;             // elements per tile = threads per block of the one-launch RHS
+constexpr int kTile = 128;            // elements per tile = threads per block of the one-launch RHS (two wavefronts)
 
 struct TileSrc {
   static constexpr bool kGatherFirst = false;  // from LDS: each edge's record when its turn comes
@@ -1033,21 +1033,24 @@ __device__ __forceinline__ double tile_sfmag(const RhsArgs &a, int s, long q, co
 
 // One RHS call of a single solve in one launch.  Grid: river blocks
 // (river_parts, kRivSeg segments each), then one block per tile of kTile
-// consecutive elements (Hilbert order), each block one wavefront, so that a
-// block's barriers cost little and blocks run independently, hiding each
-// other's memory latency.  Each element block stages in LDS what its
-// elements' fluxes read: the heads of the tile and of its halo (the first
-// ring of outside neighbours and their outside neighbours), and the records
-// of the tile and the first ring, whose FrictSlope gradients need those heads;
-// then it runs the element fluxes from them (elem_core with TileSrc).  The
-// loads for the staging (its element's and its halo entry's record inputs)
-// are issued at the start, the element's flux inputs after the barriers, and
-// each edge reads its neighbour's record from LDS when its turn comes: the
-// kernel stays within 128 VGPRs, four wavefronts per SIMD, so that a
-// 200,000-triangle RHS (3,125 tiles and 256 river blocks) runs in one round
-// of 4,096 wavefront slots instead of spilling 309 blocks into a second round
-// of 3,072.  With the _NOAH_ branches the cap is three wavefronts (140 VGPRs):
-// four would spill in the element path.  The halo's records are evaluated again by every tile that borders
+// consecutive elements (Hilbert order), two wavefronts per block: blocks run
+// independently, hiding each other's memory latency, and a tile of 128 has a
+// smaller halo per element than one of 64 (at the 200,000- and
+// 100,000-triangle meshes 128 measured 2% and 7% faster than 64; 192 and 256
+// were no better).  Each element block stages in LDS what its elements'
+// fluxes read: the heads of the tile and of its halo (the first ring of
+// outside neighbours and their outside neighbours), and the records of the
+// tile and the first ring, whose FrictSlope gradients need those heads; then
+// it runs the element fluxes from them (elem_core with TileSrc).  The loads
+// for the staging (its element's and its halo entry's record inputs, and the
+// bank fluxes of its first river edge) are issued at the start, the element's
+// flux inputs after the barriers, and each edge reads its neighbour's record
+// from LDS when its turn comes: the kernel stays within 128 VGPRs, four
+// wavefronts per SIMD, so that a 200,000-triangle RHS (3,381 wavefronts of
+// element and river blocks) runs in one round of 4,096 wavefront slots
+// instead of spilling into a second round of 3,072.  With the _NOAH_ branches
+// the cap is three wavefronts (140 VGPRs): four would spill in the element
+// path.  The halo's records are evaluated again by every tile that borders
 // them, from the same operands: every output keeps one writer, and the
 // arithmetic is that of k_prep + k_flux, bit for bit.
 template <bool N>
