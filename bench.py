@@ -33,10 +33,10 @@ BYTES_PER_SEG = 200      # ... per river segment
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --profile-kernels 20` (KiB per dispatch),
 # each scaled by what the counter reads for the calibration copy in the same run:
 # FETCH_SIZE x 2.000 (262,157 KiB read back for 524,288 KiB copied), WRITE_SIZE x 1.000.
-# One launch (k_rhs_tile): 41,456.2 x 2 + 5,034.5 KiB.  Two kernels (k_prep + k_flux):
-# (14,710.5 + 28,244.1) x 2 + (12,919.4 + 4,818.1) KiB.
-RHS_TRAFFIC_BYTES = {"one launch": (41456.2 * 2.0 + 5034.5) * 1024,
-                     "two kernels": ((14710.5 + 28244.1) * 2.0 + (12919.4 + 4818.1)) * 1024}
+# One launch (k_rhs_tile): 40,867.1 x 2 + 5,038.1 KiB.  Two kernels (k_prep + k_flux):
+# (14,789.9 + 28,239.8) x 2 + (12,919.4 + 4,818.1) KiB.
+RHS_TRAFFIC_BYTES = {"one launch": (40867.1 * 2.0 + 5038.1) * 1024,
+                     "two kernels": ((14789.9 + 28239.8) * 2.0 + (12919.4 + 4818.1)) * 1024}
 RHS_KERNELS = {"one launch": "one RHS evaluation (k_rhs_tile: one launch, tile records in LDS)",
                "two kernels": "one RHS evaluation (k_prep + k_flux, back to back)"}
 # Land-surface step (k_intcp_snow_et), algorithmic bytes per element: 15 FP64 parameters
@@ -358,8 +358,8 @@ def producers_run(mesh, seed, iters=200):
 NOAH_EXTRA_BYTES_PER_ELEM = 32   # smcmin, sh2o[0], fcr, gwet read by the _NOAH_ branches
 # HBM bytes per launch at the 100,000-triangle mesh from the FETCH_SIZE and WRITE_SIZE passes
 # (FETCH_SIZE x 2, calibrated on the copy kernel): one Noah RHS on the one-launch path
-# (k_rhs_tile with the _NOAH_ branches: 19,810.1 x 2 + 2,413.1 KiB), one soil-water step
-NOAH_RHS_TRAFFIC_BYTES = (19810.1 * 2.0 + 2413.1) * 1024
+# (k_rhs_tile with the _NOAH_ branches: 19,646.4 x 2 + 2,411.0 KiB), one soil-water step
+NOAH_RHS_TRAFFIC_BYTES = (19646.4 * 2.0 + 2411.0) * 1024
 NOAH_SOIL_TRAFFIC_BYTES = 122959155.0
 # algorithmic bytes of the soil-water step per element: reads 4 layer fields (sldpth, smc, sh2o, et;
 # 352 B), 12 FP64 scalars and 2 int32 (104 B); writes 5 layer fields (satdpth, runoff2_lyr, smflxv,
@@ -595,7 +595,7 @@ def physical_cores():
 # 200,000-triangle workload (SQ_INSTS_VALU / SQ_WAVES, the same profile passes as the traffic).
 # A wavefront's FP64 instruction occupies a SIMD for 4 cycles (16 lanes per cycle); the chip
 # has 1,024 SIMDs at 2.4 GHz.
-RHS_VALU = {"one launch": [(1926.4, 3381)], "two kernels": [(268.2, 3320), (1669.2, 3192)]}
+RHS_VALU = {"one launch": [(1768.7, 3382)], "two kernels": [(268.2, 3320), (1669.2, 3192)]}
 SIMDS, CLOCK_MHZ = 1024, 2400.0
 
 
