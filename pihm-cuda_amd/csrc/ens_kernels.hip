@@ -588,8 +588,7 @@ __global__ void k_ens_signal(unsigned *ctr, volatile unsigned *flag) {
     __threadfence_system();
     const unsigned s = *ctr + 1u;
     *ctr = s;
-    *flag = s;
-    __threadfence_system();
+    *flag = s;   // made visible by the end of the kernel
   }
 }
 

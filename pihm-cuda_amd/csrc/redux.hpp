@@ -126,8 +126,7 @@ __global__ __launch_bounds__(kRedThreads) void k_partials_final(int nb, int nv, 
     __threadfence_system();
     const unsigned s = *ctr + 1u;
     *ctr = s;
-    *flag = s;
-    __threadfence_system();
+    *flag = s;   // made visible by the end of the kernel
   }
 }
 }  // namespace
