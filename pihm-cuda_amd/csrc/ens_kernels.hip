@@ -552,7 +552,9 @@ __global__ __launch_bounds__(kChunkThreads) void k_ens_ops(Dev d, Round r) {
   }
 }
 
-// Member totals: the chunk partials of member rmem[b] in chunk order.
+// Member totals: the chunk partials of member rmem[b] in chunk order.  Each
+// thread loads its first partial of every sum before combining any (one
+// memory latency, not one per sum); further rounds of chunks follow in order.
 __global__ __launch_bounds__(kChunkThreads) void k_ens_final(Dev d, Round r) {
   __shared__ double lds[4];
   const int m = r.rmem[blockIdx.x];
@@ -560,10 +562,18 @@ __global__ __launch_bounds__(kChunkThreads) void k_ens_final(Dev d, Round r) {
   const int rslot = r.rinfo[3 * blockIdx.x + 2];
   const MemberInfo M = d.mem[m];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < nsum; k++) {
+  const int c0 = threadIdx.x;
+  double first[kSums];
+#pragma unroll
+  for (int k = 0; k < kSums; k++)
+    first[k] = (k < nsum && c0 < M.nch) ? d.partials[(long)(M.cbase + c0) * kSums + k] : 0.0;
+#pragma unroll
+  for (int k = 0; k < kSums; k++) {
+    if (k >= nsum) break;
     const bool mn = minfirst && k == 0;
     double v = mn ? DBL_MAX : 0.0;
-    for (int cc = threadIdx.x; cc < M.nch; cc += kChunkThreads) {
+    if (c0 < M.nch) v = mn ? fmin(v, first[k]) : v + first[k];
+    for (int cc = c0 + kChunkThreads; cc < M.nch; cc += kChunkThreads) {
       const double p = d.partials[(long)(M.cbase + cc) * kSums + k];
       v = mn ? fmin(v, p) : v + p;
     }
