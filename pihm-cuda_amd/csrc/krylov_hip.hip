@@ -21,18 +21,25 @@ inline hipStream_t S(N_Vector v) { return (hipStream_t)((N_VectorContent_Hip)v->
 
 constexpr int kMaxBasis = 11;   // maxl + 1 basis vectors (maxl <= 10)
 
+// The fused functors come in two parts, load(i) and apply(i, loaded, acc),
+// so that k_fused_sum can issue two entries' loads before either entry's
+// stores (a store ahead of a load through another pointer keeps the load
+// from being hoisted above it).
 struct FStart {
   double rl1, gamma;
   const double *zn1, *acor, *f, *s;
   double *b, *v0;
-  __device__ __forceinline__ void operator()(long i, double (&acc)[1]) const {
-    const double t = (rl1 * zn1[i]) + acor[i];      // LinearSum(rl1, zn1, 1, acor)
-    const double bi = (gamma * f[i]) - t;           // LinearSum(gamma, f, -1, t)
+  struct In { double zn1, acor, f, s; };
+  __device__ __forceinline__ In load(long i) const { return In{zn1[i], acor[i], f[i], s[i]}; }
+  __device__ __forceinline__ void apply(long i, const In &a, double (&acc)[1]) const {
+    const double t = (rl1 * a.zn1) + a.acor;        // LinearSum(rl1, zn1, 1, acor)
+    const double bi = (gamma * a.f) - t;            // LinearSum(gamma, f, -1, t)
     b[i] = bi;
-    const double v = s[i] * bi;                     // Prod(s1, vtemp)
+    const double v = a.s * bi;                      // Prod(s1, vtemp)
     v0[i] = v;
     acc[0] += v * v;
   }
+  __device__ __forceinline__ void operator()(long i, double (&acc)[1]) const { apply(i, load(i), acc); }
 };
 
 struct PrepArgs {
@@ -93,18 +100,29 @@ struct FArnoldi {
   const double *gp;                                 // gamma from device memory when set
   double *w;
   const double *vk[kMaxBasis];
-  __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const {
-    const double g = gp ? *gp : gamma;
-    const double si = s[i];
-    const double vt = zp ? zp[i] : vl[i] / si;      // the vector atimes is applied to
-    const double jv = siginv * (jf[i] - fy[i]);     // LinearSum(siginv, Jv, -siginv, fy)
-    const double z = (-g * jv) + vt;                // LinearSum(1, v, -gamma, Jv)
-    const double wi = si * z;                       // Prod(s1, z)
+  struct In { double g, si, v, jf, fy, vk[NV - 1]; };
+  __device__ __forceinline__ In load(long i) const {
+    In a;
+    a.g = gp ? *gp : gamma;
+    a.si = s[i];
+    a.v = zp ? zp[i] : vl[i];
+    a.jf = jf[i];
+    a.fy = fy[i];
+#pragma unroll
+    for (int k = 0; k < NV - 1; k++) a.vk[k] = vk[k][i];
+    return a;
+  }
+  __device__ __forceinline__ void apply(long i, const In &a, double (&acc)[NV]) const {
+    const double vt = zp ? a.v : a.v / a.si;        // the vector atimes is applied to
+    const double jv = siginv * (a.jf - a.fy);       // LinearSum(siginv, Jv, -siginv, fy)
+    const double z = (-a.g * jv) + vt;              // LinearSum(1, v, -gamma, Jv)
+    const double wi = a.si * z;                     // Prod(s1, z)
     w[i] = wi;
 #pragma unroll
-    for (int k = 0; k < NV - 1; k++) acc[k] += vk[k][i] * wi;
+    for (int k = 0; k < NV - 1; k++) acc[k] += a.vk[k] * wi;
     acc[NV - 1] += wi * wi;
   }
+  __device__ __forceinline__ void operator()(long i, double (&acc)[NV]) const { apply(i, load(i), acc); }
 };
 
 template <int NV>
@@ -129,28 +147,45 @@ struct FFinish {
   const double *vk[kMaxBasis];
   const double *s, *b, *zn0, *xsrc;
   double *acor, *y;
-  __device__ __forceinline__ void operator()(long i, double (&acc)[2]) const {
+  // basis entries loaded ahead: those of the default SPGMR dimension (5);
+  // a larger dimension reads the rest where they are used
+  static constexpr int kPre = 6;
+  struct In { double xin, s, acor, zn0, vk[kPre]; };
+  __device__ __forceinline__ In load(long i) const {
+    In a;
+    a.xin = (mode == 3) ? xsrc[i] : (mode == 1 ? b[i] : 0.0);
+#pragma unroll
+    for (int j = 0; j < kPre; j++) a.vk[j] = (mode == 0 && j < k) ? vk[j][i] : 0.0;
+    a.s = s[i];
+    a.acor = acor[i];
+    a.zn0 = zn0[i];
+    return a;
+  }
+  __device__ __forceinline__ void apply(long i, const In &a, double (&acc)[2]) const {
     double x;
     if (mode == 3) {
-      x = 0.0 + xsrc[i];                                     // x = 0 + P^-1 (xcor / s2)
+      x = 0.0 + a.xin;                                       // x = 0 + P^-1 (xcor / s2)
     } else if (mode == 0) {
       double xc = 0.0;
-      for (int j = 0; j < k; j++) xc += yc[j] * vk[j][i];   // LinearSum(yg, V, 1, xcor, xcor)
-      xc = xc / s[i];                                        // Div(xcor, s2)
+#pragma unroll
+      for (int j = 0; j < kMaxBasis; j++)
+        if (j < k) xc += yc[j] * (j < kPre ? a.vk[j] : vk[j][i]);   // LinearSum(yg, V, 1, xcor, xcor)
+      xc = xc / a.s;                                         // Div(xcor, s2)
       x = 0.0 + xc;                                          // x = 0 + vtemp
     } else if (mode == 1) {
-      x = b[i];
+      x = a.xin;
     } else {
       x = 0.0;
     }
-    const double si = s[i];
-    const double ac = acor[i] + x;                           // LinearSum(1, acor, 1, b, acor)
+    const double si = a.s;
+    const double ac = a.acor + x;                            // LinearSum(1, acor, 1, b, acor)
     acor[i] = ac;
-    y[i] = zn0[i] + ac;                                      // LinearSum(1, zn0, 1, acor, y)
+    y[i] = a.zn0 + ac;                                       // LinearSum(1, zn0, 1, acor, y)
     const double p = x * si, q = ac * si;
     acc[0] += p * p;
     acc[1] += q * q;
   }
+  __device__ __forceinline__ void operator()(long i, double (&acc)[2]) const { apply(i, load(i), acc); }
 };
 
 // xcor = sum_j y_j V_j (LinearSum(y_j, V_j, 1, xcor, xcor) from xcor = 0), then

@@ -39,7 +39,20 @@ __global__ __launch_bounds__(kRedThreads) void k_fused_sum(long n, F f, double *
 #pragma unroll
   for (int k = 0; k < NV; k++) acc[k] = 0.0;
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) f(i, acc);
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (requires(const F &g, long j) { g.load(j); }) {
+    // two entries' loads before either entry's stores; the sums keep the
+    // order of the entries
+    for (; i + stride < n; i += 2 * stride) {
+      const auto a = f.load(i);
+      const auto b = f.load(i + stride);
+      f.apply(i, a, acc);
+      f.apply(i + stride, b, acc);
+    }
+    if (i < n) f.apply(i, f.load(i), acc);
+  } else {
+    for (; i < n; i += stride) f(i, acc);
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
