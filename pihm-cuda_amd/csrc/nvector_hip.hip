@@ -226,7 +226,14 @@ __global__ void k_rescale(long n, int q, ZnPtrs z, double f1, double f2, double 
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double f[6] = {1.0, f1, f2, f3, f4, f5};
-  for (int j = 1; j <= q; j++) z.p[j][i] = f[j] * z.p[j][i];
+  // every entry loaded before any is stored (a store would keep the next
+  // array's load behind it)
+  double v[6];
+#pragma unroll
+  for (int j = 1; j < 6; j++) v[j] = (j <= q) ? z.p[j][i] : 0.0;
+#pragma unroll
+  for (int j = 1; j < 6; j++)
+    if (j <= q) z.p[j][i] = f[j] * v[j];
 }
 
 __global__ void k_correct(long n, int q, ZnPtrs z, const double *acor, double l0, double l1, double l2,
@@ -235,10 +242,15 @@ __global__ void k_correct(long n, int q, ZnPtrs z, const double *acor, double l0
   if (i >= n) return;
   const double l[6] = {l0, l1, l2, l3, l4, l5};
   const double a = acor[i];
-  for (int j = 0; j <= q; j++) {
+  double v[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) v[j] = (j <= q) ? z.p[j][i] : 0.0;   // all loads before any store
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    if (j > q) break;
     // LinearSum(l[j], acor, 1, zn[j], zn[j]): axpy y += a*x (or += x, -= x)
     const double lj = l[j];
-    double zj = z.p[j][i];
+    double zj = v[j];
     if (lj == 1.0) zj += a;
     else if (lj == -1.0) zj -= a;
     else zj += lj * a;
@@ -251,15 +263,22 @@ __global__ __launch_bounds__(kRedBlock) void k_ewt(long n, const double *y, doub
   __shared__ double lds[4];
   const long stride = (long)gridDim.x * blockDim.x;
   double mn = DBL_MAX, sq = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    // cvEwtSetSS: tempv = |y|; tempv = rtol*tempv; tempv += atol; ewt = 1/tempv
-    const double t = (rtol * fabs(y[i])) + atol;
+  // cvEwtSetSS: tempv = |y|; tempv = rtol*tempv; tempv += atol; ewt = 1/tempv
+  const auto one = [&](long i, double yi) {
+    const double t = (rtol * fabs(yi)) + atol;
     const double w = 1.0 / t;
     ewt[i] = w;
     mn = fmin(mn, t);
-    const double p = y[i] * w;
+    const double p = yi * w;
     sq += p * p;
+  };
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {   // two entries' loads before either store
+    const double y0 = y[i], y1 = y[i + stride];
+    one(i, y0);
+    one(i + stride, y1);
   }
+  if (i < n) one(i, y[i]);
   const double r0 = block_reduce(R_MIN, mn, lds);
   const double r1 = block_reduce(R_DOT, sq, lds);
   if (threadIdx.x == 0) {
